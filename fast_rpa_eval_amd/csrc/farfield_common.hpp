@@ -1,0 +1,89 @@
+// Far-field Hartree plan and the compile-time Cartesian term table.
+#pragma once
+
+#include <algorithm>
+#include <utility>
+
+#include "common.hpp"
+#include "far_terms.inc"  // generated: FAR_TAB_LMAX, FAR_NCC[], FAR_TERMS[] (l, m, i, j, k, nn)
+
+namespace frpa {
+
+constexpr int FAR_LMAX = 10;  // largest l_pot_max the kernels are instantiated for
+static_assert(FAR_TAB_LMAX >= FAR_LMAX, "generated term table too short");
+
+// Per hartree centre ic, everything the kernels need (read with scalar loads: uniform per wave).
+struct CenterRec {
+  double dx, dy, dz;  // coords_center(:, centers_hartree_potential(ic) - 1): distance and l reduction
+  double mx, my, mz;  // coords_center(:, ic): monomials (the reference passes the loop index)
+  int atom;           // center_to_atom(centers_hartree_potential(ic) - 1) - 1
+  int mc_atom;        // center_to_atom(ic) - 1: row of multipole_c used by the term sum
+  int l_const;        // l_hartree(species(atom) - 1)
+  int pad;
+};
+static_assert(sizeof(CenterRec) == 64, "CenterRec layout");
+
+// Number of merged (i, j, k, q) coefficients of the nested-Horner form for one l:
+// sum_{q=0}^{l/2} (l - 2q + 1)^2.
+constexpr int horner_ncoef(int L) {
+  int n = 0;
+  for (int q = 0; 2 * q <= L; q++)
+    n += (L - 2 * q + 1) * (L - 2 * q + 1);
+  return n;
+}
+constexpr int horner_prefix(int L) {  // offset of the l = L table inside one (atom, t) block
+  int n = 0;
+  for (int l = 0; l < L; l++)
+    n += horner_ncoef(l);
+  return n;
+}
+// Index of coefficient (q, i, j, k) inside the table of L.  Order: q, then i, j, k ascending with
+// i <= min(1, D), j <= D - i, k <= D - i - j, D = L - 2q.
+constexpr int horner_index(int L, int q, int i, int j, int k) {
+  int n = 0;
+  for (int qq = 0; qq < q; qq++)
+    n += (L - 2 * qq + 1) * (L - 2 * qq + 1);
+  const int D = L - 2 * q;
+  for (int ii = 0; ii < i; ii++)
+    for (int jj = 0; jj <= D - ii; jj++)
+      n += D - ii - jj + 1;
+  for (int jj = 0; jj < j; jj++)
+    n += D - i - jj + 1;
+  return n + k;
+}
+
+struct FarPlan {
+  int l_pot_max = 0, lama = 0, dim0 = 0, n_centers = 0, n_hc = 0, n_atoms = 0, n_species = 0, hfla = 1;
+  int mc_dim0 = 0;          // n_cc_lm_ijk(l_pot_max)
+  bool terms_ok = false;    // caller's table equals the compiled term table (term-by-term kernel usable)
+  int coef_stride = 0;      // horner_prefix(l_pot_max + 1)
+  std::vector<int> h_ncc;
+  DevBuf<CenterRec> d_centers;
+  DevBuf<int> d_index_cc;   // (dim0, 6)
+  DevBuf<double> d_cc;
+  DevBuf<int> d_ncc;
+  // merged-coefficient gather plan: entry e of a (atom, t) block sums terms hc_term[hc_ptr[e]..hc_ptr[e+1])
+  DevBuf<int> d_hc_ptr, d_hc_term;
+  DevBuf<double> d_hc_w;    // (-1)^nn (2nn-1)!! cc(n) per listed term
+  DevBuf<int> d_atom_ic_ptr, d_atom_ic;  // hartree centres ic whose multipole_c row is atom a
+  // per-call scratch
+  DevBuf<double> d_mc;      // (mc_dim0, n_atoms, T) reference multipole_c
+  DevBuf<double> d_coef;    // (coef_stride, n_atoms, T)
+  DevBuf<double> d_partial; // adjoint partial sums
+};
+
+struct FarCall {
+  int T = 1, n_tile = 1, n_full = 0, n_valid = 0;
+  const double *coord_points = nullptr;
+  const int *map = nullptr, *lmfd = nullptr;
+  const double *mrsq = nullptr, *opr = nullptr, *moments = nullptr;
+};
+
+FarPlan *make_far_plan(int l_pot_max, int lama, int dim0, int n_centers, int n_hc, int n_atoms, int n_species,
+                       int hfla, const int *species, const int *l_hartree, const int *chp, const int *center_to_atom,
+                       const double *coords_center, const int *index_cc, const double *cc, const int *n_cc_lm_ijk,
+                       const int *index_ijk_max_cc);
+void far_forward(FarPlan &P, const FarCall &c, int variant, double *dv, hipStream_t s);
+void far_adjoint(FarPlan &P, const FarCall &c, const double *gV, double *gM, hipStream_t s);
+
+} // namespace frpa

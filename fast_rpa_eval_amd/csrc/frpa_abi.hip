@@ -1,0 +1,388 @@
+// C-ABI of libfrpa: plan construction (host-side batch/centre preprocessing), the frpa_* entry
+// points, and the reference's Fortran-ABI entry points (declared in include/frpa.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+
+#include "farfield_common.hpp"
+#include "rho1_common.hpp"
+
+namespace frpa {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+void clear_error() { g_last_error.clear(); }
+
+__global__ void fill_f64_kernel(double *p, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+void fill_f64(double *p, size_t n, double v, hipStream_t s) {
+  if (n == 0)
+    return;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  check_launch("fill_f64_kernel");
+}
+
+Rho1Plan::~Rho1Plan() {
+  if (ozaki)
+    ozaki_release(ozaki);
+}
+
+// ---------------------------------------------------------------------------------------------
+// rho1 plan
+// ---------------------------------------------------------------------------------------------
+Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_max_batch_size,
+                                const int *n_point_batches, const int *prefix, const int *map, const int *ncc,
+                                const long long *ibasis_ptr, const int *ibasis, const long long *wave_offset) {
+  FRPA_REQUIRE(n_batches >= 0 && n_basis > 0 && n_full_points >= 0 && n_max_batch_size >= 0, FRPA_ERR_INVALID,
+               "rho1 plan: bad sizes (n_batches %d, n_basis %d, n_full_points %d, n_max_batch_size %d)", n_batches,
+               n_basis, n_full_points, n_max_batch_size);
+  FRPA_REQUIRE(n_batches == 0 || (n_point_batches && prefix && ncc && ibasis_ptr && wave_offset), FRPA_ERR_INVALID,
+               "rho1 plan: null metadata array");
+  std::unique_ptr<Rho1Plan> P(new Rho1Plan());
+  P->n_batches = n_batches;
+  P->n_basis = n_basis;
+  P->n_full = n_full_points;
+  P->nmb = n_max_batch_size;
+  P->wave_len = n_batches ? wave_offset[n_batches] : 0;
+  const int n_valid_points = n_batches ? prefix[n_batches] : 0;
+  FRPA_REQUIRE(n_valid_points >= 0 && n_valid_points <= n_full_points, FRPA_ERR_INVALID,
+               "rho1 plan: %d valid points for %d full points", n_valid_points, n_full_points);
+  for (int i = 0; i < n_valid_points; i++)
+    FRPA_REQUIRE(map[i] >= 0 && map[i] < n_full_points, FRPA_ERR_INVALID, "rho1 plan: map[%d] = %d out of range",
+                 i, map[i]);
+  std::vector<BatchDesc> desc;
+  for (int b = 0; b < n_batches; b++) {
+    const int nc = ncc[b], np = n_point_batches[b];
+    FRPA_REQUIRE(nc >= 0 && np >= 0 && np <= n_max_batch_size, FRPA_ERR_INVALID,
+                 "rho1 plan: batch %d has n_compute_c %d, n_points %d (n_max_batch_size %d)", b, nc, np,
+                 n_max_batch_size);
+    FRPA_REQUIRE(prefix[b + 1] - prefix[b] == np, FRPA_ERR_INVALID, "rho1 plan: prefix sum disagrees at batch %d", b);
+    FRPA_REQUIRE(ibasis_ptr[b + 1] - ibasis_ptr[b] == nc, FRPA_ERR_INVALID,
+                 "rho1 plan: batch %d lists %lld basis functions for n_compute_c %d", b,
+                 ibasis_ptr[b + 1] - ibasis_ptr[b], nc);
+    const long long ncp = (nc + 3) / 4 * 4;
+    FRPA_REQUIRE(wave_offset[b + 1] - wave_offset[b] >= ncp * np, FRPA_ERR_INVALID,
+                 "rho1 plan: wave block of batch %d too short", b);
+    if (nc == 0 || np == 0)
+      continue;  // the reference skips n_compute_c == 0; an empty batch writes nothing
+    for (int i = 0; i < nc; i++) {
+      const int v = ibasis[ibasis_ptr[b] + i];
+      FRPA_REQUIRE(v >= 1 && v <= n_basis, FRPA_ERR_INVALID, "rho1 plan: i_basis %d of batch %d out of range", v, b);
+    }
+    desc.push_back(BatchDesc{wave_offset[b], 0, nc, np, prefix[b], b});
+  }
+  // compact 0-based basis lists, in processing order
+  std::stable_sort(desc.begin(), desc.end(), [](const BatchDesc &x, const BatchDesc &y) {
+    return (double)x.nc * x.nc * x.np > (double)y.nc * y.nc * y.np;
+  });
+  long long off = 0;
+  for (auto &d : desc) {
+    const int b = d.b;
+    d.ib_off = off;
+    for (int i = 0; i < d.nc; i++)
+      P->h_ibasis0.push_back(ibasis[ibasis_ptr[b] + i] - 1);
+    off += d.nc;
+    P->max_np = std::max(P->max_np, d.np);
+    P->max_nc = std::max(P->max_nc, d.nc);
+  }
+  P->n_valid = (int)desc.size();
+  P->h_desc = desc;
+  P->d_desc.upload(desc);
+  P->d_ibasis0.upload(P->h_ibasis0);
+  P->d_map.upload(map, (size_t)n_valid_points);
+  return P.release();
+}
+
+// ---------------------------------------------------------------------------------------------
+// far-field plan
+// ---------------------------------------------------------------------------------------------
+static int odd_double_factorial(int n) {  // (2n-1)!!, (-1)!! = 1
+  int r = 1;
+  for (int v = 2 * n - 1; v > 0; v -= 2)
+    r *= v;
+  return r;
+}
+
+FarPlan *make_far_plan(int l_pot_max, int lama, int dim0, int n_centers, int n_hc, int n_atoms, int n_species,
+                              int hfla, const int *species, const int *l_hartree, const int *chp,
+                              const int *center_to_atom, const double *coords_center, const int *index_cc,
+                              const double *cc, const int *n_cc_lm_ijk, const int *index_ijk_max_cc) {
+  (void)index_ijk_max_cc;
+  FRPA_REQUIRE(l_pot_max >= 0 && l_pot_max <= FAR_LMAX, FRPA_ERR_UNSUPPORTED, "l_pot_max %d outside [0, %d]",
+               l_pot_max, FAR_LMAX);
+  FRPA_REQUIRE(lama >= l_pot_max && n_centers >= 0 && n_hc >= 0 && n_atoms > 0 && n_species > 0 && dim0 > 0,
+               FRPA_ERR_INVALID, "far plan: bad sizes");
+  std::unique_ptr<FarPlan> P(new FarPlan());
+  P->l_pot_max = l_pot_max;
+  P->lama = lama;
+  P->dim0 = dim0;
+  P->n_centers = n_centers;
+  P->n_hc = n_hc;
+  P->n_atoms = n_atoms;
+  P->n_species = n_species;
+  P->hfla = hfla;
+  P->h_ncc.assign(n_cc_lm_ijk, n_cc_lm_ijk + lama + 1);
+  P->mc_dim0 = n_cc_lm_ijk[l_pot_max];
+  FRPA_REQUIRE(P->h_ncc[lama] <= dim0, FRPA_ERR_INVALID, "n_cc_lm_ijk(l_max_analytic_multipole) > index_cc_dim_0");
+  for (int s = 0; s < n_species; s++)
+    FRPA_REQUIRE(l_hartree[s] >= 0 && l_hartree[s] <= l_pot_max, FRPA_ERR_INVALID,
+                 "l_hartree(%d) = %d outside [0, l_pot_max]", s + 1, l_hartree[s]);
+  // term table rows n < n_cc(l_pot_max): (l, m, i, j, k, nn)
+  auto col = [&](int n, int c) { return index_cc[n + (size_t)dim0 * c]; };
+  for (int n = 0; n < P->h_ncc[lama]; n++)
+    FRPA_REQUIRE(col(n, 0) >= 0 && col(n, 0) <= lama && std::abs(col(n, 1)) <= col(n, 0), FRPA_ERR_INVALID,
+                 "index_cc row %d: (l, m) = (%d, %d) invalid", n, col(n, 0), col(n, 1));
+  bool same = true;
+  for (int l = 0; l <= l_pot_max; l++)
+    same = same && P->h_ncc[l] == FAR_NCC[l];
+  for (int n = 0; same && n < P->mc_dim0; n++) {
+    const FarTerm &T = FAR_TERMS[n];
+    same = col(n, 0) == T.l && col(n, 1) == T.m && col(n, 2) == T.i && col(n, 3) == T.j && col(n, 4) == T.k &&
+           col(n, 5) == T.nn;
+  }
+  P->terms_ok = same;
+  // centres
+  std::vector<CenterRec> cen(n_hc);
+  std::vector<std::vector<int>> atom_ic(n_atoms);
+  for (int ic = 0; ic < n_hc; ic++) {
+    const int cur = chp[ic] - 1;
+    FRPA_REQUIRE(cur >= 0 && cur < n_centers && ic < n_centers, FRPA_ERR_INVALID,
+                 "centers_hartree_potential(%d) = %d out of range", ic + 1, chp[ic]);
+    const int atom = center_to_atom[cur] - 1;
+    const int mca = center_to_atom[ic] - 1;
+    FRPA_REQUIRE(atom >= 0 && atom < n_atoms && mca >= 0 && mca < n_atoms, FRPA_ERR_INVALID,
+                 "center_to_atom out of range at centre %d", ic + 1);
+    const int sp = species[atom] - 1;
+    FRPA_REQUIRE(sp >= 0 && sp < n_species, FRPA_ERR_INVALID, "species(%d) out of range", atom + 1);
+    cen[ic] = CenterRec{coords_center[3 * cur], coords_center[3 * cur + 1], coords_center[3 * cur + 2],
+                        coords_center[3 * ic], coords_center[3 * ic + 1], coords_center[3 * ic + 2],
+                        atom, mca, l_hartree[sp], 0};
+    atom_ic[mca].push_back(ic);
+  }
+  // merged nested-Horner gather plan: for each l and each (i, j, k, q) the terms n < n_cc(l) feeding it
+  P->coef_stride = horner_prefix(l_pot_max + 1);
+  std::vector<std::vector<std::pair<int, double>>> lists(P->coef_stride);
+  bool horner_ok = true;
+  for (int L = 0; L <= l_pot_max; L++) {
+    for (int n = 0; n < P->h_ncc[L]; n++) {
+      const int i = col(n, 2), j = col(n, 3), k = col(n, 4), nn = col(n, 5), tl = col(n, 0);
+      const int d = i + j + k, q = nn - d;
+      if (i < 0 || i > 1 || j < 0 || k < 0 || q < 0 || d + 2 * q > L || tl > L) {
+        horner_ok = false;
+        continue;
+      }
+      const double sf = ((nn & 1) ? -1.0 : 1.0) * odd_double_factorial(nn);
+      lists[horner_prefix(L) + horner_index(L, q, i, j, k)].push_back({n, sf * cc[n]});
+    }
+  }
+  FRPA_REQUIRE(horner_ok, FRPA_ERR_UNSUPPORTED,
+               "index_cc rows are not of the form x^i y^j z^k Fp(nn), i <= 1, nn = (l + i + j + k) / 2");
+  std::vector<int> hp(1, 0), ht;
+  std::vector<double> hw;
+  for (auto &v : lists) {
+    for (auto &pr : v) {
+      ht.push_back(pr.first);
+      hw.push_back(pr.second);
+    }
+    hp.push_back((int)ht.size());
+  }
+  std::vector<int> aip(1, 0), ai;
+  for (auto &v : atom_ic) {
+    ai.insert(ai.end(), v.begin(), v.end());
+    aip.push_back((int)ai.size());
+  }
+  P->d_centers.upload(cen);
+  P->d_index_cc.upload(index_cc, (size_t)dim0 * 6);
+  std::vector<double> ccp(std::max<size_t>(dim0, FAR_NCC[FAR_LMAX]), 0.0);
+  std::copy(cc, cc + dim0, ccp.begin());
+  P->d_cc.upload(ccp);
+  P->d_ncc.upload(P->h_ncc);
+  P->d_hc_ptr.upload(hp);
+  P->d_hc_term.upload(ht.empty() ? std::vector<int>{0} : ht);
+  P->d_hc_w.upload(hw.empty() ? std::vector<double>{0.0} : hw);
+  P->d_atom_ic_ptr.upload(aip);
+  P->d_atom_ic.upload(ai.empty() ? std::vector<int>{0} : ai);
+  return P.release();
+}
+
+} // namespace frpa
+
+using namespace frpa;
+
+// =============================================================================================
+// frpa_* C-ABI
+// =============================================================================================
+extern "C" {
+
+const char *frpa_last_error(void) { return g_last_error.c_str(); }
+const char *frpa_version(void) { return "fast_rpa_eval_amd 0.1 (gfx950)"; }
+
+int frpa_rho1_plan_create(frpa_rho1_plan **plan, int n_batches, int n_basis, int n_full_points, int n_max_batch_size,
+                          const int *n_point_batches, const int *n_point_batches_prefix_sum,
+                          const int *i_valid_point_2_i_full_points_map, const int *n_compute_c_batches,
+                          const long long *ibasis_ptr, const int *ibasis, const long long *wave_offset) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "plan is null");
+    *plan = reinterpret_cast<frpa_rho1_plan *>(make_rho1_plan(
+        n_batches, n_basis, n_full_points, n_max_batch_size, n_point_batches, n_point_batches_prefix_sum,
+        i_valid_point_2_i_full_points_map, n_compute_c_batches, ibasis_ptr, ibasis, wave_offset));
+  });
+}
+
+int frpa_rho1_plan_destroy(frpa_rho1_plan *plan) {
+  return guarded([&] { delete reinterpret_cast<Rho1Plan *>(plan); });
+}
+
+int frpa_rho1_forward(frpa_rho1_plan *plan, int atom_tile_size, int n_tile, int precision, const double *wave,
+                      const double *density_matrix, const double *rho_bias, double *first_order_rho,
+                      const double *gradient_wave, const double *grad_bias, double *first_order_gradient_rho,
+                      void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "plan is null");
+    Rho1Plan &P = *reinterpret_cast<Rho1Plan *>(plan);
+    FRPA_REQUIRE(atom_tile_size >= 1 && n_tile >= 0 && n_tile <= atom_tile_size, FRPA_ERR_INVALID,
+                 "atom_tile_size %d, n_tile %d", atom_tile_size, n_tile);
+    FRPA_REQUIRE(first_order_rho && density_matrix && (wave || P.n_valid == 0), FRPA_ERR_INVALID,
+                 "null device array");
+    FRPA_REQUIRE(!gradient_wave || first_order_gradient_rho, FRPA_ERR_INVALID,
+                 "gradient_wave given without first_order_gradient_rho");
+    hipStream_t s = as_stream(stream);
+    Rho1Call c;
+    c.T = atom_tile_size;
+    c.n_tile = n_tile;
+    c.wave = wave;
+    c.gwave = gradient_wave;
+    c.P1 = density_matrix;
+    c.bias = rho_bias;
+    c.gbias = grad_bias;
+    c.rho = first_order_rho;
+    c.grad = gradient_wave ? first_order_gradient_rho : nullptr;
+    // evaluate_first_order_rho.cpp:113-114: both outputs are zeroed first
+    fill_f64(first_order_rho, (size_t)P.n_full * atom_tile_size, 0.0, s);
+    if (first_order_gradient_rho)
+      fill_f64(first_order_gradient_rho, (size_t)3 * P.nmb * P.n_batches * atom_tile_size, 0.0, s);
+    if (precision == FRPA_PREC_FP64) {
+      rho1_forward_fp64(P, c, s);
+    } else if (precision == FRPA_PREC_OZAKI) {
+      rho1_forward_ozaki(P, c, s);
+    } else {
+      FRPA_REQUIRE(false, FRPA_ERR_INVALID, "unknown precision %d", precision);
+    }
+  });
+}
+
+int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, const double *wave, const double *g,
+                      double *first_order_H, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && g && first_order_H && atom_tile_size >= 1, FRPA_ERR_INVALID, "bad adjoint arguments");
+    Rho1Plan &P = *reinterpret_cast<Rho1Plan *>(plan);
+    rho1_adjoint(P, atom_tile_size, precision, wave, g, first_order_H, as_stream(stream));
+  });
+}
+
+int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, const double *wave, long long *bytes, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && wave, FRPA_ERR_INVALID, "bad arguments");
+    long long b = ozaki_prepare_wave(*reinterpret_cast<Rho1Plan *>(plan), wave, as_stream(stream));
+    if (bytes)
+      *bytes = b;
+  });
+}
+
+int frpa_synth_wave(frpa_rho1_plan *plan, int n_atoms, const double *atom_xyz, const int *basis_atom,
+                    const int *basis_l, const int *basis_m, const double *basis_alpha, const double *coord_points,
+                    double *wave, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && atom_xyz && basis_atom && basis_l && basis_m && basis_alpha && coord_points && wave,
+                 FRPA_ERR_INVALID, "null argument");
+    const Rho1Plan &P = *reinterpret_cast<Rho1Plan *>(plan);
+    synth_wave(P, n_atoms, atom_xyz, basis_atom, basis_l, basis_m, basis_alpha, P.n_basis, coord_points, wave,
+               as_stream(stream));
+  });
+}
+
+int frpa_far_plan_create(frpa_far_plan **plan, int l_pot_max, int l_max_analytic_multipole, int index_cc_dim_0,
+                         int n_centers, int n_centers_hartree_potential, int n_atoms, int n_species,
+                         int hartree_force_l_add, const int *species, const int *l_hartree,
+                         const int *centers_hartree_potential, const int *center_to_atom,
+                         const double *coords_center, const int *index_cc, const double *cc,
+                         const int *n_cc_lm_ijk, const int *index_ijk_max_cc) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && species && l_hartree && centers_hartree_potential && center_to_atom && coords_center &&
+                     index_cc && cc && n_cc_lm_ijk,
+                 FRPA_ERR_INVALID, "null argument");
+    *plan = reinterpret_cast<frpa_far_plan *>(make_far_plan(
+        l_pot_max, l_max_analytic_multipole, index_cc_dim_0, n_centers, n_centers_hartree_potential, n_atoms,
+        n_species, hartree_force_l_add, species, l_hartree, centers_hartree_potential, center_to_atom, coords_center,
+        index_cc, cc, n_cc_lm_ijk, index_ijk_max_cc));
+  });
+}
+
+int frpa_far_plan_destroy(frpa_far_plan *plan) {
+  return guarded([&] { delete reinterpret_cast<FarPlan *>(plan); });
+}
+
+static FarCall far_call(int T, int n_tile, int n_full, int n_valid, const double *cp, const int *map, const int *lmfd,
+                        const double *mrsq, const double *opr, const double *M) {
+  FRPA_REQUIRE(T >= 1 && n_tile >= 0 && n_tile <= T && n_full >= 0 && n_valid >= 0, FRPA_ERR_INVALID,
+               "far: bad sizes (T %d, n_tile %d, n_full %d, n_valid %d)", T, n_tile, n_full, n_valid);
+  FRPA_REQUIRE(n_valid == 0 || (cp && map && lmfd && mrsq && opr && M), FRPA_ERR_INVALID, "far: null array");
+  FarCall c;
+  c.T = T;
+  c.n_tile = n_tile;
+  c.n_full = n_full;
+  c.n_valid = n_valid;
+  c.coord_points = cp;
+  c.map = map;
+  c.lmfd = lmfd;
+  c.mrsq = mrsq;
+  c.opr = opr;
+  c.moments = M;
+  return c;
+}
+
+int frpa_far_forward(frpa_far_plan *plan, int atom_tile_size, int n_tile, int variant, int n_full_points,
+                     int n_valid_points, const double *coord_points, const int *i_valid_point_2_i_full_points_map,
+                     const int *l_hartree_max_far_distance, const double *multipole_radius_sq,
+                     const double *outer_potential_radius, const double *multipole_moments,
+                     double *delta_v_hartree, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && delta_v_hartree, FRPA_ERR_INVALID, "null argument");
+    FarCall c = far_call(atom_tile_size, n_tile, n_full_points, n_valid_points, coord_points,
+                         i_valid_point_2_i_full_points_map, l_hartree_max_far_distance, multipole_radius_sq,
+                         outer_potential_radius, multipole_moments);
+    far_forward(*reinterpret_cast<FarPlan *>(plan), c, variant, delta_v_hartree, as_stream(stream));
+  });
+}
+
+int frpa_far_adjoint(frpa_far_plan *plan, int atom_tile_size, int n_tile, int n_full_points, int n_valid_points,
+                     const double *coord_points, const int *i_valid_point_2_i_full_points_map,
+                     const int *l_hartree_max_far_distance, const double *multipole_radius_sq,
+                     const double *outer_potential_radius, const double *multipole_moments, const double *gV,
+                     double *grad_moments, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && gV && grad_moments, FRPA_ERR_INVALID, "null argument");
+    FarCall c = far_call(atom_tile_size, n_tile, n_full_points, n_valid_points, coord_points,
+                         i_valid_point_2_i_full_points_map, l_hartree_max_far_distance, multipole_radius_sq,
+                         outer_potential_radius, multipole_moments);
+    far_adjoint(*reinterpret_cast<FarPlan *>(plan), c, gV, grad_moments, as_stream(stream));
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,52 @@
+// rho1 plan: batch metadata resident in HBM, shared by the forward and adjoint kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace frpa {
+
+// One valid batch (n_compute_c > 0), in processing order.
+struct BatchDesc {
+  long long wave_off;  // offset of the batch's (nc_pad, np) block in wave_batches_compress
+  long long ib_off;    // offset of its basis list in Rho1Plan::d_ibasis0
+  int nc;              // n_compute_c
+  int np;              // n_point_batches
+  int pstart;          // n_point_batches_prefix_sum[b]
+  int b;               // batch index in the caller's numbering (bias / gradient layouts)
+};
+static_assert(sizeof(BatchDesc) == 32, "BatchDesc layout");
+
+struct OzakiState;  // INT8 split data (rho1_ozaki.hip)
+
+struct Rho1Plan {
+  int n_batches = 0, n_basis = 0, n_full = 0, nmb = 0;
+  int n_valid = 0, max_np = 0, max_nc = 0;
+  long long wave_len = 0;
+  std::vector<BatchDesc> h_desc;    // processing order (descending work)
+  std::vector<int> h_ibasis0;       // 0-based basis indices, compact
+  DevBuf<BatchDesc> d_desc;
+  DevBuf<int> d_ibasis0;
+  DevBuf<int> d_map;
+  OzakiState *ozaki = nullptr;
+  ~Rho1Plan();
+};
+
+struct Rho1Call {
+  int T = 1, n_tile = 1;
+  const double *wave = nullptr, *gwave = nullptr, *P1 = nullptr, *bias = nullptr, *gbias = nullptr;
+  double *rho = nullptr, *grad = nullptr;
+};
+
+Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_max_batch_size,
+                         const int *n_point_batches, const int *prefix, const int *map, const int *ncc,
+                         const long long *ibasis_ptr, const int *ibasis, const long long *wave_offset);
+void rho1_forward_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s);
+void rho1_forward_ozaki(Rho1Plan &P, const Rho1Call &c, hipStream_t s);
+void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const double *g, double *H, hipStream_t s);
+void ozaki_release(OzakiState *st);
+long long ozaki_prepare_wave(Rho1Plan &P, const double *wave, hipStream_t s);
+void synth_wave(const Rho1Plan &P, int n_atoms, const double *atom_xyz, const int *basis_atom, const int *basis_l,
+                const int *basis_m, const double *basis_alpha, int n_basis, const double *coord_points, double *wave,
+                hipStream_t s);
+
+} // namespace frpa

@@ -1,0 +1,204 @@
+// rho1 forward in FP64 on gfx950: one fused kernel per call.
+//
+// Reference: fast_rap_src_dfpt_gpu_eval/evaluate_first_order_rho.cu:20-371 runs three launches per
+// tile of 256 batches (gather of P1 into dense per-batch slots, MAGMA vbatched DGEMM writing
+// W = P1loc * phi to HBM, and a ddot kernel re-reading W with one thread per point).  Here one
+// workgroup owns one batch (and one perturbation t):
+//   * the P1 gather is folded into the A-operand load of the MFMA (rows i_basis(i), columns
+//     i_basis(j) read straight from P1; consecutive lanes read consecutive rows);
+//   * phi (the B operand) is staged through LDS in K-chunks, shared by the four waves;
+//   * W = P1loc * phi lives only in MFMA accumulators (v_mfma_f64_16x16x4f64);
+//   * the ddot rho(p) = sum_i phi(i, p) W(i, p) (and the GGA gradient sum) is the epilogue,
+//     reduced across the 16-row MFMA tile with cross-lane shuffles and across waves in LDS;
+//   * bias add and the scatter to first_order_rho(map(p), t) are fused.
+#include "rho1_common.hpp"
+
+namespace frpa {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int KC = 32;  // K chunk staged in LDS
+
+template <int MAXPG, bool GRAD>
+__global__ __launch_bounds__(256) void rho1_fp64_kernel(const BatchDesc *__restrict__ descs,
+                                                        const int *__restrict__ ibasis0,
+                                                        const int *__restrict__ map, const double *__restrict__ wave,
+                                                        const double *__restrict__ gwave,
+                                                        const double *__restrict__ P1, int n_basis,
+                                                        const double *__restrict__ bias,
+                                                        const double *__restrict__ gbias, double *__restrict__ rho,
+                                                        double *__restrict__ grad, int n_full, int nmb,
+                                                        int n_batches_all, int n_tile) {
+  constexpr int PMAX = 16 * MAXPG;
+  constexpr int PLD = PMAX + 1;  // padded LDS row: the transposed staging store avoids bank conflicts
+  constexpr int NRED = GRAD ? 4 : 1;
+  __shared__ double Bs[KC * PLD];
+  __shared__ int jbs[KC];
+  __shared__ double red[NRED][4][PMAX];
+
+  const BatchDesc d = descs[blockIdx.x];
+  const int t = blockIdx.y;
+  const int nc = d.nc, np_all = d.np;
+  const int ld = (nc + 3) & ~3;
+  const double *W = wave + d.wave_off;
+  const double *G = GRAD ? gwave + 3 * d.wave_off : nullptr;
+  const int *ib = ibasis0 + d.ib_off;
+  const double *Pt = P1 + (size_t)n_basis * n_basis * t;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  for (int pc = 0; pc < np_all; pc += PMAX) {
+    const int np = min(PMAX, np_all - pc);
+    const int npg = (np + 15) >> 4;
+    const double *Wp = W + (size_t)ld * pc;
+
+    double racc[MAXPG];
+    double gacc[3][MAXPG];
+#pragma unroll
+    for (int g = 0; g < MAXPG; g++) {
+      racc[g] = 0.0;
+      if (GRAD) {
+        gacc[0][g] = gacc[1][g] = gacc[2][g] = 0.0;
+      }
+    }
+
+    for (int i0 = 0; i0 < nc; i0 += 64) {
+      const int irow = i0 + 16 * w + lr;
+      const bool iok = irow < nc;
+      const size_t arow = iok ? (size_t)ib[irow] : 0;
+      v4d acc[MAXPG];
+#pragma unroll
+      for (int g = 0; g < MAXPG; g++)
+        acc[g] = v4d{0.0, 0.0, 0.0, 0.0};
+
+      for (int j0 = 0; j0 < nc; j0 += KC) {
+        __syncthreads();
+        for (int e = tid; e < KC * npg * 16; e += 256) {
+          const int k = e % KC, p = e / KC;
+          const int j = j0 + k;
+          Bs[k * PLD + p] = (j < nc && p < np) ? Wp[j + (size_t)ld * p] : 0.0;
+        }
+        if (tid < KC) {
+          const int j = j0 + tid;
+          jbs[tid] = j < nc ? ib[j] : -1;
+        }
+        __syncthreads();
+        double a[KC / 4];
+#pragma unroll
+        for (int s = 0; s < KC / 4; s++) {
+          const int jb = jbs[4 * s + lk];
+          a[s] = (iok && jb >= 0) ? Pt[arow + (size_t)n_basis * jb] : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < KC / 4; s++) {
+          const double *brow = &Bs[(4 * s + lk) * PLD + lr];
+#pragma unroll
+          for (int g = 0; g < MAXPG; g++)
+            if (g < npg)
+              acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], brow[16 * g], acc[g], 0, 0, 0);
+        }
+      }
+
+      // epilogue: acc[g][r] = W(i0 + 16w + 4 lk + r, pc + 16 g + lr)
+#pragma unroll
+      for (int g = 0; g < MAXPG; g++) {
+        if (g < npg) {
+          const int p = 16 * g + lr;
+          double part = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
+          if (p < np) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              const int i = i0 + 16 * w + 4 * lk + r;
+              if (i < nc) {
+                part += acc[g][r] * Wp[i + (size_t)ld * p];
+                if (GRAD) {
+                  const double *gp = G + (size_t)3 * ld * (pc + p) + i;
+                  p0 += acc[g][r] * gp[0];
+                  p1 += acc[g][r] * gp[ld];
+                  p2 += acc[g][r] * gp[2 * ld];
+                }
+              }
+            }
+          }
+          part += __shfl_xor(part, 16);
+          part += __shfl_xor(part, 32);
+          racc[g] += part;
+          if (GRAD) {
+            p0 += __shfl_xor(p0, 16);
+            p0 += __shfl_xor(p0, 32);
+            p1 += __shfl_xor(p1, 16);
+            p1 += __shfl_xor(p1, 32);
+            p2 += __shfl_xor(p2, 16);
+            p2 += __shfl_xor(p2, 32);
+            gacc[0][g] += p0;
+            gacc[1][g] += p1;
+            gacc[2][g] += p2;
+          }
+        }
+      }
+    }
+
+    __syncthreads();
+    if (lk == 0) {
+#pragma unroll
+      for (int g = 0; g < MAXPG; g++) {
+        red[0][w][16 * g + lr] = racc[g];
+        if (GRAD) {
+          red[1][w][16 * g + lr] = gacc[0][g];
+          red[2][w][16 * g + lr] = gacc[1][g];
+          red[3][w][16 * g + lr] = gacc[2][g];
+        }
+      }
+    }
+    __syncthreads();
+    for (int p = tid; p < np; p += 256) {
+      const size_t bi = (size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
+      double s = red[0][0][p] + red[0][1][p] + red[0][2][p] + red[0][3][p];
+      if (bias)
+        s += bias[bi];
+      if (t < n_tile)
+        rho[(size_t)map[d.pstart + pc + p] + (size_t)n_full * t] = s;
+      if (GRAD) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double v = red[1 + c][0][p] + red[1 + c][1][p] + red[1 + c][2][p] + red[1 + c][3][p];
+          grad[c + 3 * bi] = v * 2 - (gbias ? gbias[c + 3 * bi] : 0.0);
+        }
+      }
+    }
+  }
+}
+
+template <int MAXPG, bool GRAD>
+void launch_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
+  dim3 grid(P.n_valid, c.T);
+  hipLaunchKernelGGL((rho1_fp64_kernel<MAXPG, GRAD>), grid, dim3(256), 0, s, P.d_desc.ptr, P.d_ibasis0.ptr,
+                     P.d_map.ptr, c.wave, c.gwave, c.P1, P.n_basis, c.bias, c.gbias, c.rho, c.grad, P.n_full,
+                     P.nmb, P.n_batches, c.n_tile);
+  check_launch("rho1_fp64_kernel");
+}
+
+template <bool GRAD>
+void dispatch_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
+  if (P.max_np <= 64)
+    launch_fp64<4, GRAD>(P, c, s);
+  else if (P.max_np <= 128)
+    launch_fp64<8, GRAD>(P, c, s);
+  else
+    launch_fp64<13, GRAD>(P, c, s);  // 208 points per pass; larger batches loop over chunks
+}
+
+} // namespace
+
+void rho1_forward_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
+  if (P.n_valid == 0)
+    return;
+  if (c.grad)
+    dispatch_fp64<true>(P, c, s);
+  else
+    dispatch_fp64<false>(P, c, s);
+}
+
+} // namespace frpa

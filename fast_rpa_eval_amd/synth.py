@@ -1,0 +1,385 @@
+"""Seeded synthetic inputs for the rho1 and far-field Hartree paths, in the reference's layouts.
+
+The reference's drivers fill every numeric array with random values on top of index structures
+dumped from FHI-aims (fast_rap_src_dfpt_gpu_eval/direct_test/evaluate_first_order_rho_run.cpp:58-65).
+Those dumps are not available, so the index structures are generated here too, from a small
+physical model:
+
+* atoms uniform in a cube with a minimum separation (rejection sampling);
+* 14 basis functions per atom: 3 s shells, 2 p shells, 1 d shell, each exp(-alpha r^2) Y_lm(r^),
+  alpha log-uniform in [0.1, 10] bohr^-2, Y_lm in the SHEval convention;
+* integration batches around atoms, points ~ centre + N(0, 1 bohr^2);
+* P1 symmetric, P1_ij ~ N(0, 1) exp(-d_ij / 2 bohr), d_ij the distance of the host atoms.
+
+Layouts follow the reference's Fortran-ABI arguments (evaluate_first_order_rho.h:3-35):
+``i_basis_batches`` (n_centers_basis_I, n_batches) 1-based, ``wave_batches_compress`` batch b at
+``i_batch_2_wave_offset[b]`` as (nc_pad, np) column-major with nc_pad = ceil(nc/4)*4, the valid ->
+full point map 0-based.  Padding rows are filled with NaN so that any read of them shows.
+
+Large configurations keep the basis-index lists compact (CSR: ``ibasis`` + ``ibasis_ptr``) and
+64-bit wave offsets, since the dense (n_centers_basis_I, n_batches) table and int32 offsets of the
+reference ABI do not scale to them; ``dense_i_basis_batches`` expands on demand.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .multipole_tables import far_tables, sh_norm, solid_harmonic_terms
+
+ANG = 1.8897261  # bohr per Angstrom
+
+# Basis recipe: (l, count of shells) -> 3 s + 2 p + 1 d = 14 functions per atom.
+SHELLS = ((0, 3), (1, 2), (2, 1))
+
+
+def seeds(k):
+    """Seeds of configuration k (1-based): geometry, basis exponents, P1/M, grid, v1."""
+    return {"geometry": 1000 + k, "basis": 2000 + k, "p1": 3000 + k, "grid": 4000 + k, "v1": 5000 + k}
+
+
+def make_geometry(n_atoms, box_bohr, min_sep_bohr, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    tries = 0
+    while len(out) < n_atoms:
+        c = rng.uniform(0.0, box_bohr, size=3)
+        tries += 1
+        if tries > 2000 * n_atoms:
+            raise RuntimeError("cannot place atoms with the requested minimum separation")
+        if out and np.min(np.sum((np.asarray(out) - c) ** 2, axis=1)) < min_sep_bohr ** 2:
+            continue
+        out.append(c)
+    return np.asarray(out)
+
+
+@dataclass
+class Basis:
+    atom: np.ndarray   # int32 (n_basis,) 0-based host atom
+    l: np.ndarray      # int32
+    m: np.ndarray      # int32
+    alpha: np.ndarray  # float64, bohr^-2
+
+    @property
+    def n_basis(self):
+        return len(self.atom)
+
+
+def make_basis(n_atoms, seed):
+    rng = np.random.default_rng(seed)
+    atom, ls, ms, al = [], [], [], []
+    for a in range(n_atoms):
+        for l, nsh in SHELLS:
+            for _ in range(nsh):
+                alpha = float(np.exp(rng.uniform(np.log(0.1), np.log(10.0))))
+                for m in range(-l, l + 1):
+                    atom.append(a); ls.append(l); ms.append(m); al.append(alpha)
+    return Basis(np.asarray(atom, np.int32), np.asarray(ls, np.int32), np.asarray(ms, np.int32),
+                 np.asarray(al, np.float64))
+
+
+def _sh_poly_coeffs(l_max=2):
+    """Unit-sphere polynomial of each SHEval Y_lm, l <= l_max: {(l, m): [(i, j, k, coef)]}."""
+    out = {}
+    for l in range(l_max + 1):
+        for m in range(-l, l + 1):
+            nrm = sh_norm(l, m)
+            out[(l, m)] = [(i, j, k, float(c) * nrm) for (i, j, k, s), c in solid_harmonic_terms(l, m).items()]
+    return out
+
+
+_SH2 = _sh_poly_coeffs(2)
+
+
+def eval_basis(atom_xyz, basis, idx, pts):
+    """phi_{idx[i]}(pts[p]) as a (len(idx), len(pts)) array."""
+    d = pts[None, :, :] - atom_xyz[basis.atom[idx]][:, None, :]
+    r2 = np.sum(d * d, axis=2)
+    r = np.sqrt(r2)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        u = np.where(r[..., None] > 0, d / r[..., None], 0.0)
+    x, y, z = u[..., 0], u[..., 1], u[..., 2]
+    ang = np.zeros_like(r2)
+    ls, ms = basis.l[idx], basis.m[idx]
+    for (l, m), poly in _SH2.items():
+        sel = (ls == l) & (ms == m)
+        if not np.any(sel):
+            continue
+        v = 0.0
+        for i, j, k, c in poly:
+            v = v + c * x[sel] ** i * y[sel] ** j * z[sel] ** k
+        ang[sel] = v
+    return np.exp(-basis.alpha[idx][:, None] * r2) * ang
+
+
+def make_p1(atom_xyz, basis, seed, n_tile=1, cut=0.0):
+    """Symmetric P1 (n_basis, n_basis, n_tile), Fortran order."""
+    rng = np.random.default_rng(seed)
+    nb = basis.n_basis
+    xyz = atom_xyz[basis.atom]
+    out = np.empty((nb, nb, n_tile), order="F")
+    for t in range(n_tile):
+        a = rng.standard_normal((nb, nb))
+        a = np.triu(a) + np.triu(a, 1).T
+        d = np.sqrt(np.sum((xyz[:, None, :] - xyz[None, :, :]) ** 2, axis=2))
+        p = a * np.exp(-d / 2.0)
+        if cut > 0:
+            p[np.abs(p) < cut] = 0.0
+        out[:, :, t] = p
+    return out
+
+
+@dataclass
+class Rho1Problem:
+    """Inputs of evaluate_first_order_rho in the reference's argument layout."""
+    n_atoms: int
+    n_basis: int
+    n_full_points: int
+    n_max_batch_size: int
+    n_max_compute_ham: int
+    atom_xyz: np.ndarray
+    basis: Basis
+    batch_sizes: np.ndarray                 # int32 (nb,)
+    n_point_batches: np.ndarray             # int32 (nb,)
+    n_point_batches_prefix_sum: np.ndarray  # int32 (nb+1,)
+    i_valid_point_2_i_full_points_map: np.ndarray  # int32 (n_full_points,), first sum(np) used
+    n_compute_c_batches: np.ndarray         # int32 (nb,)
+    ibasis_ptr: np.ndarray                  # int64 (nb+1,) CSR of 1-based basis indices
+    ibasis: np.ndarray                      # int32 (sum nc,)
+    wave_offset: np.ndarray                 # int64 (nb+1,)
+    coord_points: np.ndarray                # float64 (3, n_full_points) Fortran order
+    wave: np.ndarray = None                 # float64 (wave_offset[-1],) or None (generated on GPU)
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n_batches(self):
+        return len(self.n_point_batches)
+
+    @property
+    def n_centers_basis_I(self):
+        return self.n_basis
+
+    def dense_i_basis_batches(self):
+        out = np.zeros((self.n_centers_basis_I, self.n_batches), dtype=np.int32, order="F")
+        for b in range(self.n_batches):
+            s, e = self.ibasis_ptr[b], self.ibasis_ptr[b + 1]
+            out[: e - s, b] = self.ibasis[s:e]
+        return out
+
+    def atom_valid_n_compute_c_batches(self):
+        """(n_atoms + 1, n_batches): per-atom prefix over a batch's compute rows (basis sorted by atom)."""
+        out = np.zeros((self.n_atoms + 1, self.n_batches), dtype=np.int32, order="F")
+        for b in range(self.n_batches):
+            ib = self.ibasis[self.ibasis_ptr[b]:self.ibasis_ptr[b + 1]] - 1
+            cnt = np.bincount(self.basis.atom[ib], minlength=self.n_atoms)
+            out[1:, b] = np.cumsum(cnt)
+        return out
+
+    def wave_offset_i32(self):
+        if self.wave_offset[-1] >= 2 ** 31:
+            raise OverflowError("wave_batches_compress exceeds the int32 offsets of the reference ABI")
+        return self.wave_offset.astype(np.int32)
+
+
+def _batch_layout(nps, ncs, extra_rng):
+    """Full/valid point bookkeeping: batch b owns full points [fs, fs + batch_size)."""
+    nb = len(nps)
+    batch_sizes = nps + extra_rng.integers(0, np.maximum(nps // 8, 1) + 1)
+    full_start = np.concatenate([[0], np.cumsum(batch_sizes)])
+    prefix = np.concatenate([[0], np.cumsum(nps)]).astype(np.int32)
+    n_full = int(full_start[-1])
+    mp = np.zeros(n_full, dtype=np.int32)
+    for b in range(nb):
+        sel = np.sort(extra_rng.choice(batch_sizes[b], size=nps[b], replace=False))
+        mp[prefix[b]:prefix[b + 1]] = full_start[b] + sel
+    ncp = (ncs + 3) // 4 * 4
+    wave_offset = np.concatenate([[0], np.cumsum(ncp.astype(np.int64) * nps)]).astype(np.int64)
+    return batch_sizes.astype(np.int32), prefix, mp, n_full, wave_offset
+
+
+def make_rho1_problem(n_atoms, box_ang, n_batches, np_range, nc_range=None, config_k=1, fill_wave=True,
+                      batch_spread_bohr=1.5, zero_nc=(), zero_np=()):
+    """Batched rho1 inputs.
+
+    nc_range None: natural compute sets (basis functions with max_batch |phi| > 1e-10, the rule of
+    the small CPU-runnable configuration); otherwise nc ~ U{nc_range} nearest basis functions.
+    np_range: int (fixed) or (lo, hi) inclusive.
+    zero_nc / zero_np: batch indices forced to n_compute_c = 0 / n_points = 0 (edge cases).
+    """
+    sd = seeds(config_k)
+    xyz = make_geometry(n_atoms, box_ang * ANG, 1.5 * ANG, sd["geometry"])
+    basis = make_basis(n_atoms, sd["basis"])
+    rng = np.random.default_rng(sd["grid"])
+    if isinstance(np_range, int):
+        nps = np.full(n_batches, np_range, dtype=np.int64)
+    else:
+        nps = rng.integers(np_range[0], np_range[1] + 1, size=n_batches)
+    nps[list(zero_np)] = 0
+    host = np.arange(n_batches) % n_atoms
+    if nc_range is None:
+        centres = xyz[host]
+    else:
+        centres = xyz[host] + rng.normal(0.0, batch_spread_bohr, size=(n_batches, 3))
+    pts_per_batch = [centres[b] + rng.normal(0.0, 1.0, size=(nps[b], 3)) for b in range(n_batches)]
+
+    nb_basis = basis.n_basis
+    basis_xyz = xyz[basis.atom]
+    ibasis_list = []
+    if nc_range is None:
+        for b in range(n_batches):
+            phi = eval_basis(xyz, basis, np.arange(nb_basis), pts_per_batch[b])
+            keep = np.nonzero(np.max(np.abs(phi), axis=1) > 1e-10)[0]
+            ibasis_list.append(keep.astype(np.int32) + 1)
+    else:
+        ncs = rng.integers(nc_range[0], nc_range[1] + 1, size=n_batches)
+        ncs = np.minimum(ncs, nb_basis)
+        ncs[list(zero_nc)] = 0
+        for b in range(n_batches):
+            d2 = np.sum((basis_xyz - centres[b]) ** 2, axis=1)
+            order = np.lexsort((np.arange(nb_basis), d2))
+            ibasis_list.append(np.sort(order[: ncs[b]]).astype(np.int32) + 1)
+    ncs = np.asarray([len(v) for v in ibasis_list], dtype=np.int64)
+    batch_sizes, prefix, mp, n_full, wave_offset = _batch_layout(nps, ncs, rng)
+
+    coord_points = np.zeros((3, n_full), order="F")
+    for b in range(n_batches):
+        coord_points[:, mp[prefix[b]:prefix[b + 1]]] = pts_per_batch[b].T
+
+    ibasis_ptr = np.concatenate([[0], np.cumsum(ncs)]).astype(np.int64)
+    ibasis = np.concatenate(ibasis_list).astype(np.int32) if ibasis_list else np.zeros(0, np.int32)
+    prob = Rho1Problem(
+        n_atoms=n_atoms, n_basis=nb_basis, n_full_points=n_full, n_max_batch_size=int(batch_sizes.max()),
+        n_max_compute_ham=int(ncs.max()), atom_xyz=xyz, basis=basis, batch_sizes=batch_sizes,
+        n_point_batches=nps.astype(np.int32), n_point_batches_prefix_sum=prefix,
+        i_valid_point_2_i_full_points_map=mp, n_compute_c_batches=ncs.astype(np.int32),
+        ibasis_ptr=ibasis_ptr, ibasis=ibasis, wave_offset=wave_offset, coord_points=coord_points)
+    if fill_wave:
+        prob.wave = build_wave(prob)
+    return prob
+
+
+def build_wave(prob):
+    """wave_batches_compress: batch b's (nc_pad, np) block of phi, NaN padding rows."""
+    w = np.full(int(prob.wave_offset[-1]), np.nan)
+    for b in range(prob.n_batches):
+        nc, npt = int(prob.n_compute_c_batches[b]), int(prob.n_point_batches[b])
+        if nc == 0 or npt == 0:
+            continue
+        ncp = (nc + 3) // 4 * 4
+        idx = prob.ibasis[prob.ibasis_ptr[b]:prob.ibasis_ptr[b + 1]] - 1
+        pts = prob.coord_points[:, prob.i_valid_point_2_i_full_points_map[
+            prob.n_point_batches_prefix_sum[b]:prob.n_point_batches_prefix_sum[b + 1]]].T
+        blk = np.full((ncp, npt), np.nan)
+        blk[:nc] = eval_basis(prob.atom_xyz, prob.basis, idx, pts)
+        w[prob.wave_offset[b]:prob.wave_offset[b + 1]] = blk.ravel(order="F")
+    return w
+
+
+def random_like_wave(prob, k, seed):
+    """(k * wave_offset[-1],) uniform [-1, 1] data with NaN padding rows, for gradient/hessian waves."""
+    rng = np.random.default_rng(seed)
+    out = rng.uniform(-1.0, 1.0, size=k * int(prob.wave_offset[-1]))
+    for b in range(prob.n_batches):
+        nc, npt = int(prob.n_compute_c_batches[b]), int(prob.n_point_batches[b])
+        ncp = (nc + 3) // 4 * 4
+        if ncp == nc or npt == 0:
+            continue
+        blk = out[k * prob.wave_offset[b]:k * prob.wave_offset[b + 1]].reshape((ncp, k, npt), order="F")
+        blk[nc:] = np.nan
+    return out
+
+
+# ----------------------------------------------------------------------------------------------
+# Far field
+# ----------------------------------------------------------------------------------------------
+
+@dataclass
+class FarProblem:
+    """Inputs of the far-field branch of sum_up_whole_potential (sum_up_whole_potential.h:3-48)."""
+    n_atoms: int
+    n_full_points: int
+    n_valid_points: int
+    l_pot_max: int
+    l_max_analytic_multipole: int
+    hartree_force_l_add: int
+    n_tile: int
+    coords_center: np.ndarray                 # (3, n_centers) F
+    coord_points: np.ndarray                  # (3, n_full_points) F
+    i_valid_point_2_i_full_points_map: np.ndarray  # int32 (n_valid_points,)
+    species: np.ndarray                       # int32 (n_atoms,) 1-based
+    l_hartree: np.ndarray                     # int32 (n_species,)
+    centers_hartree_potential: np.ndarray     # int32 1-based
+    center_to_atom: np.ndarray                # int32 1-based
+    l_hartree_max_far_distance: np.ndarray    # int32 (n_atoms, T) F
+    multipole_radius_sq: np.ndarray           # (n_atoms, T) F
+    outer_potential_radius: np.ndarray        # (l_pot_max+1, n_atoms, T) F
+    multipole_moments: np.ndarray             # ((l_pot_max+1)^2, n_atoms, T) F
+    tables: dict = field(default_factory=dict)
+
+    @property
+    def n_centers(self):
+        return self.coords_center.shape[1]
+
+
+def moments(l_max, l_pot_max, n_atoms, n_tile, seed):
+    rng = np.random.default_rng(seed)
+    M = np.zeros(((l_pot_max + 1) ** 2, n_atoms, n_tile), order="F")
+    fact = np.array([float(np.prod(np.arange(1, l + 2))) for l in range(l_pot_max + 1)])
+    for l in range(l_max + 1):
+        for m in range(-l, l + 1):
+            M[l * l + l + m] = rng.standard_normal((n_atoms, n_tile)) / fact[l]
+    return M
+
+
+def make_far_problem(n_atoms, box_ang, n_points, l_max, config_k=2, min_dist_bohr=8.0, all_far=True,
+                     n_tile=1, outer_radius=None, xyz=None, point_margin_bohr=8.0):
+    """Far-field inputs.
+
+    all_far: points rejected unless >= min_dist_bohr from every atom (the far-field-only case);
+    otherwise uniform in the padded box, pairs closer than min_dist_bohr are near-field and skipped.
+    outer_radius: None -> 1e30 (no l reduction); 'decreasing' -> a per-atom table that lowers l
+    with distance, exercising the reference's l-reduction loop.
+    """
+    sd = seeds(config_k)
+    if xyz is None:
+        xyz = make_geometry(n_atoms, box_ang * ANG, 1.5 * ANG, sd["geometry"])
+    rng = np.random.default_rng(sd["grid"])
+    lo = xyz.min(axis=0) - point_margin_bohr
+    hi = xyz.max(axis=0) + point_margin_bohr
+    pts = []
+    need = n_points
+    while need > 0:
+        cand = rng.uniform(lo, hi, size=(max(2 * need, 1024), 3))
+        if all_far:
+            ok = np.ones(len(cand), bool)
+            for s in range(0, n_atoms, 64):
+                d2 = np.sum((cand[:, None, :] - xyz[None, s:s + 64, :]) ** 2, axis=2)
+                ok &= np.all(d2 >= min_dist_bohr ** 2, axis=1)
+            cand = cand[ok]
+        pts.append(cand[:need])
+        need -= len(pts[-1])
+    pts = np.concatenate(pts)
+    # a few invalid full points between the valid ones: the map is a strided subset
+    n_full = n_points + n_points // 16
+    full_idx = np.sort(rng.choice(n_full, size=n_points, replace=False)).astype(np.int32)
+    coord_points = np.zeros((3, n_full), order="F")
+    coord_points[:, full_idx] = pts.T
+    l_pot_max = l_max
+    lama = l_pot_max + 1
+    tabs = far_tables(lama)
+    M = moments(l_max, l_pot_max, n_atoms, n_tile, sd["p1"])
+    opr = np.full((l_pot_max + 1, n_atoms, n_tile), 1e30, order="F")
+    if outer_radius == "decreasing":
+        base = rng.uniform(15.0, 40.0, size=(n_atoms, n_tile))
+        for l in range(l_pot_max + 1):
+            opr[l] = (base * (1.0 + 0.35 * (l_pot_max - l))) ** 2
+    return FarProblem(
+        n_atoms=n_atoms, n_full_points=n_full, n_valid_points=n_points, l_pot_max=l_pot_max,
+        l_max_analytic_multipole=lama, hartree_force_l_add=1, n_tile=n_tile,
+        coords_center=np.asfortranarray(xyz.T), coord_points=coord_points,
+        i_valid_point_2_i_full_points_map=full_idx, species=np.ones(n_atoms, np.int32),
+        l_hartree=np.array([l_max], np.int32),
+        centers_hartree_potential=np.arange(1, n_atoms + 1, dtype=np.int32),
+        center_to_atom=np.arange(1, n_atoms + 1, dtype=np.int32),
+        l_hartree_max_far_distance=np.full((n_atoms, n_tile), l_max, np.int32, order="F"),
+        multipole_radius_sq=np.full((n_atoms, n_tile), min_dist_bohr ** 2, order="F"),
+        outer_potential_radius=opr, multipole_moments=M, tables=tabs)

@@ -1,0 +1,234 @@
+/*
+ * fast_rpa_eval_amd: MI355X (gfx950) implementation of the two DFPT grid hot paths of
+ * fast_rpa_eval -- the first-order density rho1 (evaluate_first_order_rho) and the far-field
+ * Hartree multipole sum (far branch of sum_up_whole_potential) -- plus their adjoints.
+ *
+ * Two layers, both C linkage, plain pointers and sizes:
+ *
+ *  1. Reference entry points (Fortran calling convention: trailing underscore, scalars by pointer,
+ *     column-major arrays, the reference's index bases).  Same names and argument lists as the
+ *     reference; arrays are HOST pointers.  Errors print and exit(1), as the reference's DEV_CHECK.
+ *
+ *  2. The frpa_* C-ABI: DEVICE pointers, an explicit HIP stream (hipStream_t passed as void*,
+ *     NULL = default stream), int status return (FRPA_OK = 0), frpa_last_error() for the message.
+ *     Calls are asynchronous on the stream unless stated.  Plans hold the batch/centre metadata
+ *     resident in HBM; a plan is not thread-safe, distinct plans are independent.
+ *
+ * Array conventions (both layers): Fortran column-major; i_basis lists and centre/atom/species
+ * indices 1-based; the valid -> full point map 0-based; wave_batches_compress holds batch b at
+ * offset wave_offset[b] as (nc_pad, np) with nc_pad = ceil(nc / 4) * 4 (rows nc..nc_pad-1 are never
+ * read); ATOM_TILE_SIZE of the reference is the run-time argument atom_tile_size (T).
+ */
+#ifndef FRPA_H
+#define FRPA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------ */
+/* Status                                                                                     */
+/* ------------------------------------------------------------------------------------------ */
+#define FRPA_OK 0
+#define FRPA_ERR_INVALID 1     /* bad argument or inconsistent sizes */
+#define FRPA_ERR_HIP 2         /* HIP runtime error */
+#define FRPA_ERR_UNSUPPORTED 3 /* valid input outside what this build implements */
+#define FRPA_ERR_STATE 4       /* call order (e.g. reference entry before init_dfpt_device_data) */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *frpa_last_error(void);
+/* Library version string. */
+const char *frpa_version(void);
+
+/* ------------------------------------------------------------------------------------------ */
+/* rho1: first_order_rho(map(p), t) = sum_ij phi_i(p) P1(ib_i, ib_j, t) phi_j(p) + bias          */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct frpa_rho1_plan frpa_rho1_plan;
+
+/* Precision of the rho1 / adjoint GEMM. */
+#define FRPA_PREC_FP64 0   /* FP64 MFMA, FP64 accumulation                                    */
+#define FRPA_PREC_OZAKI 1  /* INT8-MFMA split (exact int32 slice products, FP64 recombination)  */
+
+/*
+ * Batch metadata (host arrays, copied into the plan):
+ *   n_point_batches[n_batches], n_point_batches_prefix_sum[n_batches + 1],
+ *   i_valid_point_2_i_full_points_map[prefix[n_batches]] (0-based full-grid index),
+ *   n_compute_c_batches[n_batches],
+ *   ibasis_ptr[n_batches + 1] + ibasis[ibasis_ptr[n_batches]]: batch b's 1-based basis indices
+ *     (the non-padded prefix of the reference's i_basis_batches(:, b)),
+ *   wave_offset[n_batches + 1]: 64-bit offsets of each batch block in wave_batches_compress.
+ * Replaces the rho1 part of init_dfpt_device_data (fast_rap_src_dfpt_gpu_eval/device_data.cu:10-361).
+ */
+int frpa_rho1_plan_create(frpa_rho1_plan **plan, int n_batches, int n_basis, int n_full_points, int n_max_batch_size,
+                          const int *n_point_batches, const int *n_point_batches_prefix_sum,
+                          const int *i_valid_point_2_i_full_points_map, const int *n_compute_c_batches,
+                          const long long *ibasis_ptr, const int *ibasis, const long long *wave_offset);
+int frpa_rho1_plan_destroy(frpa_rho1_plan *plan);
+
+/*
+ * Forward (fast_rap_src_dfpt_gpu_eval/evaluate_first_order_rho.cu:188-371, same maths as
+ * evaluate_first_order_rho.cpp:113-209).  Device arrays:
+ *   wave                       wave_batches_compress
+ *   density_matrix             first_order_density_matrix (n_basis, n_basis, T)
+ *   rho_bias (nullable)        first_order_rho_bias_part2 (n_max_batch_size, n_batches, T)
+ *   first_order_rho            (n_full_points, T): zeroed, then batch points of t < n_tile written
+ *   gradient_wave (nullable)   gradient_wave_batches_compress (3 * wave_offset[n_batches])
+ *   grad_bias (nullable)       first_order_gradient_rho_bias_batches_atoms (3, nmb, n_batches, T)
+ *   first_order_gradient_rho (nullable, required with gradient_wave) (3, nmb, n_batches, T)
+ * n_tile = j_atom_end - j_atom_begin + 1 (<= T).
+ */
+int frpa_rho1_forward(frpa_rho1_plan *plan, int atom_tile_size, int n_tile, int precision, const double *wave,
+                      const double *density_matrix, const double *rho_bias, double *first_order_rho,
+                      const double *gradient_wave, const double *grad_bias, double *first_order_gradient_rho,
+                      void *stream);
+
+/*
+ * Adjoint of the forward with respect to P1 (LDA first-order H integration,
+ * fast_rap_src_dfpt_gpu_eval/evaluate_first_order_H.cpp:144-215 with the GGA terms zero):
+ *   first_order_H(ib_i, ib_j, t) += sum_{batches, p} g(map(p), t) phi_i(p) phi_j(p)
+ * g (n_full_points, T) device; first_order_H (n_basis, n_basis, T) device, ACCUMULATED into.
+ * With g = partition_tab * (v_hartree_gradient + v2rho2 * rho1) + H_prefactor1 this is the
+ * reference's LDA H1; with g = d L / d rho1 it is d L / d P1.
+ */
+int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, const double *wave, const double *g,
+                      double *first_order_H, void *stream);
+
+/* One-off conversion of wave_batches_compress into the INT8 slices used by FRPA_PREC_OZAKI
+ * (kept in the plan; reused across calls while wave is unchanged).  Optional: the first
+ * OZAKI call converts on demand.  Returns the bytes held in *bytes (nullable). */
+int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, const double *wave, long long *bytes, void *stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Far-field Hartree multipole sum                                                            */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct frpa_far_plan frpa_far_plan;
+
+#define FRPA_FAR_TERMS 0   /* term-by-term Cartesian sum, as the reference (plain FP64 baseline) */
+#define FRPA_FAR_HORNER 1  /* merged (i,j,k,nn) coefficients, nested Horner; FP64 throughout      */
+
+/*
+ * Geometry and tables (host arrays, copied).  Arguments keep the reference's names and layouts
+ * (fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.h:3-48):
+ *   species[n_atoms], l_hartree[n_species], centers_hartree_potential[n_centers_hartree_potential],
+ *   center_to_atom[n_centers], coords_center (3, n_centers), index_cc (index_cc_dim_0, 6),
+ *   cc[index_cc_dim_0], n_cc_lm_ijk[l_max_analytic_multipole + 1],
+ *   index_ijk_max_cc (3, l_max_analytic_multipole + 1).
+ * l_pot_max <= 10.
+ */
+int frpa_far_plan_create(frpa_far_plan **plan, int l_pot_max, int l_max_analytic_multipole, int index_cc_dim_0,
+                         int n_centers, int n_centers_hartree_potential, int n_atoms, int n_species,
+                         int hartree_force_l_add, const int *species, const int *l_hartree,
+                         const int *centers_hartree_potential, const int *center_to_atom,
+                         const double *coords_center, const int *index_cc, const double *cc,
+                         const int *n_cc_lm_ijk, const int *index_ijk_max_cc);
+int frpa_far_plan_destroy(frpa_far_plan *plan);
+
+/*
+ * delta_v_hartree(p, t) += sum_centres far-field term, for the valid points, t < n_tile
+ * (far branch of sum_up_whole_potential.cpp:382-503; pairs with dist^2 < multipole_radius_sq are
+ * the near field and are not computed here).  Device arrays:
+ *   coord_points (3, n_full_points), i_valid_point_2_i_full_points_map[n_valid_points],
+ *   l_hartree_max_far_distance (n_atoms, T) int, multipole_radius_sq (n_atoms, T),
+ *   outer_potential_radius (l_pot_max + 1, n_atoms, T), multipole_moments ((l_pot_max+1)^2, n_atoms, T),
+ *   delta_v_hartree (n_full_points, T) ACCUMULATED into.
+ */
+int frpa_far_forward(frpa_far_plan *plan, int atom_tile_size, int n_tile, int variant, int n_full_points,
+                     int n_valid_points, const double *coord_points, const int *i_valid_point_2_i_full_points_map,
+                     const int *l_hartree_max_far_distance, const double *multipole_radius_sq,
+                     const double *outer_potential_radius, const double *multipole_moments,
+                     double *delta_v_hartree, void *stream);
+
+/*
+ * Adjoint with respect to the multipole moments:
+ *   grad_moments(lm, a, t) += sum_points gV(p, t) d delta_v(p, t) / d M(lm, a, t)
+ * gV (n_full_points, T) device; grad_moments ((l_pot_max+1)^2, n_atoms, T) device, ACCUMULATED.
+ * Moments with |M| <= 1e-10 are treated as the reference treats them (their terms dropped), so
+ * their gradient entries receive 0.
+ */
+int frpa_far_adjoint(frpa_far_plan *plan, int atom_tile_size, int n_tile, int n_full_points, int n_valid_points,
+                     const double *coord_points, const int *i_valid_point_2_i_full_points_map,
+                     const int *l_hartree_max_far_distance, const double *multipole_radius_sq,
+                     const double *outer_potential_radius, const double *multipole_moments, const double *gV,
+                     double *grad_moments, void *stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Synthetic wave generation on the device (benchmark inputs too large to build on the host)   */
+/* ------------------------------------------------------------------------------------------ */
+/*
+ * wave(off_b + i + nc_pad * p) = exp(-alpha_mu |r_p - R_a|^2) Y_{l_mu m_mu}(r^) for mu = ibasis(i) - 1,
+ * padding rows set to NaN.  basis_atom/basis_l/basis_m/basis_alpha (n_basis) and atom_xyz (3, n_atoms)
+ * are host arrays; coord_points (3, n_full_points) is a device array.  l <= 2.
+ */
+int frpa_synth_wave(frpa_rho1_plan *plan, int n_atoms, const double *atom_xyz, const int *basis_atom,
+                    const int *basis_l, const int *basis_m, const double *basis_alpha, const double *coord_points,
+                    double *wave, void *stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Reference entry points (drop-in names; host arrays; exit(1) on error)                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* fast_rap_src_dfpt_gpu_eval/device.hpp:191-192 (C++ linkage there; C linkage here). */
+void init_dfpt_device_info(void);
+void free_dfpt_device_info(void);
+
+/* fast_rap_src_dfpt_gpu_eval/device_data.hpp:159-227 (C++ linkage there; C linkage here).
+ * Uploads the constant arrays once and builds the rho1 and far-field plans.  The reference's
+ * compile-time ATOM_TILE_SIZE is taken from the environment variable FRPA_ATOM_TILE_SIZE
+ * (default 1), since this signature has no argument for it. */
+void init_dfpt_device_data(
+    int *n_my_batches_work_, int *n_full_points_, int *n_valid_points_, int *n_basis_, int *n_atoms_,
+    int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_, int *l_pot_max_, int *n_max_radial_,
+    int *n_hartree_grid_, int *n_centers_, int *n_centers_hartree_potential_, int *hartree_force_l_add_,
+    int *n_species_, int *l_max_analytic_multipole_, int *index_cc_dim_0_,
+    int *basis_atom_ptr, int *batch_sizes_ptr, int *n_point_batches_ptr, int *n_point_batches_prefix_sum_ptr,
+    int *i_valid_point_2_i_full_points_map_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *i_batch_2_wave_offset_ptr, double *wave_batches_compress_ptr,
+    double *gradient_wave_batches_compress_ptr, double *density_matrix_ptr,
+    int *n_radial_ptr, int *species_ptr, int *species_center_ptr, double *coord_points_ptr, int *l_hartree_ptr,
+    int *n_grid_ptr, int *n_cc_lm_ijk_ptr, int *centers_hartree_potential_ptr, int *center_to_atom_ptr,
+    int *index_cc_ptr, int *index_ijk_max_cc_ptr, double *cc_ptr, double *coords_center_ptr,
+    double *r_grid_min_ptr, double *log_r_grid_inc_ptr, double *scale_radial_ptr, double *partition_tab_ptr,
+    double *first_order_density_matrix_ptr, double *first_order_rho_ptr, double *first_order_rho_bias_part2_ptr,
+    double *first_order_gradient_rho_ptr, double *first_order_gradient_rho_bias_batches_atoms_ptr,
+    int *l_hartree_max_far_distance_ptr, double *multipole_radius_sq_ptr, double *outer_potential_radius_ptr,
+    double *multipole_moments_ptr, double *current_delta_v_hart_part_spl_tile_ptr, double *delta_v_hartree_ptr);
+void free_dfpt_device_data(void);
+
+/* fast_rap_src_dfpt_gpu_eval/evaluate_first_order_rho.h:37-69.  Runs on the device-resident
+ * batch data of init_dfpt_device_data; uploads first_order_density_matrix and the bias arrays from
+ * the given host pointers and downloads first_order_rho and first_order_gradient_rho into them
+ * (the reference's non-ALL_KERNELS_ON_GPU mode, evaluate_first_order_rho.cu:240-244,362-366).
+ * Precision: FP64, or the INT8 split when FRPA_RHO1_PRECISION=ozaki. */
+void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
+    int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
+    int *n_basis_, int *n_atoms_, int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_,
+    int *batch_sizes_ptr, int *n_point_batches_ptr, int *n_point_batches_prefix_sum_ptr,
+    int *i_valid_point_2_i_full_points_map_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *i_batch_2_wave_offset_ptr, double *wave_batches_compress_ptr,
+    double *gradient_wave_batches_compress_ptr, double *first_order_density_matrix_ptr, double *first_order_rho_ptr,
+    double *first_order_rho_bias_part2_ptr, double *first_order_gradient_rho_ptr,
+    double *first_order_gradient_rho_bias_batches_atoms_ptr, int *basis_atom_ptr, double *partition_tab_ptr,
+    double *density_matrix_ptr);
+
+/* fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.h:50-95 (C++ linkage there; C linkage here).
+ * Far-field branch only: uploads the per-call arrays, accumulates into delta_v_hartree and
+ * synchronises, as the reference (sum_up_whole_potential.cu:641-769). */
+void sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_(
+    const int j_atom_begin, const int j_atom_end, const int n_full_points, const int l_max_analytic_multipole,
+    const int index_cc_dim_0, const int n_valid_points, const int l_pot_max, const int n_max_radial,
+    const int n_hartree_grid, const int n_centers, const int n_centers_hartree_potential, const int n_atoms,
+    const int hartree_force_l_add, const int n_species, const int *n_radial_ptr, const int *species_ptr,
+    const int *species_center_ptr, const int *l_hartree_max_far_distance_ptr,
+    const int *i_valid_point_2_i_full_points_map_ptr, const double *coord_points_ptr,
+    const double *multipole_radius_sq_ptr, const double *outer_potential_radius_ptr,
+    const double *multipole_moments_ptr, const double *current_delta_v_hart_part_spl_tile_ptr,
+    const int *l_hartree_ptr, const int *n_grid_ptr, const int *n_cc_lm_ijk_ptr,
+    const int *centers_hartree_potential_ptr, const int *center_to_atom_ptr, const int *index_cc_ptr,
+    const int *index_ijk_max_cc_ptr, const double *cc_ptr, const double *coords_center_ptr,
+    const double *r_grid_min_ptr, const double *log_r_grid_inc_ptr, const double *scale_radial_ptr,
+    const double *partition_tab_ptr, double *delta_v_hartree_ptr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FRPA_H */

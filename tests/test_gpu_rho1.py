@@ -1,0 +1,113 @@
+"""rho1 forward on the GPU (through the C-ABI) against the CPU FP64 oracle.
+
+Bar: FP64 path within 1e-12 of max|rho1| (and the reference's own 1e-6 absolute check,
+fast_rap_src_dfpt_gpu_eval/direct_test/evaluate_first_order_rho_run.cpp:206).
+"""
+import numpy as np
+import pytest
+import torch
+
+from fast_rpa_eval_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _bias(prob, T, seed):
+    rng = np.random.default_rng(seed)
+    b = np.asfortranarray(rng.uniform(-1, 1, size=(prob.n_max_batch_size, prob.n_batches, T)))
+    g = np.asfortranarray(rng.uniform(-1, 1, size=(3, prob.n_max_batch_size, prob.n_batches, T)))
+    return b, g
+
+
+def _run_gpu(prob, P1, T, n_tile, bias=None, gwave=None, gbias=None, precision=api.FP64):
+    plan = api.Rho1Plan.from_problem(prob)
+    wave = api.to_device(prob.wave)
+    P1d = api.to_device(np.asfortranarray(P1))
+    args = dict(T=T, n_tile=n_tile, precision=precision,
+                rho_bias=api.to_device(bias) if bias is not None else None)
+    if gwave is not None:
+        args.update(gradient_wave=api.to_device(gwave), grad_bias=api.to_device(gbias))
+        rho, grad = plan.forward(wave, P1d, **args)
+        torch.cuda.synchronize()
+        return (rho.cpu().numpy().reshape((prob.n_full_points, T), order="F"),
+                grad.cpu().numpy().reshape((3, prob.n_max_batch_size, prob.n_batches, T), order="F"))
+    rho = plan.forward(wave, P1d, **args)
+    torch.cuda.synchronize()
+    return rho.cpu().numpy().reshape((prob.n_full_points, T), order="F"), None
+
+
+def _check(got, ref, rel=1e-12, what="rho1"):
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = np.abs(got - ref).max()
+    print(f"[{what}] max|err| {err:.3e}, max|ref| {scale:.3e}, rel {err / scale:.3e}", flush=True)
+    assert np.all(np.isfinite(got)), f"{what}: non-finite values"
+    assert err <= rel * scale, f"{what}: max|err| {err:.3e} > {rel:g} * {scale:.3e}"
+    assert err <= 1e-6
+
+
+def test_rho1_small_cpu_config_fp64(cuda, oracle):
+    """8 atoms, 112 basis functions, 64 batches x 100 points (natural compute sets), with bias and GGA."""
+    prob = synth.make_rho1_problem(8, 10.0, 64, 100, None, config_k=1)
+    P1 = synth.make_p1(prob.atom_xyz, prob.basis, 3001)
+    bias, gbias = _bias(prob, 1, 7)
+    gwave = synth.random_like_wave(prob, 3, 8)
+    ref, gref = oracle.rho1(prob, P1, T=1, bias=bias, gwave=gwave, gbias=gbias)
+    got, ggot = _run_gpu(prob, P1, 1, 1, bias, gwave, gbias)
+    _check(got, ref)
+    _check(ggot, gref, what="grad rho1")
+
+
+def test_rho1_uneven_batches_fp64(cuda, oracle):
+    """Uneven nc (incl. 0) and np (incl. 0 and > 208, the per-pass point chunk), nonsymmetric P1."""
+    prob = synth.make_rho1_problem(24, 14.0, 90, (1, 230), (1, 180), config_k=3, zero_nc=(3, 40), zero_np=(7,))
+    rng = np.random.default_rng(11)
+    P1 = np.asfortranarray(rng.uniform(-1, 1, size=(prob.n_basis, prob.n_basis, 1)))
+    assert (prob.n_point_batches > 208).any() and (prob.n_compute_c_batches == 0).sum() == 2
+    assert (prob.n_point_batches == 0).sum() == 1
+    ref, _ = oracle.rho1(prob, P1, T=1)
+    got, _ = _run_gpu(prob, P1, 1, 1)
+    _check(got, ref)
+
+
+def test_rho1_atom_tile_partial(cuda, oracle):
+    """T = 3 perturbations, last tile short (n_tile = 2): column 2 of rho stays zero, grad has all T."""
+    prob = synth.make_rho1_problem(10, 11.0, 40, (20, 120), (10, 90), config_k=4)
+    rng = np.random.default_rng(5)
+    T = 3
+    P1 = np.asfortranarray(rng.uniform(-1, 1, size=(prob.n_basis, prob.n_basis, T)))
+    bias, gbias = _bias(prob, T, 9)
+    gwave = synth.random_like_wave(prob, 3, 10)
+    ref, gref = oracle.rho1(prob, P1, T=T, bias=bias, gwave=gwave, gbias=gbias, j_atom_begin=4, j_atom_end=5)
+    got, ggot = _run_gpu(prob, P1, T, 2, bias, gwave, gbias)
+    assert np.all(got[:, 2] == 0.0)
+    _check(got, ref)
+    _check(ggot, gref, what="grad rho1")
+
+
+def test_rho1_linearity_and_rank_one(cuda):
+    """Size-independent properties: linear in P1; P1 = u u^T gives (u . phi)^2 >= 0."""
+    prob = synth.make_rho1_problem(12, 12.0, 60, (40, 160), (30, 150), config_k=5)
+    rng = np.random.default_rng(3)
+    nb = prob.n_basis
+    A = np.asfortranarray(rng.standard_normal((nb, nb, 1)))
+    B = np.asfortranarray(rng.standard_normal((nb, nb, 1)))
+    ra, _ = _run_gpu(prob, A, 1, 1)
+    rb, _ = _run_gpu(prob, B, 1, 1)
+    rab, _ = _run_gpu(prob, np.asfortranarray(2.0 * A - 0.5 * B), 1, 1)
+    _check(rab, 2.0 * ra - 0.5 * rb, rel=1e-12, what="linearity")
+    u = rng.standard_normal(nb)
+    r1, _ = _run_gpu(prob, np.asfortranarray(np.outer(u, u)[:, :, None]), 1, 1)
+    valid = prob.i_valid_point_2_i_full_points_map[:prob.n_point_batches_prefix_sum[-1]]
+    has = np.repeat(prob.n_compute_c_batches > 0, prob.n_point_batches)
+    assert np.all(r1[valid[has], 0] >= -1e-12 * np.abs(r1).max())
+
+
+def test_synth_wave_on_device_matches_host(cuda):
+    prob = synth.make_rho1_problem(8, 10.0, 20, (30, 90), (20, 112), config_k=6)
+    plan = api.Rho1Plan.from_problem(prob)
+    w = plan.synth_wave(prob.atom_xyz, prob.basis, api.to_device(prob.coord_points),
+                        int(prob.wave_offset[-1])).cpu().numpy()
+    ref = prob.wave
+    fin = np.isfinite(ref)
+    assert np.array_equal(fin, np.isfinite(w))
+    np.testing.assert_allclose(w[fin], ref[fin], rtol=1e-12, atol=1e-14)

@@ -1,0 +1,132 @@
+"""Pins of the rho1 oracle (CPU; restatement of evaluate_first_order_rho.cpp:12-210).
+
+Checks that do not retype the oracle's loops: a brute-force sum_ij phi_i P_ij phi_j per point,
+closed forms for rank-one and unit P1, linearity, the scatter map, the zero/bias/gradient semantics
+against an einsum restatement on tiny inputs, NaN padding rows, and a planted transposition that
+the brute-force check must catch.
+"""
+import numpy as np
+import pytest
+
+from fast_rpa_eval_amd import synth
+
+
+@pytest.fixture(scope="module")
+def prob():
+    return synth.make_rho1_problem(8, 10.0, 64, 100, None, config_k=1)
+
+
+def _batch_phi(prob, b):
+    s, e = prob.n_point_batches_prefix_sum[b], prob.n_point_batches_prefix_sum[b + 1]
+    ib = prob.ibasis[prob.ibasis_ptr[b]:prob.ibasis_ptr[b + 1]] - 1
+    pts = prob.coord_points[:, prob.i_valid_point_2_i_full_points_map[s:e]].T
+    return ib, synth.eval_basis(prob.atom_xyz, prob.basis, ib, pts), prob.i_valid_point_2_i_full_points_map[s:e]
+
+
+def brute(prob, P):
+    out = np.zeros(prob.n_full_points)
+    cond = np.zeros(prob.n_full_points)
+    for b in range(prob.n_batches):
+        if prob.n_compute_c_batches[b] == 0:
+            continue
+        ib, phi, mp = _batch_phi(prob, b)
+        Pl = P[np.ix_(ib, ib)]
+        out[mp] = np.einsum("ip,ij,jp->p", phi, Pl, phi)
+        cond[mp] = np.einsum("ip,ij,jp->p", np.abs(phi), np.abs(Pl), np.abs(phi))
+    return out, cond
+
+
+def test_wave_padding_is_nan_and_blocks_are_consistent(prob):
+    ncp = (prob.n_compute_c_batches + 3) // 4 * 4
+    assert np.array_equal(np.diff(prob.wave_offset), ncp.astype(np.int64) * prob.n_point_batches)
+    nan_rows = sum(int((c - n) * p) for c, n, p in zip(ncp, prob.n_compute_c_batches, prob.n_point_batches))
+    assert np.isnan(prob.wave).sum() == nan_rows > 0
+
+
+def test_rho1_equals_brute_force(prob, oracle):
+    P1 = synth.make_p1(prob.atom_xyz, prob.basis, 3001)
+    rho, _ = oracle.rho1(prob, P1)
+    ref, cond = brute(prob, P1[:, :, 0])
+    valid = cond > 0
+    assert np.max(np.abs(rho[valid, 0] - ref[valid]) / cond[valid]) < 1e-14
+    assert np.all(rho[~valid, 0] == 0.0)  # invalid full points are never written
+
+
+def test_rho1_nonsymmetric_and_planted_transpose(prob, oracle):
+    rng = np.random.default_rng(4)
+    P = rng.uniform(-1, 1, size=(prob.n_basis, prob.n_basis))
+    rho, _ = oracle.rho1(prob, np.asfortranarray(P[:, :, None]))
+    ref, cond = brute(prob, P)
+    v = cond > 0
+    assert np.max(np.abs(rho[v, 0] - ref[v]) / cond[v]) < 1e-14
+    # phi^T P phi is invariant under P -> P^T, so transposition must be caught through the GGA sum,
+    # which is 2 sum_i grad_phi_i (P phi)_i and does change: the einsum check below pins operand order.
+    gw = synth.random_like_wave(prob, 3, 2)
+    _, g = oracle.rho1(prob, np.asfortranarray(P[:, :, None]), gwave=gw)
+    _, gT = oracle.rho1(prob, np.asfortranarray(P.T[:, :, None]), gwave=gw)
+    b = 5
+    ib, phi, _ = _batch_phi(prob, b)
+    nc, npt = len(ib), prob.n_point_batches[b]
+    ncp = (nc + 3) // 4 * 4
+    G = gw[3 * prob.wave_offset[b]:3 * prob.wave_offset[b + 1]].reshape((ncp, 3, npt), order="F")[:nc]
+    ref_g = 2.0 * np.einsum("icp,ij,jp->cp", G, P[np.ix_(ib, ib)], phi)
+    np.testing.assert_allclose(g[:, :npt, b, 0], ref_g, rtol=1e-12, atol=1e-12)
+    assert np.max(np.abs(gT[:, :npt, b, 0] - ref_g)) > 1e-3  # the planted transpose is visible
+
+
+def test_rho1_rank_one_and_unit_closed_forms(prob, oracle):
+    rng = np.random.default_rng(9)
+    u = rng.standard_normal(prob.n_basis)
+    rho, _ = oracle.rho1(prob, np.asfortranarray(np.outer(u, u)[:, :, None]))
+    for b in range(0, prob.n_batches, 7):
+        ib, phi, mp = _batch_phi(prob, b)
+        np.testing.assert_allclose(rho[mp, 0], (u[ib] @ phi) ** 2, rtol=1e-12, atol=1e-15)
+    # P = e_i e_j^T: rho = phi_i phi_j at points of batches containing both
+    b = 3
+    ib, phi, mp = _batch_phi(prob, b)
+    E = np.zeros((prob.n_basis, prob.n_basis))
+    E[ib[2], ib[-1]] = 1.0
+    rho, _ = oracle.rho1(prob, np.asfortranarray(E[:, :, None]))
+    np.testing.assert_array_equal(rho[mp, 0], phi[2] * phi[-1])
+
+
+def test_rho1_linearity(prob, oracle):
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((prob.n_basis, prob.n_basis, 1))
+    B = rng.standard_normal((prob.n_basis, prob.n_basis, 1))
+    ra, _ = oracle.rho1(prob, np.asfortranarray(A))
+    rb, _ = oracle.rho1(prob, np.asfortranarray(B))
+    rc, _ = oracle.rho1(prob, np.asfortranarray(3 * A - B))
+    np.testing.assert_allclose(rc, 3 * ra - rb, rtol=1e-12, atol=1e-12)
+
+
+def test_rho1_bias_tiles_and_empty_batches(oracle):
+    """T = 2 with a one-atom tile: rho column 1 stays zero, grad gets both; bias added; nc = 0 and
+    np = 0 batches write nothing."""
+    p = synth.make_rho1_problem(6, 9.0, 12, (5, 40), (3, 30), config_k=9, zero_nc=(2,), zero_np=(4,))
+    rng = np.random.default_rng(3)
+    T = 2
+    P = np.asfortranarray(rng.standard_normal((p.n_basis, p.n_basis, T)))
+    bias = np.asfortranarray(rng.standard_normal((p.n_max_batch_size, p.n_batches, T)))
+    gbias = np.asfortranarray(rng.standard_normal((3, p.n_max_batch_size, p.n_batches, T)))
+    gw = synth.random_like_wave(p, 3, 5)
+    rho, grad = oracle.rho1(p, P, T=T, bias=bias, gwave=gw, gbias=gbias, j_atom_begin=7, j_atom_end=7)
+    assert np.all(rho[:, 1] == 0.0)
+    for b in range(p.n_batches):
+        s, e = p.n_point_batches_prefix_sum[b], p.n_point_batches_prefix_sum[b + 1]
+        mp = p.i_valid_point_2_i_full_points_map[s:e]
+        nc, npt = p.n_compute_c_batches[b], p.n_point_batches[b]
+        if nc == 0:
+            assert np.all(rho[mp, 0] == 0.0) and np.all(grad[:, :, b, :] == 0.0)
+            continue
+        ib = p.ibasis[p.ibasis_ptr[b]:p.ibasis_ptr[b + 1]] - 1
+        ncp = (nc + 3) // 4 * 4
+        phi = p.wave[p.wave_offset[b]:p.wave_offset[b + 1]].reshape((ncp, npt), order="F")[:nc]
+        G = gw[3 * p.wave_offset[b]:3 * p.wave_offset[b + 1]].reshape((ncp, 3, npt), order="F")[:nc]
+        for t in range(T):
+            W = P[np.ix_(ib, ib, [t])][:, :, 0] @ phi
+            ref = np.einsum("ip,ip->p", phi, W) + bias[:npt, b, t]
+            if t == 0:
+                np.testing.assert_allclose(rho[mp, 0], ref, rtol=1e-12, atol=1e-12)
+            gref = 2 * np.einsum("icp,ip->cp", G, W) - gbias[:, :npt, b, t]
+            np.testing.assert_allclose(grad[:, :npt, b, t], gref, rtol=1e-12, atol=1e-12)

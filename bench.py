@@ -1,0 +1,357 @@
+#!/usr/bin/env python
+"""Benchmark: rho1 + far-field Hartree grid points/s on one MI355X, fast path vs plain FP64.
+
+A step is one pass of the hot path over the synthetic workload, with every input resident in HBM:
+  rho1 forward over all batches (evaluate_first_order_rho), its adjoint (the LDA first-order H
+  integration, H1 += sum_p g phi_i phi_j), and the far-field Hartree sum over the far-field points.
+Two paths are timed on the same inputs:
+  fast  : rho1/adjoint with the INT8-split GEMM (FP64 recombination), far field by merged-coefficient
+          Horner (FP64);
+  fp64  : rho1/adjoint with FP64 MFMA, far field term by term (FP64) -- the plain FP64 GPU kernels.
+value = (rho1 points + far-field points) / fast step time.  The JSON line also carries the FP64
+figure, the fast/FP64 ratio, the fast path's error against the FP64 path, the dominant kernel's
+roofline (timed with HIP events on its stream) and a CPU baseline (the CPU FP64 oracle on a bounded
+sample of the same inputs).
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # 1024 atoms (14336 basis functions), 60k batches (50-200 points, n_compute 100-1200), rho1 forward
+    # + adjoint, far field at l_max = 8 over 6M points; P1 cut at 1e-8.
+    "combined_1024atoms_60kbatches_6Mpts_l8": dict(
+        n_atoms=1024, box_ang=50.0, n_batches=60000, np_range=(50, 200), nc_range=(100, 1200),
+        n_far=6_000_000, l_far=8, config_k=5, p1_cut=1e-8),
+    # 512 atoms (7168 basis functions), 20k batches (50-200 points, n_compute 80-800); far field at
+    # l_max = 6 over 200k points.
+    "rho1_512atoms_20kbatches_far200k_l6": dict(
+        n_atoms=512, box_ang=40.0, n_batches=20000, np_range=(50, 200), nc_range=(80, 800),
+        n_far=200_000, l_far=6, config_k=3, p1_cut=0.0),
+    # small smoke-size workload
+    "small_64atoms_2kbatches_far50k_l6": dict(
+        n_atoms=64, box_ang=20.0, n_batches=2000, np_range=(50, 200), nc_range=(80, 400),
+        n_far=50_000, l_far=6, config_k=4, p1_cut=0.0),
+}
+DEFAULT = "combined_1024atoms_60kbatches_6Mpts_l8"
+
+PEAK_FP64 = 78.6      # TFLOP/s, dense FP64 MFMA / vector
+PEAK_INT8 = 5000.0    # TOP/s, dense INT8 MFMA
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def make_inputs(w):
+    import torch
+
+    from fast_rpa_eval_amd import api, synth
+    t0 = time.time()
+    prob = synth.make_rho1_problem(w["n_atoms"], w["box_ang"], w["n_batches"], w["np_range"], w["nc_range"],
+                                   config_k=w["config_k"], fill_wave=False)
+    log(f"batches built: {prob.n_batches} batches, {prob.n_basis} basis, {int(prob.n_point_batches.sum())} points, "
+        f"wave {prob.wave_offset[-1] * 8 / 1e9:.1f} GB ({time.time() - t0:.1f}s)")
+    plan = api.Rho1Plan.from_problem(prob)
+    cp = api.to_device(prob.coord_points)
+    wave = plan.synth_wave(prob.atom_xyz, prob.basis, cp, int(prob.wave_offset[-1]))
+    # P1 symmetric N(0,1) exp(-d_ij / 2 bohr), generated on the device
+    g = torch.Generator(device="cuda").manual_seed(synth.seeds(w["config_k"])["p1"])
+    nb = prob.n_basis
+    A = torch.randn(nb, nb, generator=g, device="cuda", dtype=torch.float64)
+    A = torch.triu(A) + torch.triu(A, 1).T
+    bx = torch.from_numpy(prob.atom_xyz[prob.basis.atom]).cuda()
+    A *= torch.exp(-torch.cdist(bx, bx) / 2.0)
+    if w["p1_cut"] > 0:
+        A[A.abs() < w["p1_cut"]] = 0.0
+    P1 = A.contiguous().view(-1)
+    del A, bx
+    rng = np.random.default_rng(synth.seeds(w["config_k"])["v1"])
+    gfull = api.to_device(rng.standard_normal(prob.n_full_points))  # partition_tab * v1 with w = 1
+    log(f"device inputs ready ({time.time() - t0:.1f}s)")
+    fp = synth.make_far_problem(w["n_atoms"], w["box_ang"], w["n_far"], w["l_far"], config_k=w["config_k"],
+                                all_far=False, xyz=prob.atom_xyz)
+    fplan = api.FarPlan.from_problem(fp)
+    finp = api.FarInputs(fp)
+    log(f"far-field inputs ready: {fp.n_valid_points} points, {fp.n_atoms} centres, l={fp.l_pot_max} "
+        f"({time.time() - t0:.1f}s)")
+    torch.cuda.synchronize()
+    return dict(prob=prob, plan=plan, wave=wave, P1=P1, g=gfull, fp=fp, fplan=fplan, finp=finp)
+
+
+def step_fns(D, prec, adj_prec, far_variant, have_adj):
+    import torch
+
+    from fast_rpa_eval_amd import api
+    prob, plan = D["prob"], D["plan"]
+    rho = torch.empty(prob.n_full_points, dtype=torch.float64, device="cuda")
+    H = torch.zeros(prob.n_basis * prob.n_basis, dtype=torch.float64, device="cuda")
+    dv = torch.zeros(D["fp"].n_full_points, dtype=torch.float64, device="cuda")
+
+    def rho1():
+        plan.forward(D["wave"], D["P1"], precision=prec, out=rho)
+
+    def adj():
+        if have_adj:
+            H.zero_()
+            plan.adjoint(D["wave"], D["g"], precision=adj_prec, H=H)
+
+    def far():
+        dv.zero_()
+        D["fplan"].forward(dv, *D["finp"].args(), variant=far_variant)
+
+    return dict(rho1=rho1, adjoint=adj, far=far), dict(rho=rho, H=H, dv=dv)
+
+
+def time_it(fns, names, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        for n in names:
+            fns[n]()
+    torch.cuda.synchronize()
+    ev = {n: [] for n in names}
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record()
+    for _ in range(steps):
+        for n in names:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fns[n]()
+            b.record()
+            ev[n].append((a, b))
+    t1.record()
+    torch.cuda.synchronize()
+    total = t0.elapsed_time(t1) / steps
+    per = {n: float(np.mean([a.elapsed_time(b) for a, b in ev[n]])) for n in names}
+    return total, per
+
+
+def rho1_flops(prob):
+    nc = prob.n_compute_c_batches.astype(np.float64)
+    npt = prob.n_point_batches.astype(np.float64)
+    sel = (nc > 0) & (npt > 0)
+    return float(np.sum((2 * nc * nc * npt + 2 * nc * npt)[sel]))
+
+
+def rho1_bytes(prob):
+    nc = prob.n_compute_c_batches.astype(np.float64)
+    npt = prob.n_point_batches.astype(np.float64)
+    sel = (nc > 0) & (npt > 0)
+    return float(np.sum((8 * nc * npt + 8 * nc * nc + 4 * nc + 12 * npt)[sel]))
+
+
+def cpu_baseline(D, threads, target_s=15.0):
+    """CPU FP64 oracle on a bounded sample: random batches (rho1) and points (far field)."""
+    import torch
+
+    from fast_rpa_eval_amd import synth
+    from oracle import oracle
+    oracle.set_threads(threads)
+    prob, fp = D["prob"], D["fp"]
+    rng = np.random.default_rng(0)
+    nb = min(256, prob.n_batches)
+    sel = np.sort(rng.choice(prob.n_batches, size=nb, replace=False))
+    sub = subproblem(prob, sel, D["wave"])
+    P1 = D["P1"].view(prob.n_basis, prob.n_basis).T.contiguous().cpu().numpy()  # column-major (symmetric)
+    P1 = np.asfortranarray(P1.T)[:, :, None]
+    t0 = time.time()
+    rho_cpu, _ = oracle.rho1(sub, P1)
+    t_rho = time.time() - t0
+    pts_rho = int(sub.n_point_batches.sum())
+    nfs = min(fp.n_valid_points, max(2000, int(fp.n_valid_points * 0.002)))
+    fsub = synth.FarProblem(**{**fp.__dict__})
+    fsub.i_valid_point_2_i_full_points_map = fp.i_valid_point_2_i_full_points_map[:nfs].copy()
+    fsub.n_valid_points = nfs
+    t0 = time.time()
+    dv_cpu = oracle.far(fsub)
+    t_far = time.time() - t0
+    return dict(rho_pts=pts_rho, rho_s=t_rho, far_pts=nfs, far_s=t_far, sel=sel, sub=sub, rho_cpu=rho_cpu,
+                dv_cpu=dv_cpu, nfs=nfs)
+
+
+def subproblem(prob, sel, wave_dev):
+    """Problem restricted to the batches `sel`, with their wave blocks copied to the host."""
+    import copy
+    sub = copy.copy(prob)
+    nps = prob.n_point_batches[sel]
+    ncs = prob.n_compute_c_batches[sel]
+    ncp = (ncs.astype(np.int64) + 3) // 4 * 4
+    sub.n_point_batches = nps.astype(np.int32)
+    sub.n_compute_c_batches = ncs.astype(np.int32)
+    sub.n_point_batches_prefix_sum = np.concatenate([[0], np.cumsum(nps)]).astype(np.int32)
+    maps = [prob.i_valid_point_2_i_full_points_map[prob.n_point_batches_prefix_sum[b]:prob.n_point_batches_prefix_sum[b + 1]]
+            for b in sel]
+    sub.i_valid_point_2_i_full_points_map = np.concatenate(maps).astype(np.int32)
+    sub.batch_sizes = prob.batch_sizes[sel]
+    sub.ibasis = np.concatenate([prob.ibasis[prob.ibasis_ptr[b]:prob.ibasis_ptr[b + 1]] for b in sel])
+    sub.ibasis_ptr = np.concatenate([[0], np.cumsum(ncs)]).astype(np.int64)
+    sub.wave_offset = np.concatenate([[0], np.cumsum(ncp * nps)]).astype(np.int64)
+    sub.n_max_batch_size = int(max(nps.max(), 1))
+    sub.n_max_compute_ham = int(ncs.max())
+    sub.wave = np.concatenate([wave_dev[int(prob.wave_offset[b]):int(prob.wave_offset[b + 1])].cpu().numpy()
+                               for b in sel])
+    return sub
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--paths", default="fast,fp64")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU; the rho1 and far-field paths would shard by batches/points "
+              "with no exchange (H1 needs one all-reduce), but multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from fast_rpa_eval_amd import api
+    from fast_rpa_eval_amd._lib import FrpaError
+    w = WORKLOADS[a.workload]
+    D = make_inputs(w)
+    prob, fp = D["prob"], D["fp"]
+
+    # capability probes (small, on the real inputs)
+    have_adj = True
+    try:
+        D["plan"].adjoint(D["wave"], D["g"], precision=api.FP64)
+    except FrpaError as e:
+        have_adj = False
+        log(f"adjoint unavailable: {e}")
+    have_ozaki = True
+    try:
+        D["plan"].forward(D["wave"], D["P1"], precision=api.OZAKI)
+    except FrpaError as e:
+        have_ozaki = False
+        log(f"INT8 split unavailable: {e}")
+    have_ozaki_adj = have_adj and have_ozaki
+    if have_ozaki_adj:
+        try:
+            D["plan"].adjoint(D["wave"], D["g"], precision=api.OZAKI)
+        except FrpaError as e:
+            have_ozaki_adj = False
+            log(f"INT8-split adjoint unavailable (fast path uses the FP64 adjoint): {e}")
+    torch.cuda.synchronize()
+
+    names = ["rho1", "adjoint", "far"] if have_adj else ["rho1", "far"]
+    res = {}
+    outs = {}
+    for path in a.paths.split(","):
+        prec = api.OZAKI if (path == "fast" and have_ozaki) else api.FP64
+        adj_prec = api.OZAKI if (path == "fast" and have_ozaki_adj) else api.FP64
+        var = api.FAR_HORNER if path == "fast" else api.FAR_TERMS
+        fns, o = step_fns(D, prec, adj_prec, var, have_adj)
+        total, per = time_it(fns, names, a.steps, a.warmup)
+        res[path] = dict(ms=total, per=per, prec=prec, var=var)
+        outs[path] = {k: v.clone() for k, v in o.items()}
+        log(f"path {path}: {total:.3f} ms/step  " + "  ".join(f"{n} {per[n]:.3f} ms" for n in names))
+
+    pts_rho = int(prob.n_point_batches[prob.n_compute_c_batches > 0].sum())
+    pts = pts_rho + fp.n_valid_points
+    fast = res.get("fast", res.get("fp64"))
+    value = pts / (fast["ms"] / 1e3)
+    line = {
+        "metric": "rho1+Hartree grid points/s (fast path vs plain FP64, max rel err <= 1e-9)",
+        "value": value, "unit": "points/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": fast["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8-split+f64" if (have_ozaki and "fast" in res) else "f64", "data": "synthetic",
+        "config": {"workload": a.workload, "n_atoms": w["n_atoms"], "n_basis": prob.n_basis,
+                   "n_batches": prob.n_batches, "rho1_points": pts_rho, "far_points": fp.n_valid_points,
+                   "l_max_far": w["l_far"], "adjoint": have_adj, "int8_split_rho1": have_ozaki,
+                   "int8_split_adjoint": have_ozaki_adj},
+    }
+    if "fp64" in res:
+        line["fp64_value"] = pts / (res["fp64"]["ms"] / 1e3)
+        line["fp64_ms_per_step"] = res["fp64"]["ms"]
+        line["speedup_vs_fp64"] = res["fp64"]["ms"] / fast["ms"]
+    line["ms_per_kernel"] = {p: r["per"] for p, r in res.items()}
+    if "fast" in outs and "fp64" in outs:
+        r_f, r_d = outs["fast"]["rho"], outs["fp64"]["rho"]
+        line["rho1_rel_err_vs_fp64"] = float((r_f - r_d).abs().max() / r_d.abs().max())
+        v_f, v_d = outs["fast"]["dv"], outs["fp64"]["dv"]
+        line["far_rel_err_vs_fp64"] = float((v_f - v_d).abs().max() / v_d.abs().max())
+        if have_adj:
+            h_f, h_d = outs["fast"]["H"], outs["fp64"]["H"]
+            line["h1_rel_err_vs_fp64"] = float((h_f - h_d).abs().max() / h_d.abs().max())
+
+    # roofline of the dominant kernel of the fast path
+    dom = max(fast["per"], key=fast["per"].get)
+    t = fast["per"][dom] / 1e3
+    if dom in ("rho1", "adjoint"):
+        fl = rho1_flops(prob)
+        if fast["prec"] == api.OZAKI:
+            ach, peak, unit = 15 * fl / t / 1e12, PEAK_INT8, "TOP/s"
+        else:
+            ach, peak, unit = fl / t / 1e12, PEAK_FP64, "TFLOP/s"
+        line["roofline"] = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": peak, "unit": unit,
+                            "frac": ach / peak, "traffic": None,
+                            "algorithmic_bytes_GB": rho1_bytes(prob) / 1e9}
+    else:
+        # far field: executed FP64 FMAs of the merged-Horner form (1 FMA per merged coefficient)
+        from fast_rpa_eval_amd.multipole_tables import horner_fma_count
+        pairs = far_pairs(D)
+        fl = pairs * (2.0 * horner_fma_count(w["l_far"]) + 25) if fast["var"] == api.FAR_HORNER else \
+            pairs * (2.0 * fp.tables["n_cc_lm_ijk"][w["l_far"]] + 25)
+        ach = fl / t / 1e12
+        line["roofline"] = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                            "frac": ach / PEAK_FP64, "traffic": None}
+
+    if not a.no_cpu:
+        cb = cpu_baseline(D, a.cpu_threads)
+        cpu_pps = (cb["rho_pts"] + cb["far_pts"]) / (cb["rho_s"] + cb["far_s"])
+        line["cpu_baseline"] = {
+            "value": cpu_pps, "unit": "points/s", "cores": a.cpu_threads, "kind": "port",
+            "sample": f"{len(cb['sel'])} random batches ({cb['rho_pts']} rho1 points, {cb['rho_s']:.1f}s) + "
+                      f"{cb['far_pts']} far-field points x {fp.n_atoms} centres ({cb['far_s']:.1f}s), "
+                      f"CPU FP64 oracle, OpenMP {a.cpu_threads} threads"}
+        # FP64 GPU path against the oracle on the sample
+        if "fp64" in outs:
+            ref = cb["rho_cpu"][:, 0]
+            sub = cb["sub"]
+            got = outs["fp64"]["rho"].cpu().numpy()
+            mp_sub = sub.i_valid_point_2_i_full_points_map[:sub.n_point_batches_prefix_sum[-1]]
+            line["rho1_fp64_vs_oracle_rel"] = float(np.abs(got[mp_sub] - ref[mp_sub]).max() / np.abs(ref[mp_sub]).max())
+            if "fast" in outs:
+                gotf = outs["fast"]["rho"].cpu().numpy()
+                line["rho1_fast_vs_oracle_rel"] = float(np.abs(gotf[mp_sub] - ref[mp_sub]).max() /
+                                                        np.abs(ref[mp_sub]).max())
+            mpf = fp.i_valid_point_2_i_full_points_map[:cb["nfs"]]
+            dref = cb["dv_cpu"][mpf, 0]
+            for p in outs:
+                dg = outs[p]["dv"].cpu().numpy()[mpf]
+                line[f"far_{p}_vs_oracle_rel"] = float(np.abs(dg - dref).max() / np.abs(dref).max())
+    print(json.dumps(line), flush=True)
+
+
+def far_pairs(D):
+    """Number of (point, centre) pairs in the far-field branch (host count on the inputs)."""
+    import torch
+    fp = D["fp"]
+    pts = torch.from_numpy(fp.coord_points[:, fp.i_valid_point_2_i_full_points_map].T.copy()).cuda()
+    cen = torch.from_numpy(fp.coords_center.T.copy()).cuda()
+    n = 0
+    for s in range(0, pts.shape[0], 200000):
+        d2 = torch.cdist(pts[s:s + 200000], cen) ** 2
+        n += int(((d2 >= 64.0) & (d2 < 1e8)).sum())
+    return n
+
+
+if __name__ == "__main__":
+    main()

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libfrpa.so")
 FRPA_OK = 0
 FRPA_PREC_FP64 = 0
 FRPA_PREC_OZAKI = 1
+FRPA_PREC_OZAKI4 = 2
 FRPA_FAR_TERMS = 0
 FRPA_FAR_HORNER = 1
 
@@ -35,7 +36,7 @@ def lib():
         L.frpa_rho1_plan_destroy.argtypes = [p]
         L.frpa_rho1_forward.argtypes = [p, i, i, i, p, p, p, p, p, p, p, p]
         L.frpa_rho1_adjoint.argtypes = [p, i, i, p, p, p, p]
-        L.frpa_rho1_prepare_wave.argtypes = [p, p, ll, p]
+        L.frpa_rho1_prepare_wave.argtypes = [p, i, p, ll, p]
         L.frpa_synth_wave.argtypes = [p, i, p, p, p, p, p, p, p, p]
         L.frpa_far_plan_create.argtypes = [ctypes.POINTER(p), i, i, i, i, i, i, i, i] + [p] * 9
         L.frpa_far_plan_destroy.argtypes = [p]

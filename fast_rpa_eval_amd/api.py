@@ -21,7 +21,7 @@ import torch
 from . import _lib
 from ._lib import check, lib
 
-FP64, OZAKI = _lib.FRPA_PREC_FP64, _lib.FRPA_PREC_OZAKI
+FP64, OZAKI, OZAKI4 = _lib.FRPA_PREC_FP64, _lib.FRPA_PREC_OZAKI, _lib.FRPA_PREC_OZAKI4
 FAR_TERMS, FAR_HORNER = _lib.FRPA_FAR_TERMS, _lib.FRPA_FAR_HORNER
 
 
@@ -109,9 +109,9 @@ class Rho1Plan:
               "frpa_rho1_adjoint")
         return H
 
-    def prepare_wave(self, wave, stream=None):
+    def prepare_wave(self, wave, precision=OZAKI, stream=None):
         b = ctypes.c_longlong(0)
-        check(lib().frpa_rho1_prepare_wave(self.handle, _dp(wave), ctypes.byref(b), _stream(stream)),
+        check(lib().frpa_rho1_prepare_wave(self.handle, precision, _dp(wave), ctypes.byref(b), _stream(stream)),
               "frpa_rho1_prepare_wave")
         return b.value
 

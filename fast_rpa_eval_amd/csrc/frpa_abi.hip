@@ -41,6 +41,8 @@ void fill_f64(double *p, size_t n, double v, hipStream_t s) {
 Rho1Plan::~Rho1Plan() {
   if (ozaki)
     ozaki_release(ozaki);
+  if (adj)
+    adj_release(adj);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -279,8 +281,8 @@ int frpa_rho1_forward(frpa_rho1_plan *plan, int atom_tile_size, int n_tile, int 
       fill_f64(first_order_gradient_rho, (size_t)3 * P.nmb * P.n_batches * atom_tile_size, 0.0, s);
     if (precision == FRPA_PREC_FP64) {
       rho1_forward_fp64(P, c, s);
-    } else if (precision == FRPA_PREC_OZAKI) {
-      rho1_forward_ozaki(P, c, s);
+    } else if (precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4) {
+      rho1_forward_ozaki(P, c, precision == FRPA_PREC_OZAKI ? 5 : 4, s);
     } else {
       FRPA_REQUIRE(false, FRPA_ERR_INVALID, "unknown precision %d", precision);
     }
@@ -296,10 +298,14 @@ int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, c
   });
 }
 
-int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, const double *wave, long long *bytes, void *stream) {
+int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wave, long long *bytes,
+                           void *stream) {
   return guarded([&] {
     FRPA_REQUIRE(plan && wave, FRPA_ERR_INVALID, "bad arguments");
-    long long b = ozaki_prepare_wave(*reinterpret_cast<Rho1Plan *>(plan), wave, as_stream(stream));
+    FRPA_REQUIRE(precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4, FRPA_ERR_INVALID,
+                 "prepare_wave applies to the INT8-split precisions");
+    long long b = ozaki_prepare_wave(*reinterpret_cast<Rho1Plan *>(plan), wave,
+                                     precision == FRPA_PREC_OZAKI ? 5 : 4, as_stream(stream));
     if (bytes)
       *bytes = b;
   });

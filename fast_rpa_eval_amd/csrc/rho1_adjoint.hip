@@ -1,10 +1,212 @@
-// rho1 adjoint (first-order H integration): placeholder until the kernel lands.
+// Adjoint of rho1 with respect to P1 = the LDA first-order H integration:
+//   first_order_H(mu, nu, t) += sum_batches sum_{p in batch} g(map(p), t) phi_mu(p) phi_nu(p)
+// for every pair (mu, nu) of the batch's compute set.
+//
+// Reference: evaluate_first_order_H.cpp:144-215 builds per batch contract = 0.5 g phi, a dsyr2k
+// H_loc = contract phi^T + phi contract^T, and scatter-adds H_loc into the dense n_basis^2 matrix.
+// A per-batch GPU scatter would issue sum_b nc_b^2 atomic adds (3e10 for 60k batches of ~650
+// functions), so the work is reorganised by output ownership instead: H is cut into 64 x 64 basis
+// tiles; each workgroup owns one tile pair (I <= J) and loops over exactly the batches whose
+// compute sets touch both tiles, accumulating
+//   H_IJ += sum_b Phi_b[rows in I] diag(g) Phi_b[rows in J]^T
+// in FP64 MFMA registers (K = the batch's points), then writes the tile and, for I < J, its
+// transpose.  Every H element is written once per call, in a fixed batch order: deterministic,
+// no atomics.
+#include <algorithm>
+#include <vector>
+
 #include "rho1_common.hpp"
 
 namespace frpa {
 
-void rho1_adjoint(Rho1Plan &, int, int, const double *, const double *, double *, hipStream_t) {
-  FRPA_REQUIRE(false, FRPA_ERR_UNSUPPORTED, "rho1 adjoint is not available in this build");
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+struct AdjEntry {
+  int d;       // index into the plan's BatchDesc array
+  int i0, i1;  // rows of the batch's compute list inside tile I
+  int j0, j1;  // rows inside tile J
+};
+struct AdjTask {
+  int I, J;
+  int e0, e1;  // entries [e0, e1)
+};
+
+struct AdjState {
+  int n_tasks = 0;
+  DevBuf<AdjEntry> d_entries;
+  DevBuf<AdjTask> d_tasks;
+};
+
+void adj_release(AdjState *st) { delete st; }
+
+namespace {
+
+constexpr int TILE = 64;
+constexpr int KCH = 16;
+
+__global__ __launch_bounds__(256) void h1_tile_kernel(const AdjTask *__restrict__ tasks,
+                                                      const AdjEntry *__restrict__ entries,
+                                                      const BatchDesc *__restrict__ descs,
+                                                      const int *__restrict__ ibasis0, const int *__restrict__ map,
+                                                      const double *__restrict__ wave, const double *__restrict__ g,
+                                                      int n_full, int n_basis, double *__restrict__ H) {
+  constexpr int LDT = TILE + 1;
+  __shared__ double As[KCH * LDT];
+  __shared__ double Bs[KCH * LDT];
+  __shared__ int invI[TILE], invJ[TILE];
+  __shared__ double gk[KCH];
+
+  const AdjTask task = tasks[blockIdx.x];
+  const int t = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wm = 32 * (w >> 1), wn = 32 * (w & 1);  // this wave's 32 x 32 sub-tile
+  const double *gt = g + (size_t)n_full * t;
+
+  v4d acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+      acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  for (int e = task.e0; e < task.e1; e++) {
+    const AdjEntry en = entries[e];
+    const BatchDesc d = descs[en.d];
+    const int ld = (d.nc + 3) & ~3;
+    const double *W = wave + d.wave_off;
+    const int *ib = ibasis0 + d.ib_off;
+    __syncthreads();
+    if (tid < TILE) {
+      invI[tid] = -1;
+      invJ[tid] = -1;
+    }
+    __syncthreads();
+    for (int i = en.i0 + tid; i < en.i1; i += 256)
+      invI[ib[i] - TILE * task.I] = i;
+    for (int i = en.j0 + tid; i < en.j1; i += 256)
+      invJ[ib[i] - TILE * task.J] = i;
+    for (int p0 = 0; p0 < d.np; p0 += KCH) {
+      __syncthreads();
+      if (tid < KCH) {
+        const int p = p0 + tid;
+        gk[tid] = p < d.np ? gt[map[d.pstart + p]] : 0.0;
+      }
+      __syncthreads();
+      for (int q = tid; q < TILE * KCH; q += 256) {
+        const int m = q % TILE, k = q / TILE;
+        const int p = p0 + k;
+        const int ia = invI[m], ja = invJ[m];
+        As[k * LDT + m] = (ia >= 0 && p < d.np) ? W[ia + (size_t)ld * p] * gk[k] : 0.0;
+        Bs[k * LDT + m] = (ja >= 0 && p < d.np) ? W[ja + (size_t)ld * p] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < KCH; kk += 4) {
+        const double a0 = As[(kk + lk) * LDT + wm + lr];
+        const double a1 = As[(kk + lk) * LDT + wm + 16 + lr];
+        const double b0 = Bs[(kk + lk) * LDT + wn + lr];
+        const double b1 = Bs[(kk + lk) * LDT + wn + 16 + lr];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+    }
+  }
+
+  // acc[a][b][r] = H_tile(wm + 16 a + 4 lk + r, wn + 16 b + lr)
+  double *Ht = H + (size_t)n_basis * n_basis * t;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int mu = TILE * task.I + wm + 16 * a + 4 * lk + r;
+        const int nu = TILE * task.J + wn + 16 * b + lr;
+        if (mu < n_basis && nu < n_basis) {
+          const double v = acc[a][b][r];
+          Ht[mu + (size_t)n_basis * nu] += v;
+          if (task.I != task.J)
+            Ht[nu + (size_t)n_basis * mu] += v;
+        }
+      }
+}
+
+AdjState *build_adj(const Rho1Plan &P) {
+  const int nt = (P.n_basis + TILE - 1) / TILE;
+  std::vector<std::vector<AdjEntry>> per(size_t(nt) * nt);
+  for (int di = 0; di < P.n_valid; di++) {
+    const BatchDesc &d = P.h_desc[di];
+    const int *ib = P.h_ibasis0.data() + d.ib_off;
+    // tile runs of the sorted compute list
+    std::vector<int> tl, r0, r1;
+    for (int i = 0; i < d.nc;) {
+      const int tile = ib[i] / TILE;
+      int j = i;
+      while (j < d.nc && ib[j] / TILE == tile)
+        j++;
+      tl.push_back(tile);
+      r0.push_back(i);
+      r1.push_back(j);
+      i = j;
+    }
+    for (size_t x = 0; x < tl.size(); x++)
+      for (size_t y = x; y < tl.size(); y++) {
+        const int I = std::min(tl[x], tl[y]), J = std::max(tl[x], tl[y]);
+        const bool sw = tl[x] > tl[y];
+        AdjEntry e{di, sw ? r0[y] : r0[x], sw ? r1[y] : r1[x], sw ? r0[x] : r0[y], sw ? r1[x] : r1[y]};
+        per[size_t(I) * nt + J].push_back(e);
+      }
+  }
+  std::vector<AdjEntry> entries;
+  std::vector<AdjTask> tasks;
+  std::vector<double> work;
+  for (int I = 0; I < nt; I++)
+    for (int J = I; J < nt; J++) {
+      auto &v = per[size_t(I) * nt + J];
+      if (v.empty())
+        continue;
+      double wk = 0;
+      for (auto &e : v)
+        wk += (double)P.h_desc[e.d].np;
+      tasks.push_back(AdjTask{I, J, (int)entries.size(), (int)(entries.size() + v.size())});
+      work.push_back(wk);
+      entries.insert(entries.end(), v.begin(), v.end());
+    }
+  // largest work lists first
+  std::vector<int> ord(tasks.size());
+  for (size_t i = 0; i < ord.size(); i++)
+    ord[i] = (int)i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work[a] > work[b]; });
+  std::vector<AdjTask> sorted(tasks.size());
+  for (size_t i = 0; i < ord.size(); i++)
+    sorted[i] = tasks[ord[i]];
+  AdjState *st = new AdjState();
+  st->n_tasks = (int)sorted.size();
+  st->d_entries.upload(entries.empty() ? std::vector<AdjEntry>{AdjEntry{0, 0, 0, 0, 0}} : entries);
+  st->d_tasks.upload(sorted.empty() ? std::vector<AdjTask>{AdjTask{0, 0, 0, 0}} : sorted);
+  return st;
+}
+
+} // namespace
+
+void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const double *g, double *H, hipStream_t s) {
+  FRPA_REQUIRE(wave || P.n_valid == 0, FRPA_ERR_INVALID, "null wave");
+  if (P.n_valid == 0)
+    return;
+  FRPA_REQUIRE(precision != FRPA_PREC_OZAKI, FRPA_ERR_UNSUPPORTED,
+               "the INT8-split adjoint is not available in this build (use FRPA_PREC_FP64)");
+  FRPA_REQUIRE(precision == FRPA_PREC_FP64, FRPA_ERR_INVALID, "unknown precision %d", precision);
+  if (!P.adj)
+    P.adj = build_adj(P);
+  if (P.adj->n_tasks == 0)
+    return;
+  hipLaunchKernelGGL(h1_tile_kernel, dim3(P.adj->n_tasks, T), dim3(256), 0, s, P.adj->d_tasks.ptr,
+                     P.adj->d_entries.ptr, P.d_desc.ptr, P.d_ibasis0.ptr, P.d_map.ptr, wave, g, P.n_full, P.n_basis,
+                     H);
+  check_launch("h1_tile_kernel");
 }
 
 } // namespace frpa

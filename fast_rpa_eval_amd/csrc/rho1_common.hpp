@@ -17,6 +17,7 @@ struct BatchDesc {
 static_assert(sizeof(BatchDesc) == 32, "BatchDesc layout");
 
 struct OzakiState;  // INT8 split data (rho1_ozaki.hip)
+struct AdjState;    // basis-tile work lists of the adjoint (rho1_adjoint.hip)
 
 struct Rho1Plan {
   int n_batches = 0, n_basis = 0, n_full = 0, nmb = 0;
@@ -28,6 +29,7 @@ struct Rho1Plan {
   DevBuf<int> d_ibasis0;
   DevBuf<int> d_map;
   OzakiState *ozaki = nullptr;
+  AdjState *adj = nullptr;
   ~Rho1Plan();
 };
 
@@ -41,10 +43,11 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
                          const int *n_point_batches, const int *prefix, const int *map, const int *ncc,
                          const long long *ibasis_ptr, const int *ibasis, const long long *wave_offset);
 void rho1_forward_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s);
-void rho1_forward_ozaki(Rho1Plan &P, const Rho1Call &c, hipStream_t s);
+void rho1_forward_ozaki(Rho1Plan &P, const Rho1Call &c, int slices, hipStream_t s);
 void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const double *g, double *H, hipStream_t s);
 void ozaki_release(OzakiState *st);
-long long ozaki_prepare_wave(Rho1Plan &P, const double *wave, hipStream_t s);
+void adj_release(AdjState *st);
+long long ozaki_prepare_wave(Rho1Plan &P, const double *wave, int slices, hipStream_t s);
 void synth_wave(const Rho1Plan &P, int n_atoms, const double *atom_xyz, const int *basis_atom, const int *basis_l,
                 const int *basis_m, const double *basis_alpha, int n_basis, const double *coord_points, double *wave,
                 hipStream_t s);

@@ -164,3 +164,31 @@ def real_sph_harm(l_max, xyz):
                 v = v + float(c) * x ** i * y ** j * z ** k
             out[..., l * l + l + m] = sh_norm(l, m) * v
     return out
+
+
+def horner_fma_count(L):
+    """FMAs of the device's nested-Horner evaluation of one far-field pair at l = L.
+
+    Mirrors the kernel's structure: sum_q u^q Q_q(X, Y, Z), Q_q nested in X (degree <= 1), Y, Z
+    (fast_rpa_eval_amd/csrc/farfield.hip, hz/hy/hq/hu).  Equals the merged-coefficient count plus
+    the Horner joins.
+    """
+    def hz(q, i, j):
+        return L - 2 * q - i - j
+
+    def hy(q, i, j):
+        jm = L - 2 * q - i
+        return hz(q, i, j) if j == jm else 1 + hy(q, i, j + 1) + hz(q, i, j)
+
+    def hq(q):
+        return hy(q, 0, 0) if L - 2 * q == 0 else 1 + hy(q, 1, 0) + hy(q, 0, 0)
+
+    def hu(q):
+        return hq(q) if 2 * (q + 1) > L else 1 + hu(q + 1) + hq(q)
+
+    return hu(0)
+
+
+def horner_ncoef(L):
+    """Merged (i, j, k, q) coefficients at l = L: sum_q (L - 2q + 1)^2."""
+    return sum((L - 2 * q + 1) ** 2 for q in range(L // 2 + 1))

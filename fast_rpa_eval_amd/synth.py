@@ -233,10 +233,23 @@ def make_rho1_problem(n_atoms, box_ang, n_batches, np_range, nc_range=None, conf
         ncs = rng.integers(nc_range[0], nc_range[1] + 1, size=n_batches)
         ncs = np.minimum(ncs, nb_basis)
         ncs[list(zero_nc)] = 0
-        for b in range(n_batches):
-            d2 = np.sum((basis_xyz - centres[b]) ** 2, axis=1)
-            order = np.lexsort((np.arange(nb_basis), d2))
-            ibasis_list.append(np.sort(order[: ncs[b]]).astype(np.int32) + 1)
+        # nearest basis functions to the batch centre: all functions of an atom share its distance,
+        # so whole atoms are taken in (distance, atom index) order and the last one is truncated
+        # (the basis of atom a is the contiguous range [start[a], start[a + 1])).
+        start = np.searchsorted(basis.atom, np.arange(n_atoms + 1))
+        for c0 in range(0, n_batches, 4096):
+            c1 = min(c0 + 4096, n_batches)
+            d2 = np.sum((xyz[None, :, :] - centres[c0:c1, None, :]) ** 2, axis=2)
+            order = np.argsort(d2, axis=1, kind="stable")
+            for b in range(c0, c1):
+                need, got, parts = int(ncs[b]), 0, []
+                for a in order[b - c0]:
+                    if got >= need:
+                        break
+                    parts.append(np.arange(start[a], start[a + 1]))
+                    got += start[a + 1] - start[a]
+                idx = np.concatenate(parts)[:need] if parts else np.zeros(0, np.int64)
+                ibasis_list.append(np.sort(idx).astype(np.int32) + 1)
     ncs = np.asarray([len(v) for v in ibasis_list], dtype=np.int64)
     batch_sizes, prefix, mp, n_full, wave_offset = _batch_layout(nps, ncs, rng)
 

@@ -46,8 +46,10 @@ const char *frpa_version(void);
 typedef struct frpa_rho1_plan frpa_rho1_plan;
 
 /* Precision of the rho1 / adjoint GEMM. */
-#define FRPA_PREC_FP64 0   /* FP64 MFMA, FP64 accumulation                                    */
-#define FRPA_PREC_OZAKI 1  /* INT8-MFMA split (exact int32 slice products, FP64 recombination)  */
+#define FRPA_PREC_FP64 0   /* FP64 MFMA, FP64 accumulation                                     */
+#define FRPA_PREC_OZAKI 1  /* INT8-MFMA split, 5 radix-254 digits per operand, 15 exact INT8     */
+                           /* GEMMs, FP64 recombination (max|d rho1|/max|rho1| ~ 1e-12)          */
+#define FRPA_PREC_OZAKI4 2 /* INT8-MFMA split with 4 digits, 10 INT8 GEMMs (~ 5e-10)            */
 
 /*
  * Batch metadata (host arrays, copied into the plan):
@@ -93,10 +95,12 @@ int frpa_rho1_forward(frpa_rho1_plan *plan, int atom_tile_size, int n_tile, int 
 int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, const double *wave, const double *g,
                       double *first_order_H, void *stream);
 
-/* One-off conversion of wave_batches_compress into the INT8 slices used by FRPA_PREC_OZAKI
- * (kept in the plan; reused across calls while wave is unchanged).  Optional: the first
- * OZAKI call converts on demand.  Returns the bytes held in *bytes (nullable). */
-int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, const double *wave, long long *bytes, void *stream);
+/* One-off conversion of wave_batches_compress into the INT8 digits used by the INT8-split
+ * precisions (kept in the plan and reused while the same wave pointer is passed; call again after
+ * changing wave in place).  Optional: the first INT8-split forward converts on demand.
+ * Returns the device bytes held in *bytes (nullable). */
+int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wave, long long *bytes,
+                           void *stream);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Far-field Hartree multipole sum                                                            */

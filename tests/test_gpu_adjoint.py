@@ -1,0 +1,55 @@
+"""Adjoint of rho1 w.r.t. P1 on the GPU (the LDA first-order H integration) against the CPU oracle
+of evaluate_first_order_H (fast_rap_src_dfpt_gpu_eval/evaluate_first_order_H.cpp:13-222) and the
+adjoint identity <g, rho1(P)> = <H1(g), P>_F."""
+import numpy as np
+import pytest
+import torch
+
+from fast_rpa_eval_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_h1(oracle, prob, g_full, T):
+    """LDA H1 through the reference's argument list: partition_tab = 1, v_hartree_gradient = g."""
+    nmb, nb, nf = prob.n_max_batch_size, prob.n_batches, prob.n_full_points
+    z = lambda *s: np.zeros(s, order="F")
+    a = dict(vrho=z(nmb * nb), vsigma=z(3 * nmb * nb), gradient_rho=z(3, nf),
+             gwave=np.zeros(3 * int(prob.wave_offset[-1])), hwave=np.zeros(6 * int(prob.wave_offset[-1])),
+             H_prefactor1=z(nmb, nb, T), grad_rho1=z(3, nmb, nb, T), v_hartree_gradient=np.asfortranarray(g_full),
+             first_order_rho=z(nf, T), v2rho2=z(3 * nmb * nb), v2rhosigma=z(6 * nmb * nb), v2sigma2=z(6 * nmb * nb),
+             partition_tab=np.ones(nf))
+    H = np.zeros((prob.n_basis, prob.n_basis, T), order="F")
+    return oracle.h1(prob, T, 1, T, 1, H, a)
+
+
+@pytest.mark.parametrize("k,args", [(1, (8, 10.0, 64, 100, None)), (3, (40, 16.0, 120, (1, 220), (1, 260)))])
+def test_h1_lda_against_oracle(cuda, oracle, k, args):
+    prob = synth.make_rho1_problem(*args, config_k=k, zero_nc=(2,) if k == 3 else ())
+    T = 2
+    rng = np.random.default_rng(k)
+    g = np.asfortranarray(rng.standard_normal((prob.n_full_points, T)))
+    ref = _oracle_h1(oracle, prob, g, T)
+    plan = api.Rho1Plan.from_problem(prob)
+    H0 = rng.standard_normal(prob.n_basis * prob.n_basis * T)
+    Hd = api.to_device(H0)  # accumulated into
+    plan.adjoint(api.to_device(prob.wave), api.to_device(g), T=T, H=Hd)
+    got = Hd.cpu().numpy().reshape((prob.n_basis, prob.n_basis, T), order="F") - H0.reshape(
+        (prob.n_basis, prob.n_basis, T), order="F")
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"[h1 lda config {k}] rel {err:.3e}", flush=True)
+    assert err <= 1e-12
+
+
+def test_h1_adjoint_identity_with_gpu_forward(cuda):
+    prob = synth.make_rho1_problem(30, 15.0, 100, (20, 200), (20, 300), config_k=11)
+    rng = np.random.default_rng(0)
+    plan = api.Rho1Plan.from_problem(prob)
+    wave = api.to_device(prob.wave)
+    P = rng.standard_normal((prob.n_basis, prob.n_basis))
+    g = rng.standard_normal(prob.n_full_points)
+    rho = plan.forward(wave, api.to_device(np.asfortranarray(P[:, :, None]))).cpu().numpy()
+    H = plan.adjoint(wave, api.to_device(g)).cpu().numpy().reshape((prob.n_basis, prob.n_basis), order="F")
+    lhs, rhs = float(np.dot(g, rho)), float(np.sum(H * P))
+    print(f"[adjoint identity] {lhs:.15e} {rhs:.15e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-11 * max(1.0, abs(lhs))

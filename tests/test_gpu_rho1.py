@@ -111,3 +111,27 @@ def test_synth_wave_on_device_matches_host(cuda):
     fin = np.isfinite(ref)
     assert np.array_equal(fin, np.isfinite(w))
     np.testing.assert_allclose(w[fin], ref[fin], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("precision,tol", [(api.OZAKI, 1e-10), (api.OZAKI4, 1e-9)])
+def test_rho1_int8_split_against_oracle(cuda, oracle, precision, tol):
+    """INT8-split GEMM: max|d rho1| / max|rho1| within the stated bound of the FP64 oracle."""
+    for k, args in ((1, (8, 10.0, 64, 100, None)), (3, (24, 14.0, 90, (1, 230), (1, 180)))):
+        prob = synth.make_rho1_problem(*args, config_k=k, zero_nc=(3,) if k == 3 else ())
+        P1 = synth.make_p1(prob.atom_xyz, prob.basis, 3000 + k)
+        bias, _ = _bias(prob, 1, 4)
+        ref, _ = oracle.rho1(prob, P1, T=1, bias=bias)
+        got, _ = _run_gpu(prob, P1, 1, 1, bias=bias, precision=precision)
+        _check(got, ref, rel=tol, what=f"rho1 int8 split (precision {precision}, config {k})")
+
+
+def test_rho1_int8_split_rejects_gradient_and_reports_bytes(cuda):
+    prob = synth.make_rho1_problem(6, 9.0, 10, (10, 50), (5, 60), config_k=9)
+    plan = api.Rho1Plan.from_problem(prob)
+    wave = api.to_device(prob.wave)
+    assert plan.prepare_wave(wave, api.OZAKI) > 0
+    P1 = api.to_device(np.asfortranarray(np.eye(prob.n_basis)[:, :, None]))
+    gw = api.to_device(synth.random_like_wave(prob, 3, 1))
+    from fast_rpa_eval_amd._lib import FrpaError
+    with pytest.raises(FrpaError, match="rho1 only"):
+        plan.forward(wave, P1, precision=api.OZAKI, gradient_wave=gw, grad_bias=torch.zeros_like(gw))

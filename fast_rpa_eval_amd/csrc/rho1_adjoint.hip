@@ -12,6 +12,9 @@
 // in FP64 MFMA registers (K = the batch's points), then writes the tile and, for I < J, its
 // transpose.  Every H element is written once per call, in a fixed batch order: deterministic,
 // no atomics.
+// Tiles are cut in a relabelled basis order pi (order of first appearance over the caller's batch
+// order), so that functions that share batches share tiles even when the caller's basis numbering
+// is not spatially ordered; results are written back at the caller's indices.
 #include <algorithm>
 #include <vector>
 
@@ -23,8 +26,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 struct AdjEntry {
   int d;       // index into the plan's BatchDesc array
-  int i0, i1;  // rows of the batch's compute list inside tile I
-  int j0, j1;  // rows inside tile J
+  int i0, i1;  // positions in the batch's pi-sorted row list that fall in tile I
+  int j0, j1;  // ... in tile J
 };
 struct AdjTask {
   int I, J;
@@ -35,6 +38,8 @@ struct AdjState {
   int n_tasks = 0;
   DevBuf<AdjEntry> d_entries;
   DevBuf<AdjTask> d_tasks;
+  DevBuf<int> d_pi, d_pinv;  // basis relabelling and its inverse
+  DevBuf<int> d_srow;        // per batch (at its ib_off): row indices sorted by pi(i_basis)
 };
 
 void adj_release(AdjState *st) { delete st; }
@@ -47,7 +52,9 @@ constexpr int KCH = 16;
 __global__ __launch_bounds__(256) void h1_tile_kernel(const AdjTask *__restrict__ tasks,
                                                       const AdjEntry *__restrict__ entries,
                                                       const BatchDesc *__restrict__ descs,
-                                                      const int *__restrict__ ibasis0, const int *__restrict__ map,
+                                                      const int *__restrict__ ibasis0, const int *__restrict__ srow,
+                                                      const int *__restrict__ pi, const int *__restrict__ pinv,
+                                                      const int *__restrict__ map,
                                                       const double *__restrict__ wave, const double *__restrict__ g,
                                                       int n_full, int n_basis, double *__restrict__ H) {
   constexpr int LDT = TILE + 1;
@@ -76,16 +83,21 @@ __global__ __launch_bounds__(256) void h1_tile_kernel(const AdjTask *__restrict_
     const int ld = (d.nc + 3) & ~3;
     const double *W = wave + d.wave_off;
     const int *ib = ibasis0 + d.ib_off;
+    const int *sr = srow + d.ib_off;
     __syncthreads();
     if (tid < TILE) {
       invI[tid] = -1;
       invJ[tid] = -1;
     }
     __syncthreads();
-    for (int i = en.i0 + tid; i < en.i1; i += 256)
-      invI[ib[i] - TILE * task.I] = i;
-    for (int i = en.j0 + tid; i < en.j1; i += 256)
-      invJ[ib[i] - TILE * task.J] = i;
+    for (int k = en.i0 + tid; k < en.i1; k += 256) {
+      const int i = sr[k];
+      invI[pi[ib[i]] - TILE * task.I] = i;
+    }
+    for (int k = en.j0 + tid; k < en.j1; k += 256) {
+      const int i = sr[k];
+      invJ[pi[ib[i]] - TILE * task.J] = i;
+    }
     for (int p0 = 0; p0 < d.np; p0 += KCH) {
       __syncthreads();
       if (tid < KCH) {
@@ -115,7 +127,8 @@ __global__ __launch_bounds__(256) void h1_tile_kernel(const AdjTask *__restrict_
     }
   }
 
-  // acc[a][b][r] = H_tile(wm + 16 a + 4 lk + r, wn + 16 b + lr)
+  // v_mfma_f64_16x16x4f64 leaves D(4 r + lane / 16, lane % 16) in acc[r]:
+  // acc[a][b][r] = H_tile(wm + 16 a + 4 r + lk, wn + 16 b + lr)
   double *Ht = H + (size_t)n_basis * n_basis * t;
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -123,9 +136,10 @@ __global__ __launch_bounds__(256) void h1_tile_kernel(const AdjTask *__restrict_
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int mu = TILE * task.I + wm + 16 * a + 4 * lk + r;
-        const int nu = TILE * task.J + wn + 16 * b + lr;
-        if (mu < n_basis && nu < n_basis) {
+        const int pm = TILE * task.I + wm + 16 * a + 4 * r + lk;
+        const int pn = TILE * task.J + wn + 16 * b + lr;
+        if (pm < n_basis && pn < n_basis) {
+          const int mu = pinv[pm], nu = pinv[pn];
           const double v = acc[a][b][r];
           Ht[mu + (size_t)n_basis * nu] += v;
           if (task.I != task.J)
@@ -135,30 +149,52 @@ __global__ __launch_bounds__(256) void h1_tile_kernel(const AdjTask *__restrict_
 }
 
 AdjState *build_adj(const Rho1Plan &P) {
-  const int nt = (P.n_basis + TILE - 1) / TILE;
+  const int nb = P.n_basis;
+  const int nt = (nb + TILE - 1) / TILE;
+  // relabel basis functions by first appearance over the batches in the caller's order
+  std::vector<int> order(P.n_valid);
+  for (int i = 0; i < P.n_valid; i++)
+    order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return P.h_desc[a].b < P.h_desc[b].b; });
+  std::vector<int> pi(nb, -1), pinv(nb);
+  int next = 0;
+  for (int di : order) {
+    const BatchDesc &d = P.h_desc[di];
+    for (int i = 0; i < d.nc; i++) {
+      const int mu = P.h_ibasis0[d.ib_off + i];
+      if (pi[mu] < 0)
+        pi[mu] = next++;
+    }
+  }
+  for (int mu = 0; mu < nb; mu++)
+    if (pi[mu] < 0)
+      pi[mu] = next++;
+  for (int mu = 0; mu < nb; mu++)
+    pinv[pi[mu]] = mu;
+  std::vector<int> srow(P.h_ibasis0.size());
   std::vector<std::vector<AdjEntry>> per(size_t(nt) * nt);
   for (int di = 0; di < P.n_valid; di++) {
     const BatchDesc &d = P.h_desc[di];
     const int *ib = P.h_ibasis0.data() + d.ib_off;
-    // tile runs of the sorted compute list
+    int *sr = srow.data() + d.ib_off;
+    for (int i = 0; i < d.nc; i++)
+      sr[i] = i;
+    std::sort(sr, sr + d.nc, [&](int a, int b) { return pi[ib[a]] < pi[ib[b]]; });
+    // tile runs of the pi-sorted row list
     std::vector<int> tl, r0, r1;
-    for (int i = 0; i < d.nc;) {
-      const int tile = ib[i] / TILE;
-      int j = i;
-      while (j < d.nc && ib[j] / TILE == tile)
+    for (int k = 0; k < d.nc;) {
+      const int tile = pi[ib[sr[k]]] / TILE;
+      int j = k;
+      while (j < d.nc && pi[ib[sr[j]]] / TILE == tile)
         j++;
       tl.push_back(tile);
-      r0.push_back(i);
+      r0.push_back(k);
       r1.push_back(j);
-      i = j;
+      k = j;
     }
     for (size_t x = 0; x < tl.size(); x++)
-      for (size_t y = x; y < tl.size(); y++) {
-        const int I = std::min(tl[x], tl[y]), J = std::max(tl[x], tl[y]);
-        const bool sw = tl[x] > tl[y];
-        AdjEntry e{di, sw ? r0[y] : r0[x], sw ? r1[y] : r1[x], sw ? r0[x] : r0[y], sw ? r1[x] : r1[y]};
-        per[size_t(I) * nt + J].push_back(e);
-      }
+      for (size_t y = x; y < tl.size(); y++)
+        per[size_t(tl[x]) * nt + tl[y]].push_back(AdjEntry{di, r0[x], r1[x], r0[y], r1[y]});
   }
   std::vector<AdjEntry> entries;
   std::vector<AdjTask> tasks;
@@ -185,6 +221,9 @@ AdjState *build_adj(const Rho1Plan &P) {
     sorted[i] = tasks[ord[i]];
   AdjState *st = new AdjState();
   st->n_tasks = (int)sorted.size();
+  st->d_pi.upload(pi);
+  st->d_pinv.upload(pinv);
+  st->d_srow.upload(srow.empty() ? std::vector<int>{0} : srow);
   st->d_entries.upload(entries.empty() ? std::vector<AdjEntry>{AdjEntry{0, 0, 0, 0, 0}} : entries);
   st->d_tasks.upload(sorted.empty() ? std::vector<AdjTask>{AdjTask{0, 0, 0, 0}} : sorted);
   return st;
@@ -204,8 +243,8 @@ void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const d
   if (P.adj->n_tasks == 0)
     return;
   hipLaunchKernelGGL(h1_tile_kernel, dim3(P.adj->n_tasks, T), dim3(256), 0, s, P.adj->d_tasks.ptr,
-                     P.adj->d_entries.ptr, P.d_desc.ptr, P.d_ibasis0.ptr, P.d_map.ptr, wave, g, P.n_full, P.n_basis,
-                     H);
+                     P.adj->d_entries.ptr, P.d_desc.ptr, P.d_ibasis0.ptr, P.adj->d_srow.ptr, P.adj->d_pi.ptr,
+                     P.adj->d_pinv.ptr, P.d_map.ptr, wave, g, P.n_full, P.n_basis, H);
   check_launch("h1_tile_kernel");
 }
 

@@ -101,7 +101,8 @@ __global__ __launch_bounds__(256) void rho1_fp64_kernel(const BatchDesc *__restr
         }
       }
 
-      // epilogue: acc[g][r] = W(i0 + 16w + 4 lk + r, pc + 16 g + lr)
+      // epilogue: v_mfma_f64_16x16x4f64 leaves D(4 r + lane / 16, lane % 16) in acc[g][r], so
+      // acc[g][r] = W(i0 + 16w + 4 r + lk, pc + 16 g + lr)
 #pragma unroll
       for (int g = 0; g < MAXPG; g++) {
         if (g < npg) {
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(256) void rho1_fp64_kernel(const BatchDesc *__restr
           if (p < np) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-              const int i = i0 + 16 * w + 4 * lk + r;
+              const int i = i0 + 16 * w + 4 * r + lk;
               if (i < nc) {
                 part += acc[g][r] * Wp[i + (size_t)ld * p];
                 if (GRAD) {

@@ -217,7 +217,8 @@ __global__ __launch_bounds__(256) void rho1_ozaki_kernel(
         }
       }
 
-      // epilogue: acc[l][g][r] = C_l(i0 + 16w + 4lk + r, pc + 16g + lr)
+      // epilogue: v_mfma_i32_16x16x64_i8 leaves D(4 (lane / 16) + r, lane % 16) in acc[r], so
+      // acc[l][g][r] = C_l(i0 + 16w + 4lk + r, pc + 16g + lr)
 #pragma unroll
       for (int g = 0; g < NPG; g++) {
         if (g < npg) {

@@ -8,7 +8,8 @@ physical model:
 * atoms uniform in a cube with a minimum separation (rejection sampling);
 * 14 basis functions per atom: 3 s shells, 2 p shells, 1 d shell, each exp(-alpha r^2) Y_lm(r^),
   alpha log-uniform in [0.1, 10] bohr^-2, Y_lm in the SHEval convention;
-* integration batches around atoms, points ~ centre + N(0, 1 bohr^2);
+* integration batches around atoms, points ~ centre + N(0, 1 bohr^2), listed in Morton order of
+  their centres (neighbouring batches are spatial neighbours, as in a grid partition);
 * P1 symmetric, P1_ij ~ N(0, 1) exp(-d_ij / 2 bohr), d_ij the distance of the host atoms.
 
 Layouts follow the reference's Fortran-ABI arguments (evaluate_first_order_rho.h:3-35):
@@ -35,6 +36,17 @@ SHELLS = ((0, 3), (1, 2), (2, 1))
 def seeds(k):
     """Seeds of configuration k (1-based): geometry, basis exponents, P1/M, grid, v1."""
     return {"geometry": 1000 + k, "basis": 2000 + k, "p1": 3000 + k, "grid": 4000 + k, "v1": 5000 + k}
+
+
+def morton_order(xyz, bits=10):
+    """Permutation sorting points along a Z-order (Morton) curve of their bounding box."""
+    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+    q = ((xyz - lo) / np.maximum(hi - lo, 1e-300) * (2 ** bits - 1)).astype(np.int64)
+    code = np.zeros(len(xyz), dtype=np.int64)
+    for b in range(bits):
+        for c in range(3):
+            code |= ((q[:, c] >> b) & 1) << (3 * b + c)
+    return np.argsort(code, kind="stable")
 
 
 def make_geometry(n_atoms, box_bohr, min_sep_bohr, seed):
@@ -219,6 +231,9 @@ def make_rho1_problem(n_atoms, box_ang, n_batches, np_range, nc_range=None, conf
         centres = xyz[host]
     else:
         centres = xyz[host] + rng.normal(0.0, batch_spread_bohr, size=(n_batches, 3))
+        # integration batches come from a spatial partition of the grid: list them along a
+        # space-filling curve, as neighbouring batches are in a real grid
+        centres = centres[morton_order(centres)]
     pts_per_batch = [centres[b] + rng.normal(0.0, 1.0, size=(nps[b], 3)) for b in range(n_batches)]
 
     nb_basis = basis.n_basis

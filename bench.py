@@ -145,6 +145,21 @@ def rho1_flops(prob):
     return float(np.sum((2 * nc * nc * npt + 2 * nc * npt)[sel]))
 
 
+def adjoint_flops(prob):
+    """FLOPs of the adjoint as executed by the default kernel: 64 x 64 tiles (I <= J) of each batch's
+    nc x nc local H, 2 * 64 * 64 * np per tile, counting only the rows/columns inside nc."""
+    tot = 0.0
+    for nc, npt in zip(prob.n_compute_c_batches, prob.n_point_batches):
+        if nc <= 0 or npt <= 0:
+            continue
+        nt = (int(nc) + 63) // 64
+        sizes = [min(64, int(nc) - 64 * i) for i in range(nt)]
+        for I in range(nt):
+            for J in range(I, nt):
+                tot += 2.0 * sizes[I] * sizes[J] * npt
+    return tot
+
+
 def rho1_bytes(prob):
     nc = prob.n_compute_c_batches.astype(np.float64)
     npt = prob.n_point_batches.astype(np.float64)
@@ -161,7 +176,7 @@ def cpu_baseline(D, threads, target_s=15.0):
     oracle.set_threads(threads)
     prob, fp = D["prob"], D["fp"]
     rng = np.random.default_rng(0)
-    nb = min(256, prob.n_batches)
+    nb = min(4096, prob.n_batches)
     sel = np.sort(rng.choice(prob.n_batches, size=nb, replace=False))
     sub = subproblem(prob, sel, D["wave"])
     P1 = D["P1"].view(prob.n_basis, prob.n_basis).T.contiguous().cpu().numpy()  # column-major (symmetric)
@@ -170,7 +185,7 @@ def cpu_baseline(D, threads, target_s=15.0):
     rho_cpu, _ = oracle.rho1(sub, P1)
     t_rho = time.time() - t0
     pts_rho = int(sub.n_point_batches.sum())
-    nfs = min(fp.n_valid_points, max(2000, int(fp.n_valid_points * 0.002)))
+    nfs = min(fp.n_valid_points, max(2000, int(fp.n_valid_points * 0.03)))
     fsub = synth.FarProblem(**{**fp.__dict__})
     fsub.i_valid_point_2_i_full_points_map = fp.i_valid_point_2_i_full_points_map[:nfs].copy()
     fsub.n_valid_points = nfs
@@ -214,7 +229,26 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--paths", default="fast,fp64")
+    ap.add_argument("--summarize-profile", metavar="DIR",
+                    help="only summarise a rocprofv3 --output-format csv directory (kernel stats and/or "
+                         "per-kernel counter sums) as a text table and exit")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device-to-device copy of GIB GiB five times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiler run) and exit")
     a = ap.parse_args()
+    if a.summarize_profile:
+        summarize_profile(a.summarize_profile)
+        return
+    if a.copy_calibration > 0:
+        import torch
+        n = int(a.copy_calibration * 2 ** 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 5}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU; the rho1 and far-field paths would shard by batches/points "
               "with no exchange (H1 needs one all-reduce), but multi-GPU runs are not part of this build",
@@ -259,7 +293,7 @@ def main():
         var = api.FAR_HORNER if path == "fast" else api.FAR_TERMS
         fns, o = step_fns(D, prec, adj_prec, var, have_adj)
         total, per = time_it(fns, names, a.steps, a.warmup)
-        res[path] = dict(ms=total, per=per, prec=prec, var=var)
+        res[path] = dict(ms=total, per=per, prec=prec, adj_prec=adj_prec, var=var)
         outs[path] = {k: v.clone() for k, v in o.items()}
         log(f"path {path}: {total:.3f} ms/step  " + "  ".join(f"{n} {per[n]:.3f} ms" for n in names))
 
@@ -291,27 +325,35 @@ def main():
             h_f, h_d = outs["fast"]["H"], outs["fp64"]["H"]
             line["h1_rel_err_vs_fp64"] = float((h_f - h_d).abs().max() / h_d.abs().max())
 
-    # roofline of the dominant kernel of the fast path
-    dom = max(fast["per"], key=fast["per"].get)
-    t = fast["per"][dom] / 1e3
-    if dom in ("rho1", "adjoint"):
-        fl = rho1_flops(prob)
-        if fast["prec"] == api.OZAKI:
-            ach, peak, unit = 15 * fl / t / 1e12, PEAK_INT8, "TOP/s"
+    # roofline of every kernel of the fast path; the dominant one is reported as "roofline"
+    pairs = far_pairs(D)
+    fl_rho = rho1_flops(prob)
+    fl_adj = adjoint_flops(prob)
+    rl = {}
+    for name, tms in fast["per"].items():
+        t = tms / 1e3
+        if name in ("rho1", "adjoint"):
+            prec = fast["prec"] if name == "rho1" else fast["adj_prec"]
+            fl = fl_rho if name == "rho1" else fl_adj
+            if prec in (api.OZAKI, api.OZAKI4):
+                npr = 15 if prec == api.OZAKI else 10
+                rl[name] = {"kernel": name, "bound": "mfma", "achieved": npr * fl / t / 1e12, "peak": PEAK_INT8,
+                            "unit": "TOP/s", "frac": npr * fl / t / 1e12 / PEAK_INT8, "traffic": None,
+                            "fp64_equivalent_TFLOPs": fl / t / 1e12}
+            else:
+                rl[name] = {"kernel": name, "bound": "mfma", "achieved": fl / t / 1e12, "peak": PEAK_FP64,
+                            "unit": "TFLOP/s", "frac": fl / t / 1e12 / PEAK_FP64, "traffic": None}
         else:
-            ach, peak, unit = fl / t / 1e12, PEAK_FP64, "TFLOP/s"
-        line["roofline"] = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": peak, "unit": unit,
-                            "frac": ach / peak, "traffic": None,
-                            "algorithmic_bytes_GB": rho1_bytes(prob) / 1e9}
-    else:
-        # far field: executed FP64 FMAs of the merged-Horner form (1 FMA per merged coefficient)
-        from fast_rpa_eval_amd.multipole_tables import horner_fma_count
-        pairs = far_pairs(D)
-        fl = pairs * (2.0 * horner_fma_count(w["l_far"]) + 25) if fast["var"] == api.FAR_HORNER else \
-            pairs * (2.0 * fp.tables["n_cc_lm_ijk"][w["l_far"]] + 25)
-        ach = fl / t / 1e12
-        line["roofline"] = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                            "frac": ach / PEAK_FP64, "traffic": None}
+            from fast_rpa_eval_amd.multipole_tables import horner_fma_count
+            per_pair = (2.0 * horner_fma_count(w["l_far"]) + 25) if fast["var"] == api.FAR_HORNER else \
+                (2.0 * fp.tables["n_cc_lm_ijk"][w["l_far"]] + 25)
+            ach = pairs * per_pair / t / 1e12
+            rl[name] = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                        "frac": ach / PEAK_FP64, "traffic": None, "pairs": pairs, "flop_per_pair": per_pair}
+    dom = max(fast["per"], key=fast["per"].get)
+    line["roofline"] = rl[dom]
+    line["roofline_all"] = rl
+    line["rho1_algorithmic_bytes_GB"] = rho1_bytes(prob) / 1e9
 
     if not a.no_cpu:
         cb = cpu_baseline(D, a.cpu_threads)
@@ -338,6 +380,34 @@ def main():
                 dg = outs[p]["dv"].cpu().numpy()[mpf]
                 line[f"far_{p}_vs_oracle_rel"] = float(np.abs(dg - dref).max() / np.abs(dref).max())
     print(json.dumps(line), flush=True)
+
+
+def summarize_profile(d):
+    """Per-kernel table of a rocprofv3 CSV output directory: kernel stats (calls, average duration) and
+    counter sums / per-dispatch means for each counter collected."""
+    import csv
+    import glob
+    stats = sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True))
+    ctrs = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
+    short = lambda n: (n.split("(")[0].split("<")[0].replace("frpa::(anonymous namespace)::", "")[:48]
+                       + ("<" + n.split("<", 1)[1].split(">")[0] + ">" if "<" in n.split("(")[0] else ""))
+    for f in stats:
+        print(f"# kernel stats: {os.path.relpath(f, d)}")
+        print(f"{'kernel':60s} {'calls':>6s} {'avg_ms':>10s} {'total_ms':>10s} {'pct':>6s}")
+        for r in csv.DictReader(open(f)):
+            print(f"{short(r['Name'])[:60]:60s} {int(r['Calls']):6d} {float(r['AverageNs']) / 1e6:10.3f} "
+                  f"{float(r['TotalDurationNs']) / 1e6:10.3f} {float(r['Percentage']):6.2f}")
+    for f in ctrs:
+        print(f"# counters: {os.path.relpath(f, d)}")
+        agg = {}
+        for r in csv.DictReader(open(f)):
+            k = (short(r["Kernel_Name"]), r["Counter_Name"])
+            v = agg.setdefault(k, [0.0, 0])
+            v[0] += float(r["Counter_Value"])
+            v[1] += 1
+        print(f"{'kernel':60s} {'counter':>12s} {'dispatches':>10s} {'sum':>16s} {'mean/dispatch':>16s}")
+        for (kn, cn), (sm, n) in sorted(agg.items()):
+            print(f"{kn[:60]:60s} {cn:>12s} {n:10d} {sm:16.1f} {sm / n:16.1f}")
 
 
 def far_pairs(D):

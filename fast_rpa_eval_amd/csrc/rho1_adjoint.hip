@@ -12,10 +12,18 @@
 // in FP64 MFMA registers (K = the batch's points), then writes the tile and, for I < J, its
 // transpose.  Every H element is written once per call, in a fixed batch order: deterministic,
 // no atomics.
+// That tile-owner kernel is the deterministic mode (FRPA_ADJOINT=tiles).  Its tiles are only as
+// full as the compute sets are contiguous in the basis order, and compact 3-D compute sets never are
+// in any 1-D order (25-40% fill measured on the synthetic clusters), so the default mode instead
+// computes each batch's local H densely (64 x 64 tiles of its own rows, I <= J) and adds it into H
+// with hardware FP64 atomics (global_atomic_add_f64): every MFMA flop is useful; the summation
+// order across batches, and so the last bits, can vary from run to run.
 // Tiles are cut in a relabelled basis order pi (order of first appearance over the caller's batch
 // order), so that functions that share batches share tiles even when the caller's basis numbering
 // is not spatially ordered; results are written back at the caller's indices.
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "rho1_common.hpp"
@@ -148,6 +156,80 @@ __global__ __launch_bounds__(256) void h1_tile_kernel(const AdjTask *__restrict_
       }
 }
 
+// Default mode: one workgroup per batch; local tile pairs (I <= J) of its compute rows; atomics.
+__global__ __launch_bounds__(256) void h1_push_kernel(const BatchDesc *__restrict__ descs,
+                                                      const int *__restrict__ ibasis0, const int *__restrict__ map,
+                                                      const double *__restrict__ wave, const double *__restrict__ g,
+                                                      int n_full, int n_basis, double *__restrict__ H) {
+  constexpr int LDT = TILE + 1;
+  __shared__ double As[KCH * LDT];
+  __shared__ double Bs[KCH * LDT];
+  __shared__ double gk[KCH];
+  const BatchDesc d = descs[blockIdx.x];
+  const int t = blockIdx.y;
+  const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
+  const double *W = wave + d.wave_off;
+  const int *ib = ibasis0 + d.ib_off;
+  const double *gt = g + (size_t)n_full * t;
+  double *Ht = H + (size_t)n_basis * n_basis * t;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wm = 32 * (w >> 1), wn = 32 * (w & 1);
+  const int nT = (nc + TILE - 1) / TILE;
+  for (int I = 0; I < nT; I++) {
+    for (int J = I; J < nT; J++) {
+      v4d acc[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+          acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int p0 = 0; p0 < np; p0 += KCH) {
+        __syncthreads();
+        if (tid < KCH) {
+          const int p = p0 + tid;
+          gk[tid] = p < np ? gt[map[d.pstart + p]] : 0.0;
+        }
+        __syncthreads();
+        for (int q = tid; q < TILE * KCH; q += 256) {
+          const int m = q % TILE, k = q / TILE;
+          const int p = p0 + k, i = TILE * I + m, j = TILE * J + m;
+          As[k * LDT + m] = (i < nc && p < np) ? W[i + (size_t)ld * p] * gk[k] : 0.0;
+          Bs[k * LDT + m] = (j < nc && p < np) ? W[j + (size_t)ld * p] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KCH; kk += 4) {
+          const double a0 = As[(kk + lk) * LDT + wm + lr];
+          const double a1 = As[(kk + lk) * LDT + wm + 16 + lr];
+          const double b0 = Bs[(kk + lk) * LDT + wn + lr];
+          const double b1 = Bs[(kk + lk) * LDT + wn + 16 + lr];
+          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+      }
+      // acc[a][b][r] = Hloc(64 I + wm + 16 a + 4 r + lk, 64 J + wn + 16 b + lr)
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int i = TILE * I + wm + 16 * a + 4 * r + lk;
+            const int j = TILE * J + wn + 16 * b + lr;
+            if (i < nc && j < nc) {
+              const size_t mu = ib[i], nu = ib[j];
+              atomicAdd(&Ht[mu + (size_t)n_basis * nu], acc[a][b][r]);
+              if (I != J)
+                atomicAdd(&Ht[nu + (size_t)n_basis * mu], acc[a][b][r]);
+            }
+          }
+    }
+  }
+}
+
 AdjState *build_adj(const Rho1Plan &P) {
   const int nb = P.n_basis;
   const int nt = (nb + TILE - 1) / TILE;
@@ -238,6 +320,13 @@ void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const d
   FRPA_REQUIRE(precision != FRPA_PREC_OZAKI, FRPA_ERR_UNSUPPORTED,
                "the INT8-split adjoint is not available in this build (use FRPA_PREC_FP64)");
   FRPA_REQUIRE(precision == FRPA_PREC_FP64, FRPA_ERR_INVALID, "unknown precision %d", precision);
+  const char *mode = getenv("FRPA_ADJOINT");
+  if (!(mode && strcmp(mode, "tiles") == 0)) {
+    hipLaunchKernelGGL(h1_push_kernel, dim3(P.n_valid, T), dim3(256), 0, s, P.d_desc.ptr, P.d_ibasis0.ptr,
+                       P.d_map.ptr, wave, g, P.n_full, P.n_basis, H);
+    check_launch("h1_push_kernel");
+    return;
+  }
   if (!P.adj)
     P.adj = build_adj(P);
   if (P.adj->n_tasks == 0)

@@ -18,6 +18,7 @@
 // (P1 changes every call): one int64 per element holding its S digits, gathered exactly like the
 // FP64 path gathers P1 (one 8-byte load per element), then byte-transposed into the S A fragments.
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "rho1_common.hpp"
@@ -129,138 +130,193 @@ __device__ __forceinline__ int gather_bytes(unsigned long long e0, unsigned long
 }
 
 // ---- fused forward ------------------------------------------------------------------------
-template <int S, int NPG>
+constexpr int BROW = 80;  // bytes per (digit, point) row of a staged B chunk: 64 k + 16 pad
+
+// One point chunk with exactly NPG groups of 16 points (compile-time loops, no per-group branches),
+// K-chunks of 64 double-buffered in LDS with the next chunk's phi digits prefetched into registers.
+template <int S, int NPG, int PCH>
+__device__ __forceinline__ void ozaki_chunk(signed char *Bs0, signed char *Bs1, int *jbs0, int *jbs1, double *red,
+                                            const BatchDesc &d, int t, int pc, int npc, const int *__restrict__ ib,
+                                            const int *__restrict__ map, const double *__restrict__ W,
+                                            const long long *__restrict__ At, const double *__restrict__ ai,
+                                            const signed char *__restrict__ Bg, const double *__restrict__ binv,
+                                            int n_basis, const double *__restrict__ bias, double *__restrict__ rho,
+                                            int n_full, int nmb, int n_batches_all, int n_tile) {
+  constexpr int NQ = S * NPG * 16 * 4;          // 16-byte pieces of one staged chunk
+  constexpr int QPT = (NQ + 255) / 256;         // per thread
+  const int nc = d.nc, ld = (nc + 3) & ~3, ncK = ceil64(nc);
+  const size_t plane = (size_t)ncK * d.np;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nchunk = ncK / 64;
+  double racc[NPG];
+#pragma unroll
+  for (int g = 0; g < NPG; g++)
+    racc[g] = 0.0;
+
+  for (int i0 = 0; i0 < nc; i0 += 64) {
+    const int irow = i0 + 16 * w + lr;
+    const bool iok = irow < nc;
+    const size_t arow = iok ? (size_t)ib[irow] : 0;
+    v4i acc[S][NPG];
+#pragma unroll
+    for (int l = 0; l < S; l++)
+#pragma unroll
+      for (int g = 0; g < NPG; g++)
+        acc[l][g] = v4i{0, 0, 0, 0};
+
+    v4i bq[QPT];
+    auto load_b = [&](int c) {
+      const int j0 = 64 * c;
+#pragma unroll
+      for (int u = 0; u < QPT; u++) {
+        const int e = tid + 256 * u;
+        const int q = e & 3, p = (e >> 2) % (NPG * 16), sl = (e >> 2) / (NPG * 16);
+        bq[u] = (e < NQ && p < npc)
+                    ? *reinterpret_cast<const v4i *>(Bg + sl * plane + (size_t)ncK * (pc + p) + j0 + 16 * q)
+                    : v4i{0, 0, 0, 0};
+      }
+    };
+    auto store_b = [&](int c, signed char *Bs, int *jbs) {
+#pragma unroll
+      for (int u = 0; u < QPT; u++) {
+        const int e = tid + 256 * u;
+        const int q = e & 3, p = (e >> 2) % (NPG * 16), sl = (e >> 2) / (NPG * 16);
+        if (e < NQ)
+          *reinterpret_cast<v4i *>(&Bs[(sl * PCH + p) * BROW + 16 * q]) = bq[u];
+      }
+      if (tid < 64) {
+        const int j = 64 * c + tid;
+        jbs[tid] = j < nc ? ib[j] : -1;
+      }
+    };
+    __syncthreads();
+    load_b(0);
+    store_b(0, Bs0, jbs0);
+    __syncthreads();
+    for (int c = 0; c < nchunk; c++) {
+      const signed char *Bs = (c & 1) ? Bs1 : Bs0;
+      const int *jbs = (c & 1) ? jbs1 : jbs0;
+      // A fragments: row irow, k = 64 c + 16 lk + q, q < 16 (P1 digits gathered like P1 itself)
+      unsigned long long e[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int jb = jbs[16 * lk + q];
+        e[q] = (iok && jb >= 0) ? (unsigned long long)At[arow + (size_t)n_basis * jb] : 0ull;
+      }
+      if (c + 1 < nchunk)
+        load_b(c + 1);
+      v4i a[S];
+#pragma unroll
+      for (int sl = 0; sl < S; sl++)
+        a[sl] = v4i{gather_bytes(e[0], e[1], e[2], e[3], sl), gather_bytes(e[4], e[5], e[6], e[7], sl),
+                    gather_bytes(e[8], e[9], e[10], e[11], sl), gather_bytes(e[12], e[13], e[14], e[15], sl)};
+#pragma unroll
+      for (int g = 0; g < NPG; g++) {
+        v4i b[S];
+#pragma unroll
+        for (int sl = 0; sl < S; sl++)
+          b[sl] = *reinterpret_cast<const v4i *>(&Bs[(sl * PCH + 16 * g + lr) * BROW + 16 * lk]);
+#pragma unroll
+        for (int sa = 0; sa < S; sa++)
+#pragma unroll
+          for (int sb = 0; sb < S - sa; sb++)
+            acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
+      }
+      if (c + 1 < nchunk)
+        store_b(c + 1, (c & 1) ? Bs0 : Bs1, (c & 1) ? jbs0 : jbs1);
+      __syncthreads();
+    }
+
+    // epilogue: v_mfma_i32_16x16x64_i8 leaves D(4 (lane / 16) + r, lane % 16) in acc[r], so
+    // acc[l][g][r] = C_l(i0 + 16w + 4lk + r, pc + 16g + lr)
+    double rinv[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = i0 + 16 * w + 4 * lk + r;
+      rinv[r] = i < nc ? ai[ib[i]] : 0.0;
+    }
+#pragma unroll
+    for (int g = 0; g < NPG; g++) {
+      const int p = 16 * g + lr;
+      double part = 0.0;
+      if (p < npc) {
+        const double bi = binv[d.pstart + pc + p];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = i0 + 16 * w + 4 * lk + r;
+          if (i < nc) {
+            double wv = (double)acc[S - 1][g][r];
+#pragma unroll
+            for (int l = S - 2; l >= 0; l--)
+              wv = wv * (1.0 / RADIX) + (double)acc[l][g][r];
+            part += wv * rinv[r] * bi * W[i + (size_t)ld * (pc + p)];
+          }
+        }
+      }
+      part += __shfl_xor(part, 16);
+      part += __shfl_xor(part, 32);
+      racc[g] += part;
+    }
+  }
+
+  double(*rd)[PCH] = reinterpret_cast<double(*)[PCH]>(red);
+  __syncthreads();
+  if (lk == 0) {
+#pragma unroll
+    for (int g = 0; g < NPG; g++)
+      rd[w][16 * g + lr] = racc[g];
+  }
+  __syncthreads();
+  for (int p = tid; p < npc; p += 256) {
+    const size_t bi = (size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
+    double sum = rd[0][p] + rd[1][p] + rd[2][p] + rd[3][p];
+    if (bias)
+      sum += bias[bi];
+    if (t < n_tile)
+      rho[(size_t)map[d.pstart + pc + p] + (size_t)n_full * t] = sum;
+  }
+}
+
+template <int S, int MAXPG, int... Gs>
+__device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>, int npg, signed char *Bs0,
+                                               signed char *Bs1, int *jbs0, int *jbs1, double *red,
+                                               const BatchDesc &d, int t, int pc, int npc, const int *ib,
+                                               const int *map, const double *W, const long long *At,
+                                               const double *ai, const signed char *Bg, const double *binv,
+                                               int n_basis, const double *bias, double *rho, int n_full, int nmb,
+                                               int n_batches_all, int n_tile) {
+  ((npg == Gs + 1 ? ozaki_chunk<S, Gs + 1, 16 * MAXPG>(Bs0, Bs1, jbs0, jbs1, red, d, t, pc, npc, ib, map, W, At, ai,
+                                                         Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all,
+                                                         n_tile)
+                  : (void)0),
+   ...);
+}
+
+template <int S, int MAXPG>
 __global__ __launch_bounds__(256) void rho1_ozaki_kernel(
     const BatchDesc *__restrict__ descs, const int *__restrict__ ibasis0, const int *__restrict__ map,
     const double *__restrict__ wave, const long long *__restrict__ adig, const double *__restrict__ ainv,
     const signed char *__restrict__ bdig, const long long *__restrict__ boff, const double *__restrict__ binv,
     int n_basis, const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all,
     int n_tile) {
-  constexpr int PCH = 16 * NPG;
-  constexpr int BROW = 80;  // bytes per (slice, point) row in LDS: 64 k + 16 pad
-  __shared__ __attribute__((aligned(16))) signed char Bs[S * PCH * BROW];
-  __shared__ int jbs[64];
-  __shared__ double red[4][PCH];
-
+  constexpr int PCH = 16 * MAXPG;
+  __shared__ __attribute__((aligned(16))) signed char Bs0[S * PCH * BROW];
+  __shared__ __attribute__((aligned(16))) signed char Bs1[S * PCH * BROW];
+  __shared__ int jbs0[64], jbs1[64];
+  __shared__ double red[4 * PCH];
   const BatchDesc d = descs[blockIdx.x];
   const int t = blockIdx.y;
-  const int nc = d.nc, np_all = d.np;
-  const int ld = (nc + 3) & ~3, ncK = ceil64(nc);
   const double *W = wave + d.wave_off;
   const int *ib = ibasis0 + d.ib_off;
   const long long *At = adig + (size_t)n_basis * n_basis * t;
   const double *ai = ainv + (size_t)n_basis * t;
   const signed char *Bg = bdig + boff[blockIdx.x];
-  const size_t plane = (size_t)ncK * np_all;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
-
-  for (int pc = 0; pc < np_all; pc += PCH) {
-    const int npc = min(PCH, np_all - pc);
+  for (int pc = 0; pc < d.np; pc += PCH) {
+    const int npc = min(PCH, d.np - pc);
     const int npg = (npc + 15) >> 4;
-    double racc[NPG];
-#pragma unroll
-    for (int g = 0; g < NPG; g++)
-      racc[g] = 0.0;
-
-    for (int i0 = 0; i0 < nc; i0 += 64) {
-      const int irow = i0 + 16 * w + lr;
-      const bool iok = irow < nc;
-      const size_t arow = iok ? (size_t)ib[irow] : 0;
-      v4i acc[S][NPG];
-#pragma unroll
-      for (int l = 0; l < S; l++)
-#pragma unroll
-        for (int g = 0; g < NPG; g++)
-          acc[l][g] = v4i{0, 0, 0, 0};
-
-      for (int j0 = 0; j0 < ncK; j0 += 64) {
-        __syncthreads();
-        // stage the S digit planes of phi(j0..j0+63, pc..pc+npc-1): 16 bytes per thread-iteration
-        for (int e = tid; e < S * npg * 16 * 4; e += 256) {
-          const int q = e & 3, p = (e >> 2) % (npg * 16), s = (e >> 2) / (npg * 16);
-          v4i v = v4i{0, 0, 0, 0};
-          if (p < npc)
-            v = *reinterpret_cast<const v4i *>(Bg + s * plane + (size_t)ncK * (pc + p) + j0 + 16 * q);
-          *reinterpret_cast<v4i *>(&Bs[(s * PCH + p) * BROW + 16 * q]) = v;
-        }
-        if (tid < 64) {
-          const int j = j0 + tid;
-          jbs[tid] = j < nc ? ib[j] : -1;
-        }
-        __syncthreads();
-        // A fragments: row irow, k = j0 + 16 lk + q, q < 16
-        unsigned long long e[16];
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-          const int jb = jbs[16 * lk + q];
-          e[q] = (iok && jb >= 0) ? (unsigned long long)At[arow + (size_t)n_basis * jb] : 0ull;
-        }
-        v4i a[S];
-#pragma unroll
-        for (int s = 0; s < S; s++)
-          a[s] = v4i{gather_bytes(e[0], e[1], e[2], e[3], s), gather_bytes(e[4], e[5], e[6], e[7], s),
-                     gather_bytes(e[8], e[9], e[10], e[11], s), gather_bytes(e[12], e[13], e[14], e[15], s)};
-#pragma unroll
-        for (int g = 0; g < NPG; g++) {
-          if (g < npg) {
-            v4i b[S];
-#pragma unroll
-            for (int s = 0; s < S; s++)
-              b[s] = *reinterpret_cast<const v4i *>(&Bs[(s * PCH + 16 * g + lr) * BROW + 16 * lk]);
-#pragma unroll
-            for (int sa = 0; sa < S; sa++)
-#pragma unroll
-              for (int sb = 0; sb < S - sa; sb++)
-                acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
-          }
-        }
-      }
-
-      // epilogue: v_mfma_i32_16x16x64_i8 leaves D(4 (lane / 16) + r, lane % 16) in acc[r], so
-      // acc[l][g][r] = C_l(i0 + 16w + 4lk + r, pc + 16g + lr)
-#pragma unroll
-      for (int g = 0; g < NPG; g++) {
-        if (g < npg) {
-          const int p = 16 * g + lr;
-          double part = 0.0;
-          if (p < npc) {
-            const double bi = binv[d.pstart + pc + p];
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-              const int i = i0 + 16 * w + 4 * lk + r;
-              if (i < nc) {
-                double wv = (double)acc[S - 1][g][r];
-#pragma unroll
-                for (int l = S - 2; l >= 0; l--)
-                  wv = wv * (1.0 / RADIX) + (double)acc[l][g][r];
-                wv *= ai[ib[i]] * bi;
-                part += wv * W[i + (size_t)ld * (pc + p)];
-              }
-            }
-          }
-          part += __shfl_xor(part, 16);
-          part += __shfl_xor(part, 32);
-          racc[g] += part;
-        }
-      }
-    }
-
-    __syncthreads();
-    if (lk == 0) {
-#pragma unroll
-      for (int g = 0; g < NPG; g++)
-        red[w][16 * g + lr] = racc[g];
-    }
-    __syncthreads();
-    for (int p = tid; p < npc; p += 256) {
-      const size_t bi = (size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
-      double s = red[0][p] + red[1][p] + red[2][p] + red[3][p];
-      if (bias)
-        s += bias[bi];
-      if (t < n_tile)
-        rho[(size_t)map[d.pstart + pc + p] + (size_t)n_full * t] = s;
-    }
+    ozaki_dispatch<S, MAXPG>(std::make_integer_sequence<int, MAXPG>{}, npg, Bs0, Bs1, jbs0, jbs1, red, d, t, pc, npc,
+                             ib, map, W, At, ai, Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all, n_tile);
   }
 }
 
@@ -315,10 +371,14 @@ void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   hipLaunchKernelGGL(p1_digits_kernel<S>, dim3(blocks), dim3(256), 0, s, c.P1, st.d_ainv.ptr, P.n_basis, nbb,
                      st.d_adig.ptr);
   check_launch("p1_digits_kernel");
+  // point-chunk width: the largest that keeps the accumulators and staged digits in registers
+  // (96 points for 5 digits, 112 for 4); wider batches take several passes
   if (P.max_np <= 64)
     launch<S, 4>(P, c, s);
+  else if (S == 5 || P.max_np <= 96)
+    launch<S, 6>(P, c, s);
   else
-    launch<S, 8>(P, c, s);
+    launch<S, 7>(P, c, s);
 }
 
 } // namespace

@@ -23,8 +23,14 @@ def _oracle_h1(oracle, prob, g_full, T):
     return oracle.h1(prob, T, 1, T, 1, H, a)
 
 
+@pytest.mark.parametrize("mode", ["atomics", "tiles"])
 @pytest.mark.parametrize("k,args", [(1, (8, 10.0, 64, 100, None)), (3, (40, 16.0, 120, (1, 220), (1, 260)))])
-def test_h1_lda_against_oracle(cuda, oracle, k, args):
+def test_h1_lda_against_oracle(cuda, oracle, k, args, mode, monkeypatch):
+    """Both adjoint modes: per-batch tiles + FP64 atomics (default) and deterministic tile owners."""
+    if mode == "tiles":
+        monkeypatch.setenv("FRPA_ADJOINT", "tiles")
+    else:
+        monkeypatch.delenv("FRPA_ADJOINT", raising=False)
     prob = synth.make_rho1_problem(*args, config_k=k, zero_nc=(2,) if k == 3 else ())
     T = 2
     rng = np.random.default_rng(k)
@@ -37,7 +43,7 @@ def test_h1_lda_against_oracle(cuda, oracle, k, args):
     got = Hd.cpu().numpy().reshape((prob.n_basis, prob.n_basis, T), order="F") - H0.reshape(
         (prob.n_basis, prob.n_basis, T), order="F")
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    print(f"[h1 lda config {k}] rel {err:.3e}", flush=True)
+    print(f"[h1 lda config {k} {mode}] rel {err:.3e}", flush=True)
     assert err <= 1e-12
 
 

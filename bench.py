@@ -167,31 +167,50 @@ def rho1_bytes(prob):
     return float(np.sum((8 * nc * npt + 8 * nc * nc + 4 * nc + 12 * npt)[sel]))
 
 
-def cpu_baseline(D, threads, target_s=15.0):
-    """CPU FP64 oracle on a bounded sample: random batches (rho1) and points (far field)."""
-    import torch
-
+def cpu_baseline(D, threads, rho_s=12.0, far_s=6.0):
+    """CPU FP64 oracle on a bounded sample of the same inputs: random batches (rho1) and the first
+    far-field points, each sized from a short pilot run to take about rho_s / far_s seconds."""
     from fast_rpa_eval_amd import synth
     from oracle import oracle
     oracle.set_threads(threads)
     prob, fp = D["prob"], D["fp"]
     rng = np.random.default_rng(0)
-    nb = min(4096, prob.n_batches)
-    sel = np.sort(rng.choice(prob.n_batches, size=nb, replace=False))
-    sub = subproblem(prob, sel, D["wave"])
-    P1 = D["P1"].view(prob.n_basis, prob.n_basis).T.contiguous().cpu().numpy()  # column-major (symmetric)
-    P1 = np.asfortranarray(P1.T)[:, :, None]
-    t0 = time.time()
-    rho_cpu, _ = oracle.rho1(sub, P1)
-    t_rho = time.time() - t0
+    # the device P1 is column-major: element (r, c) at r + n_basis * c
+    P1 = np.asfortranarray(D["P1"].view(prob.n_basis, prob.n_basis).cpu().numpy().T)[:, :, None]
+    perm = rng.permutation(prob.n_batches)
+
+    def run_rho(nb):
+        sel = np.sort(perm[:nb])
+        sub = subproblem(prob, sel, D["wave"])
+        t0 = time.time()
+        out, _ = oracle.rho1(sub, P1)
+        return sel, sub, out, time.time() - t0
+
+    nb = min(256, prob.n_batches)
+    sel, sub, rho_cpu, t = run_rho(nb)
+    nb2 = int(min(prob.n_batches, max(nb, nb * rho_s / max(t, 1e-3))))
+    if nb2 > nb:
+        sel, sub, rho_cpu, t_rho = run_rho(nb2)
+    else:
+        t_rho = t
+
+    def run_far(n):
+        fsub = synth.FarProblem(**{**fp.__dict__})
+        fsub.i_valid_point_2_i_full_points_map = fp.i_valid_point_2_i_full_points_map[:n].copy()
+        fsub.n_valid_points = n
+        t0 = time.time()
+        out = oracle.far(fsub)
+        return out, time.time() - t0
+
+    nfs = min(fp.n_valid_points, 2000)
+    dv_cpu, t = run_far(nfs)
+    nfs2 = int(min(fp.n_valid_points, max(nfs, nfs * far_s / max(t, 1e-3))))
+    if nfs2 > nfs:
+        nfs = nfs2
+        dv_cpu, t_far = run_far(nfs)
+    else:
+        t_far = t
     pts_rho = int(sub.n_point_batches.sum())
-    nfs = min(fp.n_valid_points, max(2000, int(fp.n_valid_points * 0.03)))
-    fsub = synth.FarProblem(**{**fp.__dict__})
-    fsub.i_valid_point_2_i_full_points_map = fp.i_valid_point_2_i_full_points_map[:nfs].copy()
-    fsub.n_valid_points = nfs
-    t0 = time.time()
-    dv_cpu = oracle.far(fsub)
-    t_far = time.time() - t0
     return dict(rho_pts=pts_rho, rho_s=t_rho, far_pts=nfs, far_s=t_far, sel=sel, sub=sub, rho_cpu=rho_cpu,
                 dv_cpu=dv_cpu, nfs=nfs)
 
@@ -229,6 +248,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--paths", default="fast,fp64")
+    ap.add_argument("--kernels", default="rho1,adjoint,far",
+                    help="subset of the step to time (a partial step is not a bench line)")
     ap.add_argument("--summarize-profile", metavar="DIR",
                     help="only summarise a rocprofv3 --output-format csv directory (kernel stats and/or "
                          "per-kernel counter sums) as a text table and exit")
@@ -285,6 +306,7 @@ def main():
     torch.cuda.synchronize()
 
     names = ["rho1", "adjoint", "far"] if have_adj else ["rho1", "far"]
+    names = [n for n in names if n in a.kernels.split(",")]
     res = {}
     outs = {}
     for path in a.paths.split(","):
@@ -316,6 +338,8 @@ def main():
         line["fp64_ms_per_step"] = res["fp64"]["ms"]
         line["speedup_vs_fp64"] = res["fp64"]["ms"] / fast["ms"]
     line["ms_per_kernel"] = {p: r["per"] for p, r in res.items()}
+    if len(names) < (3 if have_adj else 2):
+        line["partial_step"] = names  # timing experiment: value does not cover the whole step
     if "fast" in outs and "fp64" in outs:
         r_f, r_d = outs["fast"]["rho"], outs["fp64"]["rho"]
         line["rho1_rel_err_vs_fp64"] = float((r_f - r_d).abs().max() / r_d.abs().max())
@@ -389,8 +413,10 @@ def summarize_profile(d):
     import glob
     stats = sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True))
     ctrs = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
-    short = lambda n: (n.split("(")[0].split("<")[0].replace("frpa::(anonymous namespace)::", "")[:48]
-                       + ("<" + n.split("<", 1)[1].split(">")[0] + ">" if "<" in n.split("(")[0] else ""))
+    def short(n):
+        # drop the namespace decoration and the argument list, keep template arguments
+        n = n.replace("(anonymous namespace)::", "").replace("void ", "", 1)
+        return n.split("(")[0].replace(" ", "")
     for f in stats:
         print(f"# kernel stats: {os.path.relpath(f, d)}")
         print(f"{'kernel':60s} {'calls':>6s} {'avg_ms':>10s} {'total_ms':>10s} {'pct':>6s}")

@@ -43,6 +43,49 @@ Rho1Plan::~Rho1Plan() {
     ozaki_release(ozaki);
   if (adj)
     adj_release(adj);
+  delete relabel;
+}
+
+const Relabel &get_relabel(Rho1Plan &P) {
+  if (P.relabel)
+    return *P.relabel;
+  std::unique_ptr<Relabel> R(new Relabel());
+  const int nb = P.n_basis;
+  std::vector<int> order(P.n_valid);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return P.h_desc[a].b < P.h_desc[b].b; });
+  R->h_pi.assign(nb, -1);
+  int next = 0;
+  for (int di : order) {
+    const BatchDesc &d = P.h_desc[di];
+    for (int i = 0; i < d.nc; i++) {
+      const int mu = P.h_ibasis0[d.ib_off + i];
+      if (R->h_pi[mu] < 0)
+        R->h_pi[mu] = next++;
+    }
+  }
+  for (int mu = 0; mu < nb; mu++)
+    if (R->h_pi[mu] < 0)
+      R->h_pi[mu] = next++;
+  R->h_pinv.resize(nb);
+  for (int mu = 0; mu < nb; mu++)
+    R->h_pinv[R->h_pi[mu]] = mu;
+  R->h_srow.resize(P.h_ibasis0.size());
+  R->h_sib.resize(P.h_ibasis0.size());
+  for (const BatchDesc &d : P.h_desc) {
+    const int *ib = P.h_ibasis0.data() + d.ib_off;
+    int *sr = R->h_srow.data() + d.ib_off;
+    std::iota(sr, sr + d.nc, 0);
+    std::sort(sr, sr + d.nc, [&](int a, int b) { return R->h_pi[ib[a]] < R->h_pi[ib[b]]; });
+    for (int k = 0; k < d.nc; k++)
+      R->h_sib[d.ib_off + k] = R->h_pi[ib[sr[k]]];
+  }
+  R->d_pi.upload(R->h_pi);
+  R->d_pinv.upload(R->h_pinv);
+  R->d_srow.upload(R->h_srow.empty() ? std::vector<int>{0} : R->h_srow);
+  R->d_sib.upload(R->h_sib.empty() ? std::vector<int>{0} : R->h_sib);
+  P.relabel = R.release();
+  return *P.relabel;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -89,10 +132,14 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
     }
     desc.push_back(BatchDesc{wave_offset[b], 0, nc, np, prefix[b], b});
   }
+  // processing order: descending work (nc^2 np), or the caller's batch order (neighbouring batches
+  // of a spatial grid partition share basis functions, so concurrent blocks reuse P1 in the caches)
+  const char *order = getenv("FRPA_BATCH_ORDER");
+  if (!(order && strcmp(order, "caller") == 0))
+    std::stable_sort(desc.begin(), desc.end(), [](const BatchDesc &x, const BatchDesc &y) {
+      return (double)x.nc * x.nc * x.np > (double)y.nc * y.nc * y.np;
+    });
   // compact 0-based basis lists, in processing order
-  std::stable_sort(desc.begin(), desc.end(), [](const BatchDesc &x, const BatchDesc &y) {
-    return (double)x.nc * x.nc * x.np > (double)y.nc * y.nc * y.np;
-  });
   long long off = 0;
   for (auto &d : desc) {
     const int b = d.b;

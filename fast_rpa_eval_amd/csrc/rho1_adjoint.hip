@@ -15,12 +15,17 @@
 // That tile-owner kernel is the deterministic mode (FRPA_ADJOINT=tiles).  Its tiles are only as
 // full as the compute sets are contiguous in the basis order, and compact 3-D compute sets never are
 // in any 1-D order (25-40% fill measured on the synthetic clusters), so the default mode instead
-// computes each batch's local H densely (64 x 64 tiles of its own rows, I <= J) and adds it into H
-// with hardware FP64 atomics (global_atomic_add_f64): every MFMA flop is useful; the summation
-// order across batches, and so the last bits, can vary from run to run.
-// Tiles are cut in a relabelled basis order pi (order of first appearance over the caller's batch
-// order), so that functions that share batches share tiles even when the caller's basis numbering
-// is not spatially ordered; results are written back at the caller's indices.
+// computes each batch's local H densely (64 x 64 tiles of its own rows, I <= J) and adds it with
+// hardware FP64 atomics (global_atomic_add_f64): every MFMA flop is useful; the summation order
+// across batches, and so the last bits, can vary from run to run.  The atomics are the cost of that
+// mode, so they are made few and line-friendly: the rows are taken in the relabelled order
+// (get_relabel), only the upper triangle x <= y of the relabelled H is accumulated, into a zeroed
+// scratch matrix Hs, and the MFMA operands are swapped so that the 16 lanes of an atomic run along
+// 16 consecutive rows (one 128-byte line when the labels are contiguous); a final pass adds
+// Hs(min(x, y), max(x, y)) into H(pinv x, pinv y) for every element.
+// Tiles of both modes are cut in the relabelled order pi (first appearance over the caller's batch
+// order), so that functions that share batches share tiles even when the caller's basis numbering is
+// not spatially ordered; results are written back at the caller's indices.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -43,11 +48,11 @@ struct AdjTask {
 };
 
 struct AdjState {
+  bool tiles_built = false;  // work lists of the tile-owner mode
   int n_tasks = 0;
   DevBuf<AdjEntry> d_entries;
   DevBuf<AdjTask> d_tasks;
-  DevBuf<int> d_pi, d_pinv;  // basis relabelling and its inverse
-  DevBuf<int> d_srow;        // per batch (at its ib_off): row indices sorted by pi(i_basis)
+  DevBuf<double> d_hs;       // push mode: upper triangle of the relabelled H, (n_basis, n_basis, T)
 };
 
 void adj_release(AdjState *st) { delete st; }
@@ -156,28 +161,37 @@ __global__ __launch_bounds__(256) void h1_tile_kernel(const AdjTask *__restrict_
       }
 }
 
-// Default mode: one workgroup per batch; local tile pairs (I <= J) of its compute rows; atomics.
+// Default mode: one workgroup per batch; local tile pairs (I <= J) of its sorted compute rows;
+// atomics into the upper triangle of the relabelled scratch Hs.
 __global__ __launch_bounds__(256) void h1_push_kernel(const BatchDesc *__restrict__ descs,
-                                                      const int *__restrict__ ibasis0, const int *__restrict__ map,
-                                                      const double *__restrict__ wave, const double *__restrict__ g,
-                                                      int n_full, int n_basis, double *__restrict__ H) {
+                                                      const int *__restrict__ sib_all, const int *__restrict__ srow_all,
+                                                      const int *__restrict__ map, const double *__restrict__ wave,
+                                                      const double *__restrict__ g, int n_full, int n_basis,
+                                                      double *__restrict__ Hs) {
   constexpr int LDT = TILE + 1;
-  __shared__ double As[KCH * LDT];
-  __shared__ double Bs[KCH * LDT];
+  __shared__ double As[KCH * LDT];  // rows of tile I times g
+  __shared__ double Bs[KCH * LDT];  // rows of tile J
   __shared__ double gk[KCH];
   const BatchDesc d = descs[blockIdx.x];
   const int t = blockIdx.y;
   const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
   const double *W = wave + d.wave_off;
-  const int *ib = ibasis0 + d.ib_off;
+  const int *sib = sib_all + d.ib_off;
+  const int *srow = srow_all + d.ib_off;
   const double *gt = g + (size_t)n_full * t;
-  double *Ht = H + (size_t)n_basis * n_basis * t;
+  double *Ht = Hs + (size_t)n_basis * n_basis * t;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int wm = 32 * (w >> 1), wn = 32 * (w & 1);
   const int nT = (nc + TILE - 1) / TILE;
+  // this thread's staged rows (fixed over the K loop): m = tid % 64
+  const int m = tid % TILE;
   for (int I = 0; I < nT; I++) {
+    const int i_st = TILE * I + m;
+    const int ri = i_st < nc ? srow[i_st] : -1;
     for (int J = I; J < nT; J++) {
+      const int j_st = TILE * J + m;
+      const int rj = j_st < nc ? srow[j_st] : -1;
       v4d acc[2][2];
 #pragma unroll
       for (int a = 0; a < 2; a++)
@@ -191,11 +205,11 @@ __global__ __launch_bounds__(256) void h1_push_kernel(const BatchDesc *__restric
           gk[tid] = p < np ? gt[map[d.pstart + p]] : 0.0;
         }
         __syncthreads();
+#pragma unroll
         for (int q = tid; q < TILE * KCH; q += 256) {
-          const int m = q % TILE, k = q / TILE;
-          const int p = p0 + k, i = TILE * I + m, j = TILE * J + m;
-          As[k * LDT + m] = (i < nc && p < np) ? W[i + (size_t)ld * p] * gk[k] : 0.0;
-          Bs[k * LDT + m] = (j < nc && p < np) ? W[j + (size_t)ld * p] : 0.0;
+          const int k = q / TILE, p = p0 + k;
+          As[k * LDT + m] = (ri >= 0 && p < np) ? W[ri + (size_t)ld * p] * gk[k] : 0.0;
+          Bs[k * LDT + m] = (rj >= 0 && p < np) ? W[rj + (size_t)ld * p] : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -204,64 +218,76 @@ __global__ __launch_bounds__(256) void h1_push_kernel(const BatchDesc *__restric
           const double a1 = As[(kk + lk) * LDT + wm + 16 + lr];
           const double b0 = Bs[(kk + lk) * LDT + wn + lr];
           const double b1 = Bs[(kk + lk) * LDT + wn + 16 + lr];
-          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+          // D = (J rows) x (I rows): the I row runs along lane % 16 in the result
+          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
         }
       }
-      // acc[a][b][r] = Hloc(64 I + wm + 16 a + 4 r + lk, 64 J + wn + 16 b + lr)
+      // acc[a][b][r] = Hloc(i = 64 I + wm + 16 a + lr, j = 64 J + wn + 16 b + 4 r + lk)
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int a = 0; a < 2; a++) {
+        const int i = TILE * I + wm + 16 * a + lr;
+        const size_t x = i < nc ? (size_t)sib[i] : 0;
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            const int i = TILE * I + wm + 16 * a + 4 * r + lk;
-            const int j = TILE * J + wn + 16 * b + lr;
-            if (i < nc && j < nc) {
-              const size_t mu = ib[i], nu = ib[j];
-              atomicAdd(&Ht[mu + (size_t)n_basis * nu], acc[a][b][r]);
-              if (I != J)
-                atomicAdd(&Ht[nu + (size_t)n_basis * mu], acc[a][b][r]);
-            }
+            const int j = TILE * J + wn + 16 * b + 4 * r + lk;
+            if (i < nc && j < nc && i <= j)
+              atomicAdd(&Ht[x + (size_t)n_basis * sib[j]], acc[a][b][r]);
           }
+      }
     }
   }
 }
 
-AdjState *build_adj(const Rho1Plan &P) {
-  const int nb = P.n_basis;
-  const int nt = (nb + TILE - 1) / TILE;
-  // relabel basis functions by first appearance over the batches in the caller's order
-  std::vector<int> order(P.n_valid);
-  for (int i = 0; i < P.n_valid; i++)
-    order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return P.h_desc[a].b < P.h_desc[b].b; });
-  std::vector<int> pi(nb, -1), pinv(nb);
-  int next = 0;
-  for (int di : order) {
-    const BatchDesc &d = P.h_desc[di];
-    for (int i = 0; i < d.nc; i++) {
-      const int mu = P.h_ibasis0[d.ib_off + i];
-      if (pi[mu] < 0)
-        pi[mu] = next++;
+// H(pinv x, pinv y) += Hs(min(x, y), max(x, y)) for every element; 64 x 64 tiles of the relabelled
+// indices, the transposed tile read through LDS so that both reads run down columns.
+__global__ __launch_bounds__(256) void h1_unrelabel_kernel(const double *__restrict__ Hs, const int *__restrict__ pinv,
+                                                           int n_basis, double *__restrict__ H) {
+  __shared__ double tr[64][65];
+  const int x0 = 64 * blockIdx.x, y0 = 64 * blockIdx.y, t = blockIdx.z;
+  const size_t nn = (size_t)n_basis * n_basis;
+  const double *Hst = Hs + nn * t;
+  double *Ht = H + nn * t;
+  const int a = threadIdx.x & 63, b0 = threadIdx.x >> 6;
+  if (x0 > y0 || x0 == y0) {
+    // tr[i][j] = Hs(y0 + j, x0 + i)
+    const int y = y0 + a;
+#pragma unroll 4
+    for (int u = 0; u < 16; u++) {
+      const int i = b0 + 4 * u, x = x0 + i;
+      tr[i][a] = (x < n_basis && y < n_basis) ? Hst[y + (size_t)n_basis * x] : 0.0;
     }
   }
-  for (int mu = 0; mu < nb; mu++)
-    if (pi[mu] < 0)
-      pi[mu] = next++;
-  for (int mu = 0; mu < nb; mu++)
-    pinv[pi[mu]] = mu;
-  std::vector<int> srow(P.h_ibasis0.size());
+  __syncthreads();
+  const int x = x0 + a;
+  if (x >= n_basis)
+    return;
+  const size_t px = pinv[x];
+#pragma unroll 4
+  for (int u = 0; u < 16; u++) {
+    const int j = b0 + 4 * u, y = y0 + j;
+    if (y >= n_basis)
+      break;
+    const double v = x <= y ? Hst[x + (size_t)n_basis * y] : tr[a][j];
+    Ht[px + (size_t)n_basis * pinv[y]] += v;
+  }
+}
+
+void build_tiles(Rho1Plan &P, AdjState &st) {
+  const int nb = P.n_basis;
+  const int nt = (nb + TILE - 1) / TILE;
+  const Relabel &R = get_relabel(P);
+  const std::vector<int> &pi = R.h_pi;
+  const std::vector<int> &srow = R.h_srow;
   std::vector<std::vector<AdjEntry>> per(size_t(nt) * nt);
   for (int di = 0; di < P.n_valid; di++) {
     const BatchDesc &d = P.h_desc[di];
     const int *ib = P.h_ibasis0.data() + d.ib_off;
-    int *sr = srow.data() + d.ib_off;
-    for (int i = 0; i < d.nc; i++)
-      sr[i] = i;
-    std::sort(sr, sr + d.nc, [&](int a, int b) { return pi[ib[a]] < pi[ib[b]]; });
+    const int *sr = srow.data() + d.ib_off;
     // tile runs of the pi-sorted row list
     std::vector<int> tl, r0, r1;
     for (int k = 0; k < d.nc;) {
@@ -301,14 +327,10 @@ AdjState *build_adj(const Rho1Plan &P) {
   std::vector<AdjTask> sorted(tasks.size());
   for (size_t i = 0; i < ord.size(); i++)
     sorted[i] = tasks[ord[i]];
-  AdjState *st = new AdjState();
-  st->n_tasks = (int)sorted.size();
-  st->d_pi.upload(pi);
-  st->d_pinv.upload(pinv);
-  st->d_srow.upload(srow.empty() ? std::vector<int>{0} : srow);
-  st->d_entries.upload(entries.empty() ? std::vector<AdjEntry>{AdjEntry{0, 0, 0, 0, 0}} : entries);
-  st->d_tasks.upload(sorted.empty() ? std::vector<AdjTask>{AdjTask{0, 0, 0, 0}} : sorted);
-  return st;
+  st.n_tasks = (int)sorted.size();
+  st.d_entries.upload(entries.empty() ? std::vector<AdjEntry>{AdjEntry{0, 0, 0, 0, 0}} : entries);
+  st.d_tasks.upload(sorted.empty() ? std::vector<AdjTask>{AdjTask{0, 0, 0, 0}} : sorted);
+  st.tiles_built = true;
 }
 
 } // namespace
@@ -320,20 +342,31 @@ void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const d
   FRPA_REQUIRE(precision != FRPA_PREC_OZAKI, FRPA_ERR_UNSUPPORTED,
                "the INT8-split adjoint is not available in this build (use FRPA_PREC_FP64)");
   FRPA_REQUIRE(precision == FRPA_PREC_FP64, FRPA_ERR_INVALID, "unknown precision %d", precision);
+  if (!P.adj)
+    P.adj = new AdjState();
+  AdjState &st = *P.adj;
+  const Relabel &R = get_relabel(P);
+  const int n = P.n_basis;
   const char *mode = getenv("FRPA_ADJOINT");
   if (!(mode && strcmp(mode, "tiles") == 0)) {
-    hipLaunchKernelGGL(h1_push_kernel, dim3(P.n_valid, T), dim3(256), 0, s, P.d_desc.ptr, P.d_ibasis0.ptr,
-                       P.d_map.ptr, wave, g, P.n_full, P.n_basis, H);
+    const size_t nn = (size_t)n * n * T;
+    st.d_hs.alloc(nn);
+    fill_f64(st.d_hs.ptr, nn, 0.0, s);
+    hipLaunchKernelGGL(h1_push_kernel, dim3(P.n_valid, T), dim3(256), 0, s, P.d_desc.ptr, R.d_sib.ptr, R.d_srow.ptr,
+                       P.d_map.ptr, wave, g, P.n_full, n, st.d_hs.ptr);
     check_launch("h1_push_kernel");
+    hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, T), dim3(256), 0, s, st.d_hs.ptr,
+                       R.d_pinv.ptr, n, H);
+    check_launch("h1_unrelabel_kernel");
     return;
   }
-  if (!P.adj)
-    P.adj = build_adj(P);
-  if (P.adj->n_tasks == 0)
+  if (!st.tiles_built)
+    build_tiles(P, st);
+  if (st.n_tasks == 0)
     return;
-  hipLaunchKernelGGL(h1_tile_kernel, dim3(P.adj->n_tasks, T), dim3(256), 0, s, P.adj->d_tasks.ptr,
-                     P.adj->d_entries.ptr, P.d_desc.ptr, P.d_ibasis0.ptr, P.adj->d_srow.ptr, P.adj->d_pi.ptr,
-                     P.adj->d_pinv.ptr, P.d_map.ptr, wave, g, P.n_full, P.n_basis, H);
+  hipLaunchKernelGGL(h1_tile_kernel, dim3(st.n_tasks, T), dim3(256), 0, s, st.d_tasks.ptr, st.d_entries.ptr,
+                     P.d_desc.ptr, P.d_ibasis0.ptr, R.d_srow.ptr, R.d_pi.ptr, R.d_pinv.ptr, P.d_map.ptr, wave, g,
+                     P.n_full, n, H);
   check_launch("h1_tile_kernel");
 }
 

@@ -16,6 +16,16 @@ struct BatchDesc {
 };
 static_assert(sizeof(BatchDesc) == 32, "BatchDesc layout");
 
+// Basis relabeling: new label pi[mu] = order of first appearance of mu over the batches in the
+// caller's order (a batch's functions in ascending index), so that the compute sets of a spatially
+// ordered batch list become long runs of consecutive labels.  Per batch, its compute rows sorted by
+// new label: srow[k] = local row of the caller's list, sib[k] = pi of that row's basis function
+// (ascending), both compact with the same offsets as Rho1Plan::d_ibasis0.
+struct Relabel {
+  std::vector<int> h_pi, h_pinv, h_srow, h_sib;
+  DevBuf<int> d_pi, d_pinv, d_srow, d_sib;
+};
+
 struct OzakiState;  // INT8 split data (rho1_ozaki.hip)
 struct AdjState;    // basis-tile work lists of the adjoint (rho1_adjoint.hip)
 
@@ -30,6 +40,7 @@ struct Rho1Plan {
   DevBuf<int> d_map;
   OzakiState *ozaki = nullptr;
   AdjState *adj = nullptr;
+  Relabel *relabel = nullptr;       // built on first use (get_relabel)
   ~Rho1Plan();
 };
 
@@ -45,6 +56,7 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
 void rho1_forward_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s);
 void rho1_forward_ozaki(Rho1Plan &P, const Rho1Call &c, int slices, hipStream_t s);
 void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const double *g, double *H, hipStream_t s);
+const Relabel &get_relabel(Rho1Plan &P);
 void ozaki_release(OzakiState *st);
 void adj_release(AdjState *st);
 long long ozaki_prepare_wave(Rho1Plan &P, const double *wave, int slices, hipStream_t s);

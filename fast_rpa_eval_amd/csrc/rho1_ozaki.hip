@@ -1,22 +1,28 @@
 // rho1 forward with the GEMM W = P1loc * phi emulated on INT8 MFMA (Ozaki-style split).
 //
 // Both operands are written as sums of signed 8-bit digits in radix 254:
-//   P1(r, c) * sa_r = sum_{s<S} a_s(r, c) 254^-s,   sa_r = 127 / max_c |P1(r, c)|   (per P1 row)
+//   P(r, c) * sa_r = sum_{s<S} a_s(r, c) 254^-s,   sa_r = 127 / bound_r             (per P row)
 //   phi(i, p) * sb_p = sum_{s<S} b_s(i, p) 254^-s,  sb_p = 127 / max_i |phi(i, p)|  (per point)
 // with round-to-nearest digits: |x| <= 127 and |x - rint(x)| <= 1/2, so every digit fits int8 and
 // carries log2(254) ~ 8 bits.  The digit products a_s * b_u are exact INT8 MFMA GEMMs with exact
-// int32 accumulation (|sum| <= 5 * 1200 * 127^2 < 2^31); the levels s + u <= S - 1 are kept,
-// S(S+1)/2 products, and recombined in FP64:
+// int32 accumulation (|sum| <= 5 * nc * 127^2 < 2^31 for nc < 26000); the levels s + u <= S - 1
+// are kept, S(S+1)/2 products, and recombined in FP64:
 //   W(i, p) = sum_l 254^-l C_l(i, p) / (sa_i sb_p),   C_l = sum_{s+u=l} a_s b_u   (exact int32)
-// The ddot rho(p) = sum_i phi(i, p) W(i, p), the bias and the scatter stay FP64, fused as in the
-// FP64 kernel.  S = 5 (15 INT8 GEMMs) keeps max|d rho1| / max|rho1| ~ 1e-12; S = 4 (10 GEMMs)
-// ~ 5e-10 on the synthetic workloads.
+// S = 5 (15 INT8 GEMMs) keeps max|d rho1| / max|rho1| ~ 1e-12; S = 4 (10 GEMMs) ~ 5e-10 on the
+// synthetic workloads.
 //
-// Data: the phi digits are prepared once per wave (frpa_rho1_prepare_wave) as S int8 planes per
-// batch, (ncK, np) column-major with ncK = nc rounded up to 64 (zero rows), so a lane's B
-// fragment (16 consecutive k of one point) is one 16-byte load.  The P1 digits are rebuilt per call
-// (P1 changes every call): one int64 per element holding its S digits, gathered exactly like the
-// FP64 path gathers P1 (one 8-byte load per element), then byte-transposed into the S A fragments.
+// Symmetry: rho(p) = phi^T P phi only sees the symmetric part Ps = (P1 + P1^T) / 2, so with the
+// batch's compute rows in ascending order
+//   rho(p) = 2 sum_i phi_i(p) sum_{k >= i} P'(i, k) phi_k(p),   P'(i, k) = Ps(i, k) (k > i), Ps(i, i) / 2
+// and only the 64 x 64 tile pairs J >= I of P' are multiplied (half the MFMAs and half the P1
+// gathers of the full product).  The ddot, the bias and the scatter stay FP64.
+//
+// Data: P1 is relabeled by first appearance over the batches (get_relabel), so a batch's compute set
+// is a few long runs of consecutive labels and the per-tile gathers touch few cache lines.  Per call
+// the P' digits are written as one int64 per element (its S digits) in the new labels; the phi
+// digits are prepared once per wave (frpa_rho1_prepare_wave) as S int8 planes per batch, rows in the
+// batch's sorted order, (ncK, np) column-major with ncK = nc rounded up to 64 (zero rows), so a
+// lane's B fragment (16 consecutive k of one point) is one 16-byte load.
 #include <algorithm>
 #include <utility>
 #include <vector>
@@ -35,9 +41,9 @@ struct OzakiState {
   DevBuf<signed char> d_bdig;       // phi digits
   DevBuf<double> d_binv;            // 1 / sb_p per valid point (indexed by pstart + p)
   // per call
-  DevBuf<long long> d_adig;         // P1 digits, (n_basis, n_basis, T)
-  DevBuf<double> d_ainv;            // 1 / sa_r, (n_basis, T)
-  DevBuf<unsigned long long> d_rowbits;  // row maxima of |P1| as IEEE bits
+  DevBuf<long long> d_adig;         // P' digits in the new labels, (n_basis, n_basis, T)
+  DevBuf<double> d_ainv;            // 1 / sa in the new labels, (n_basis, T)
+  DevBuf<unsigned long long> d_rowbits;  // bound_r = max(max_c |P1(r, c)|, max_c |P1(c, r)|), IEEE bits
   int n_valid_points = 0;
 };
 
@@ -49,38 +55,54 @@ constexpr double RADIX = 254.0;
 
 __device__ __forceinline__ int ceil64(int x) { return (x + 63) & ~63; }
 
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
 // ---- phi digits (one-off) -------------------------------------------------------------------
+// One wave per point: lanes run over the batch's rows in sorted order (row k is the caller's local
+// row srow[k]); the digit planes are written 64 consecutive bytes at a time.
 template <int S>
 __global__ __launch_bounds__(256) void phi_digits_kernel(const BatchDesc *__restrict__ descs,
                                                          const long long *__restrict__ boff,
+                                                         const int *__restrict__ srow_all,
                                                          const double *__restrict__ wave,
                                                          signed char *__restrict__ bdig, double *__restrict__ binv) {
   const BatchDesc d = descs[blockIdx.x];
   const int ld = (d.nc + 3) & ~3, ncK = ceil64(d.nc);
   const double *W = wave + d.wave_off;
+  const int *srow = srow_all + d.ib_off;
   signed char *out = bdig + boff[blockIdx.x];
   const size_t plane = (size_t)ncK * d.np;
-  for (int p = threadIdx.x; p < d.np; p += blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  for (int p = threadIdx.x >> 6; p < d.np; p += 4) {
+    const double *Wp = W + (size_t)ld * p;
     double mx = 0.0;
-    for (int i = 0; i < d.nc; i++)
-      mx = fmax(mx, fabs(W[i + (size_t)ld * p]));
+    for (int k = lane; k < d.nc; k += 64)
+      mx = fmax(mx, fabs(Wp[srow[k]]));
+    mx = wave_max(mx);
     const double sb = mx > 0.0 ? 127.0 / mx : 1.0;
-    binv[d.pstart + p] = 1.0 / sb;
-    for (int i = 0; i < ncK; i++) {
-      double r = i < d.nc ? W[i + (size_t)ld * p] * sb : 0.0;
+    if (lane == 0)
+      binv[d.pstart + p] = 1.0 / sb;
+    for (int k = lane; k < ncK; k += 64) {
+      double r = k < d.nc ? Wp[srow[k]] * sb : 0.0;
 #pragma unroll
-      for (int s = 0; s < S; s++) {
+      for (int sl = 0; sl < S; sl++) {
         const double dg = rint(r);
-        out[s * plane + (size_t)ncK * p + i] = (signed char)dg;
+        out[sl * plane + (size_t)ncK * p + k] = (signed char)dg;
         r = (r - dg) * RADIX;
       }
     }
   }
 }
 
-// ---- P1 digits (per call) -------------------------------------------------------------------
-// max_c |P1(r, c, t)|: column chunks per block, combined with atomicMax on the bit pattern
-// (non-negative doubles order like their IEEE bits); rowbits must start at 0.
+// ---- P' digits (per call) -------------------------------------------------------------------
+// bound_r: row maxima (column chunks per block) and column maxima (one block per column), combined
+// with atomicMax on the bit pattern (non-negative doubles order like their IEEE bits); rowbits
+// starts at 0.  For a symmetric P1 both give max_c |P1(r, c)|.
 __global__ void p1_rowmax_kernel(const double *__restrict__ P1, int n_basis, int cols_per_block,
                                  unsigned long long *__restrict__ rowbits) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,29 +117,79 @@ __global__ void p1_rowmax_kernel(const double *__restrict__ P1, int n_basis, int
   atomicMax(&rowbits[r + (size_t)n_basis * t], (unsigned long long)__double_as_longlong(mx));
 }
 
-__global__ void p1_rowscale_kernel(const unsigned long long *__restrict__ rowbits, size_t n,
-                                   double *__restrict__ ainv) {
+__global__ __launch_bounds__(256) void p1_colmax_kernel(const double *__restrict__ P1, int n_basis,
+                                                        unsigned long long *__restrict__ rowbits) {
+  __shared__ double red[4];
+  const int c = blockIdx.x, t = blockIdx.y;
+  const double *col = P1 + (size_t)n_basis * n_basis * t + (size_t)n_basis * c;
+  double mx = 0.0;
+  for (int r = threadIdx.x; r < n_basis; r += 256)
+    mx = fmax(mx, fabs(col[r]));
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax(&rowbits[c + (size_t)n_basis * t], (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+// 1 / sa in the new labels: ainv[x + n t] = bound[pinv[x] + n t] / 127
+__global__ void p1_rowscale_kernel(const unsigned long long *__restrict__ rowbits, const int *__restrict__ pinv,
+                                   int n_basis, size_t n, double *__restrict__ ainv) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n)
     return;
-  const double mx = __longlong_as_double((long long)rowbits[i]);
+  const size_t x = i % n_basis, t = i / n_basis;
+  const double mx = __longlong_as_double((long long)rowbits[pinv[x] + (size_t)n_basis * t]);
   ainv[i] = mx > 0.0 ? mx / 127.0 : 1.0;
 }
 
+// adig[x + n y] = digits of P'(pinv[x], pinv[y]) / ainv[x]: one 64 x 64 tile of the new labels per
+// block; the transposed tile P1(pinv[y], pinv[x]) goes through LDS so both reads run down columns.
 template <int S>
-__global__ void p1_digits_kernel(const double *__restrict__ P1, const double *__restrict__ ainv, int n_basis,
-                                 size_t n_total, long long *__restrict__ adig) {
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n_total; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t r = e % n_basis, t = e / ((size_t)n_basis * n_basis);
-    double x = P1[e] / ainv[r + (size_t)n_basis * t];
+__global__ __launch_bounds__(256) void p1_digits_kernel(const double *__restrict__ P1, const int *__restrict__ pinv,
+                                                        const double *__restrict__ ainv, int n_basis,
+                                                        long long *__restrict__ adig) {
+  __shared__ double tr[64][65];
+  const int x0 = 64 * blockIdx.x, y0 = 64 * blockIdx.y, t = blockIdx.z;
+  const size_t nn = (size_t)n_basis * n_basis;
+  const double *Pt = P1 + nn * t;
+  const int a = threadIdx.x & 63, b0 = threadIdx.x >> 6;
+  // transposed read: thread (j = a, i = b0 + 4u) loads P1(pinv[y0 + j], pinv[x0 + i])
+  {
+    const int y = y0 + a;
+    const int cy = y < n_basis ? pinv[y] : 0;
+#pragma unroll 4
+    for (int u = 0; u < 16; u++) {
+      const int i = b0 + 4 * u, x = x0 + i;
+      tr[i][a] = (x < n_basis && y < n_basis) ? Pt[cy + (size_t)n_basis * pinv[x]] : 0.0;
+    }
+  }
+  __syncthreads();
+  const int x = x0 + a;
+  if (x >= n_basis)
+    return;
+  const int rx = pinv[x];
+  const double inv = 1.0 / ainv[x + (size_t)n_basis * t];
+#pragma unroll 4
+  for (int u = 0; u < 16; u++) {
+    const int j = b0 + 4 * u, y = y0 + j;
+    if (y >= n_basis)
+      break;
+    double v = 0.5 * (Pt[rx + (size_t)n_basis * pinv[y]] + tr[a][j]);
+    if (x == y)
+      v *= 0.5;
+    double z = v * inv;
     unsigned long long packed = 0;
 #pragma unroll
-    for (int s = 0; s < S; s++) {
-      const double dg = rint(x);
-      packed |= (unsigned long long)(unsigned char)(signed char)dg << (8 * s);
-      x = (x - dg) * RADIX;
+    for (int sl = 0; sl < S; sl++) {
+      const double dg = rint(z);
+      packed |= (unsigned long long)(unsigned char)(signed char)dg << (8 * sl);
+      z = (z - dg) * RADIX;
     }
-    adig[e] = (long long)packed;
+    adig[x + (size_t)n_basis * y + nn * t] = (long long)packed;
   }
 }
 
@@ -130,127 +202,155 @@ __device__ __forceinline__ int gather_bytes(unsigned long long e0, unsigned long
 }
 
 // ---- fused forward ------------------------------------------------------------------------
+// Block: 8 waves; wave w owns rows 16 (w & 3) .. +15 of each 64-row tile I and one half of the
+// block's point chunk (w >> 2): up to 2 * 16 * NPG points per pass, so the P' digits of a batch
+// are gathered once for batches of up to 32 NPG points.  For row tile I the K loop runs over the
+// column tiles J = I .. nT - 1 (chunks of 64 sorted rows); the phi digits of chunk J + 1 are staged
+// through registers into the other LDS buffer and the P' digits of chunk J + 1 are gathered into
+// registers while chunk J's MFMAs run (one barrier per chunk).
 constexpr int BROW = 80;  // bytes per (digit, point) row of a staged B chunk: 64 k + 16 pad
+constexpr int OZ_THREADS = 512;
 
-// One point chunk with exactly NPG groups of 16 points (compile-time loops, no per-group branches),
-// K-chunks of 64 double-buffered in LDS with the next chunk's phi digits prefetched into registers.
-template <int S, int NPG, int PCH>
-__device__ __forceinline__ void ozaki_chunk(signed char *Bs0, signed char *Bs1, int *jbs0, int *jbs1, double *red,
-                                            const BatchDesc &d, int t, int pc, int npc, const int *__restrict__ ib,
-                                            const int *__restrict__ map, const double *__restrict__ W,
-                                            const long long *__restrict__ At, const double *__restrict__ ai,
-                                            const signed char *__restrict__ Bg, const double *__restrict__ binv,
-                                            int n_basis, const double *__restrict__ bias, double *__restrict__ rho,
-                                            int n_full, int nmb, int n_batches_all, int n_tile) {
-  constexpr int NQ = S * NPG * 16 * 4;          // 16-byte pieces of one staged chunk
-  constexpr int QPT = (NQ + 255) / 256;         // per thread
+template <int S>
+__device__ __forceinline__ void repack_a(const unsigned long long (&e)[16], v4i (&a)[S]) {
+#pragma unroll
+  for (int sl = 0; sl < S; sl++)
+    a[sl] = v4i{gather_bytes(e[0], e[1], e[2], e[3], sl), gather_bytes(e[4], e[5], e[6], e[7], sl),
+                gather_bytes(e[8], e[9], e[10], e[11], sl), gather_bytes(e[12], e[13], e[14], e[15], sl)};
+}
+
+// One point chunk; this wave multiplies NG (exact, compile-time) groups of 16 points starting at
+// group gp0 of the chunk.  Every wave of the block runs the same barriers whatever its NG.
+template <int S, int NPG, int NG>
+__device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *red, const BatchDesc &d, int t,
+                                            int pc, int npc, int gp0, const int *__restrict__ sib,
+                                            const int *__restrict__ srow, const int *__restrict__ map,
+                                            const double *__restrict__ W, const long long *__restrict__ At,
+                                            const double *__restrict__ ai, const signed char *__restrict__ Bg,
+                                            const double *__restrict__ binv, int n_basis,
+                                            const double *__restrict__ bias, double *__restrict__ rho, int n_full,
+                                            int nmb, int n_batches_all, int n_tile) {
+  constexpr int PCH = 32 * NPG;                 // point slots per chunk
+  constexpr int BUF = S * PCH * BROW;           // bytes per LDS buffer
+  constexpr int NQ = S * PCH * 4;               // 16-byte pieces of one staged chunk
+  constexpr int QPT = (NQ + OZ_THREADS - 1) / OZ_THREADS;
   const int nc = d.nc, ld = (nc + 3) & ~3, ncK = ceil64(nc);
   const size_t plane = (size_t)ncK * d.np;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rg = w & 3;
   const int lr = lane & 15, lk = lane >> 4;
-  const int nchunk = ncK / 64;
-  double racc[NPG];
+  const int nT = ncK / 64;
+  double racc[NG > 0 ? NG : 1];
 #pragma unroll
-  for (int g = 0; g < NPG; g++)
+  for (int g = 0; g < NG; g++)
     racc[g] = 0.0;
 
-  for (int i0 = 0; i0 < nc; i0 += 64) {
-    const int irow = i0 + 16 * w + lr;
+  v4i bq[QPT];
+  auto load_b = [&](int c) {
+    const int j0 = 64 * c;
+#pragma unroll
+    for (int u = 0; u < QPT; u++) {
+      const int e = tid + OZ_THREADS * u;
+      const int q = e & 3, p = (e >> 2) % PCH, sl = (e >> 2) / PCH;
+      bq[u] = (e < NQ && p < npc)
+                  ? *reinterpret_cast<const v4i *>(Bg + sl * plane + (size_t)ncK * (pc + p) + j0 + 16 * q)
+                  : v4i{0, 0, 0, 0};
+    }
+  };
+  auto store_b = [&](int c, int buf) {
+    signed char *B = Bs + buf * BUF;
+#pragma unroll
+    for (int u = 0; u < QPT; u++) {
+      const int e = tid + OZ_THREADS * u;
+      const int q = e & 3, p = (e >> 2) % PCH, sl = (e >> 2) / PCH;
+      if (e < NQ)
+        *reinterpret_cast<v4i *>(&B[(sl * PCH + p) * BROW + 16 * q]) = bq[u];
+    }
+    if (tid < 64) {
+      const int j = 64 * c + tid;
+      jbs[64 * buf + tid] = j < nc ? sib[j] : -1;
+    }
+  };
+
+  for (int I = 0; I < nT; I++) {
+    const int irow = 64 * I + 16 * rg + lr;
     const bool iok = irow < nc;
-    const size_t arow = iok ? (size_t)ib[irow] : 0;
-    v4i acc[S][NPG];
+    const size_t arow = iok ? (size_t)sib[irow] : 0;
+    v4i acc[S][NG > 0 ? NG : 1];
 #pragma unroll
     for (int l = 0; l < S; l++)
 #pragma unroll
-      for (int g = 0; g < NPG; g++)
+      for (int g = 0; g < NG; g++)
         acc[l][g] = v4i{0, 0, 0, 0};
 
-    v4i bq[QPT];
-    auto load_b = [&](int c) {
-      const int j0 = 64 * c;
-#pragma unroll
-      for (int u = 0; u < QPT; u++) {
-        const int e = tid + 256 * u;
-        const int q = e & 3, p = (e >> 2) % (NPG * 16), sl = (e >> 2) / (NPG * 16);
-        bq[u] = (e < NQ && p < npc)
-                    ? *reinterpret_cast<const v4i *>(Bg + sl * plane + (size_t)ncK * (pc + p) + j0 + 16 * q)
-                    : v4i{0, 0, 0, 0};
-      }
-    };
-    auto store_b = [&](int c, signed char *Bs, int *jbs) {
-#pragma unroll
-      for (int u = 0; u < QPT; u++) {
-        const int e = tid + 256 * u;
-        const int q = e & 3, p = (e >> 2) % (NPG * 16), sl = (e >> 2) / (NPG * 16);
-        if (e < NQ)
-          *reinterpret_cast<v4i *>(&Bs[(sl * PCH + p) * BROW + 16 * q]) = bq[u];
-      }
-      if (tid < 64) {
-        const int j = 64 * c + tid;
-        jbs[tid] = j < nc ? ib[j] : -1;
-      }
-    };
-    __syncthreads();
-    load_b(0);
-    store_b(0, Bs0, jbs0);
-    __syncthreads();
-    for (int c = 0; c < nchunk; c++) {
-      const signed char *Bs = (c & 1) ? Bs1 : Bs0;
-      const int *jbs = (c & 1) ? jbs1 : jbs0;
-      // A fragments: row irow, k = 64 c + 16 lk + q, q < 16 (P1 digits gathered like P1 itself)
-      unsigned long long e[16];
+    unsigned long long e[16];
+    // A fragments: row irow, k = 64 c + 16 lk + q, q < 16; P'(i, k) = 0 below the diagonal (k < i)
+    auto load_a = [&](int c, int buf) {
 #pragma unroll
       for (int q = 0; q < 16; q++) {
-        const int jb = jbs[16 * lk + q];
-        e[q] = (iok && jb >= 0) ? (unsigned long long)At[arow + (size_t)n_basis * jb] : 0ull;
+        const int jb = jbs[64 * buf + 16 * lk + q];
+        const bool upper = c > I || 64 * c + 16 * lk + q >= irow;
+        e[q] = (iok && jb >= 0 && upper) ? (unsigned long long)At[arow + (size_t)n_basis * jb] : 0ull;
       }
-      if (c + 1 < nchunk)
+    };
+    __syncthreads();  // the previous tile is done with both buffers
+    load_b(I);
+    store_b(I, 0);
+    __syncthreads();
+    v4i a[S];
+    load_a(I, 0);
+    repack_a<S>(e, a);
+    for (int c = I; c < nT; c++) {
+      const int cur = (c - I) & 1;
+      const bool more = c + 1 < nT;
+      if (more)
         load_b(c + 1);
-      v4i a[S];
+      const signed char *B = Bs + cur * BUF;
 #pragma unroll
-      for (int sl = 0; sl < S; sl++)
-        a[sl] = v4i{gather_bytes(e[0], e[1], e[2], e[3], sl), gather_bytes(e[4], e[5], e[6], e[7], sl),
-                    gather_bytes(e[8], e[9], e[10], e[11], sl), gather_bytes(e[12], e[13], e[14], e[15], sl)};
-#pragma unroll
-      for (int g = 0; g < NPG; g++) {
+      for (int g = 0; g < NG; g++) {
         v4i b[S];
 #pragma unroll
         for (int sl = 0; sl < S; sl++)
-          b[sl] = *reinterpret_cast<const v4i *>(&Bs[(sl * PCH + 16 * g + lr) * BROW + 16 * lk]);
+          b[sl] = *reinterpret_cast<const v4i *>(&B[(sl * PCH + 16 * (gp0 + g) + lr) * BROW + 16 * lk]);
 #pragma unroll
         for (int sa = 0; sa < S; sa++)
 #pragma unroll
           for (int sb = 0; sb < S - sa; sb++)
             acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
       }
-      if (c + 1 < nchunk)
-        store_b(c + 1, (c & 1) ? Bs0 : Bs1, (c & 1) ? jbs0 : jbs1);
+      if (more)
+        store_b(c + 1, cur ^ 1);
       __syncthreads();
+      if (more) {
+        load_a(c + 1, cur ^ 1);
+        repack_a<S>(e, a);
+      }
     }
 
     // epilogue: v_mfma_i32_16x16x64_i8 leaves D(4 (lane / 16) + r, lane % 16) in acc[r], so
-    // acc[l][g][r] = C_l(i0 + 16w + 4lk + r, pc + 16g + lr)
+    // acc[l][g][r] = C_l(64 I + 16 rg + 4 lk + r, pc + 16 (gp0 + g) + lr)
     double rinv[4];
+    int rrow[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int i = i0 + 16 * w + 4 * lk + r;
-      rinv[r] = i < nc ? ai[ib[i]] : 0.0;
+      const int i = 64 * I + 16 * rg + 4 * lk + r;
+      rinv[r] = i < nc ? ai[sib[i]] : 0.0;
+      rrow[r] = i < nc ? srow[i] : 0;
     }
 #pragma unroll
-    for (int g = 0; g < NPG; g++) {
-      const int p = 16 * g + lr;
+    for (int g = 0; g < NG; g++) {
+      const int p = 16 * (gp0 + g) + lr;
       double part = 0.0;
       if (p < npc) {
         const double bi = binv[d.pstart + pc + p];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int i = i0 + 16 * w + 4 * lk + r;
+          const int i = 64 * I + 16 * rg + 4 * lk + r;
           if (i < nc) {
             double wv = (double)acc[S - 1][g][r];
 #pragma unroll
             for (int l = S - 2; l >= 0; l--)
               wv = wv * (1.0 / RADIX) + (double)acc[l][g][r];
-            part += wv * rinv[r] * bi * W[i + (size_t)ld * (pc + p)];
+            part += wv * rinv[r] * bi * W[rrow[r] + (size_t)ld * (pc + p)];
           }
         }
       }
@@ -264,13 +364,13 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs0, signed char *Bs1, 
   __syncthreads();
   if (lk == 0) {
 #pragma unroll
-    for (int g = 0; g < NPG; g++)
-      rd[w][16 * g + lr] = racc[g];
+    for (int g = 0; g < NG; g++)
+      rd[rg][16 * (gp0 + g) + lr] = racc[g];
   }
   __syncthreads();
-  for (int p = tid; p < npc; p += 256) {
+  for (int p = tid; p < npc; p += OZ_THREADS) {
     const size_t bi = (size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
-    double sum = rd[0][p] + rd[1][p] + rd[2][p] + rd[3][p];
+    double sum = 2.0 * (rd[0][p] + rd[1][p] + rd[2][p] + rd[3][p]);
     if (bias)
       sum += bias[bi];
     if (t < n_tile)
@@ -278,45 +378,46 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs0, signed char *Bs1, 
   }
 }
 
-template <int S, int MAXPG, int... Gs>
-__device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>, int npg, signed char *Bs0,
-                                               signed char *Bs1, int *jbs0, int *jbs1, double *red,
-                                               const BatchDesc &d, int t, int pc, int npc, const int *ib,
-                                               const int *map, const double *W, const long long *At,
-                                               const double *ai, const signed char *Bg, const double *binv,
-                                               int n_basis, const double *bias, double *rho, int n_full, int nmb,
-                                               int n_batches_all, int n_tile) {
-  ((npg == Gs + 1 ? ozaki_chunk<S, Gs + 1, 16 * MAXPG>(Bs0, Bs1, jbs0, jbs1, red, d, t, pc, npc, ib, map, W, At, ai,
-                                                         Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all,
-                                                         n_tile)
-                  : (void)0),
+template <int S, int NPG, int... Gs>
+__device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>, int ng, signed char *Bs, int *jbs,
+                                               double *red, const BatchDesc &d, int t, int pc, int npc, int gp0,
+                                               const int *sib, const int *srow, const int *map, const double *W,
+                                               const long long *At, const double *ai, const signed char *Bg,
+                                               const double *binv, int n_basis, const double *bias, double *rho,
+                                               int n_full, int nmb, int n_batches_all, int n_tile) {
+  ((ng == Gs ? ozaki_chunk<S, NPG, Gs>(Bs, jbs, red, d, t, pc, npc, gp0, sib, srow, map, W, At, ai, Bg, binv, n_basis,
+                                       bias, rho, n_full, nmb, n_batches_all, n_tile)
+             : (void)0),
    ...);
 }
 
-template <int S, int MAXPG>
-__global__ __launch_bounds__(256) void rho1_ozaki_kernel(
-    const BatchDesc *__restrict__ descs, const int *__restrict__ ibasis0, const int *__restrict__ map,
-    const double *__restrict__ wave, const long long *__restrict__ adig, const double *__restrict__ ainv,
-    const signed char *__restrict__ bdig, const long long *__restrict__ boff, const double *__restrict__ binv,
-    int n_basis, const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all,
-    int n_tile) {
-  constexpr int PCH = 16 * MAXPG;
-  __shared__ __attribute__((aligned(16))) signed char Bs0[S * PCH * BROW];
-  __shared__ __attribute__((aligned(16))) signed char Bs1[S * PCH * BROW];
-  __shared__ int jbs0[64], jbs1[64];
+template <int S, int NPG>
+__global__ __launch_bounds__(OZ_THREADS) void rho1_ozaki_kernel(
+    const BatchDesc *__restrict__ descs, const int *__restrict__ sib_all, const int *__restrict__ srow_all,
+    const int *__restrict__ map, const double *__restrict__ wave, const long long *__restrict__ adig,
+    const double *__restrict__ ainv, const signed char *__restrict__ bdig, const long long *__restrict__ boff,
+    const double *__restrict__ binv, int n_basis, const double *__restrict__ bias, double *__restrict__ rho,
+    int n_full, int nmb, int n_batches_all, int n_tile) {
+  constexpr int PCH = 32 * NPG;
+  __shared__ __attribute__((aligned(16))) signed char Bs[2 * S * PCH * BROW];
+  __shared__ int jbs[128];
   __shared__ double red[4 * PCH];
   const BatchDesc d = descs[blockIdx.x];
   const int t = blockIdx.y;
   const double *W = wave + d.wave_off;
-  const int *ib = ibasis0 + d.ib_off;
+  const int *sib = sib_all + d.ib_off;
+  const int *srow = srow_all + d.ib_off;
   const long long *At = adig + (size_t)n_basis * n_basis * t;
   const double *ai = ainv + (size_t)n_basis * t;
   const signed char *Bg = bdig + boff[blockIdx.x];
+  const int half = threadIdx.x >> 8;
   for (int pc = 0; pc < d.np; pc += PCH) {
     const int npc = min(PCH, d.np - pc);
-    const int npg = (npc + 15) >> 4;
-    ozaki_dispatch<S, MAXPG>(std::make_integer_sequence<int, MAXPG>{}, npg, Bs0, Bs1, jbs0, jbs1, red, d, t, pc, npc,
-                             ib, map, W, At, ai, Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all, n_tile);
+    const int ngt = (npc + 15) >> 4;       // groups of 16 points in this chunk
+    const int n0 = (ngt + 1) >> 1;         // the first half takes the extra group
+    const int ng = half ? ngt - n0 : n0, gp0 = half ? n0 : 0;
+    ozaki_dispatch<S, NPG>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc, gp0, sib,
+                           srow, map, W, At, ai, Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all, n_tile);
   }
 }
 
@@ -334,8 +435,9 @@ void prepare(Rho1Plan &P, OzakiState &st, const double *wave, hipStream_t s) {
     nvp = std::max(nvp, d.pstart + d.np);
   st.n_valid_points = nvp;
   st.d_binv.alloc(std::max(nvp, 1));
-  hipLaunchKernelGGL(phi_digits_kernel<S>, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, st.d_boff.ptr, wave,
-                     st.d_bdig.ptr, st.d_binv.ptr);
+  const Relabel &R = get_relabel(P);
+  hipLaunchKernelGGL(phi_digits_kernel<S>, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, st.d_boff.ptr,
+                     R.d_srow.ptr, wave, st.d_bdig.ptr, st.d_binv.ptr);
   check_launch("phi_digits_kernel");
   st.S = S;
   st.wave = wave;
@@ -345,8 +447,9 @@ void prepare(Rho1Plan &P, OzakiState &st, const double *wave, hipStream_t s) {
 template <int S, int NPG>
 void launch(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   OzakiState &st = *P.ozaki;
-  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG>), dim3(P.n_valid, c.T), dim3(256), 0, s, P.d_desc.ptr,
-                     P.d_ibasis0.ptr, P.d_map.ptr, c.wave, st.d_adig.ptr, st.d_ainv.ptr, st.d_bdig.ptr,
+  const Relabel &R = get_relabel(P);
+  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG>), dim3(P.n_valid, c.T), dim3(OZ_THREADS), 0, s, P.d_desc.ptr,
+                     R.d_sib.ptr, R.d_srow.ptr, P.d_map.ptr, c.wave, st.d_adig.ptr, st.d_ainv.ptr, st.d_bdig.ptr,
                      st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full, P.nmb, P.n_batches, c.n_tile);
   check_launch("rho1_ozaki_kernel");
 }
@@ -354,31 +457,36 @@ void launch(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
 template <int S>
 void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   OzakiState &st = *P.ozaki;
-  const size_t nbb = (size_t)P.n_basis * P.n_basis * c.T;
-  const size_t nr = (size_t)P.n_basis * c.T;
+  const Relabel &R = get_relabel(P);
+  const int n = P.n_basis;
+  const size_t nbb = (size_t)n * n * c.T;
+  const size_t nr = (size_t)n * c.T;
   st.d_adig.alloc(nbb);
   st.d_ainv.alloc(nr);
   st.d_rowbits.alloc(nr);
   fill_f64(reinterpret_cast<double *>(st.d_rowbits.ptr), nr, 0.0, s);  // +0.0 has all-zero bits
   const int chunk = 256;
-  hipLaunchKernelGGL(p1_rowmax_kernel, dim3((P.n_basis + 255) / 256, (P.n_basis + chunk - 1) / chunk, c.T),
-                     dim3(256), 0, s, c.P1, P.n_basis, chunk, st.d_rowbits.ptr);
+  hipLaunchKernelGGL(p1_rowmax_kernel, dim3((n + 255) / 256, (n + chunk - 1) / chunk, c.T), dim3(256), 0, s, c.P1,
+                     n, chunk, st.d_rowbits.ptr);
   check_launch("p1_rowmax_kernel");
-  hipLaunchKernelGGL(p1_rowscale_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, st.d_rowbits.ptr, nr,
-                     st.d_ainv.ptr);
+  hipLaunchKernelGGL(p1_colmax_kernel, dim3(n, c.T), dim3(256), 0, s, c.P1, n, st.d_rowbits.ptr);
+  check_launch("p1_colmax_kernel");
+  hipLaunchKernelGGL(p1_rowscale_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, st.d_rowbits.ptr,
+                     R.d_pinv.ptr, n, nr, st.d_ainv.ptr);
   check_launch("p1_rowscale_kernel");
-  const unsigned blocks = (unsigned)std::min<size_t>((nbb + 255) / 256, 65536);
-  hipLaunchKernelGGL(p1_digits_kernel<S>, dim3(blocks), dim3(256), 0, s, c.P1, st.d_ainv.ptr, P.n_basis, nbb,
-                     st.d_adig.ptr);
+  hipLaunchKernelGGL(p1_digits_kernel<S>, dim3((n + 63) / 64, (n + 63) / 64, c.T), dim3(256), 0, s, c.P1,
+                     R.d_pinv.ptr, st.d_ainv.ptr, n, st.d_adig.ptr);
   check_launch("p1_digits_kernel");
-  // point-chunk width: the largest that keeps the accumulators and staged digits in registers
-  // (96 points for 5 digits, 112 for 4); wider batches take several passes
-  if (P.max_np <= 64)
+  // point-chunk width 32 NPG: the widest whose accumulators, prefetched digits and staged B fit
+  // 256 registers per lane (two waves per SIMD) without spilling: 128 points for 5 digits, 160 for 4
+  if constexpr (S == 5) {
     launch<S, 4>(P, c, s);
-  else if (S == 5 || P.max_np <= 96)
-    launch<S, 6>(P, c, s);
-  else
-    launch<S, 7>(P, c, s);
+  } else {
+    if (P.max_np <= 128)
+      launch<S, 4>(P, c, s);
+    else
+      launch<S, 5>(P, c, s);
+  }
 }
 
 } // namespace

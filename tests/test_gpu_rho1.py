@@ -135,3 +135,18 @@ def test_rho1_int8_split_rejects_gradient_and_reports_bytes(cuda):
     from fast_rpa_eval_amd._lib import FrpaError
     with pytest.raises(FrpaError, match="rho1 only"):
         plan.forward(wave, P1, precision=api.OZAKI, gradient_wave=gw, grad_bias=torch.zeros_like(gw))
+
+
+def test_rho1_int8_split_nonsymmetric_multi_tile(cuda, oracle):
+    """The INT8 split multiplies the symmetric part of P1 over the upper tile pairs: a nonsymmetric
+    P1, T = 3 with a short last tile, nc up to 3 row tiles and np over one 128-point pass."""
+    prob = synth.make_rho1_problem(20, 13.0, 70, (1, 230), (1, 180), config_k=7, zero_nc=(5,), zero_np=(9,))
+    assert (prob.n_compute_c_batches > 128).any() and (prob.n_point_batches > 128).any()
+    rng = np.random.default_rng(17)
+    T = 3
+    P1 = np.asfortranarray(rng.uniform(-1, 1, size=(prob.n_basis, prob.n_basis, T)))
+    bias, _ = _bias(prob, T, 12)
+    ref, _ = oracle.rho1(prob, P1, T=T, bias=bias, j_atom_begin=4, j_atom_end=5)
+    got, _ = _run_gpu(prob, P1, T, 2, bias=bias, precision=api.OZAKI)
+    assert np.all(got[:, 2] == 0.0)
+    _check(got, ref, rel=1e-10, what="rho1 int8 split, nonsymmetric P1, T = 3")

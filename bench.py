@@ -46,6 +46,7 @@ DEFAULT = "combined_1024atoms_60kbatches_6Mpts_l8"
 
 PEAK_FP64 = 78.6      # TFLOP/s, dense FP64 MFMA / vector
 PEAK_INT8 = 5000.0    # TOP/s, dense INT8 MFMA
+PEAK_HBM = 8000.0     # GB/s, HBM3E
 
 
 def log(msg):
@@ -112,6 +113,34 @@ def step_fns(D, prec, adj_prec, far_variant, have_adj):
     return dict(rho1=rho1, adjoint=adj, far=far), dict(rho=rho, H=H, dv=dv)
 
 
+def time_graph(fns, names, steps, warmup):
+    """The step captured once in a HIP graph (torch.cuda.CUDAGraph over the C-ABI launches) and
+    replayed: ms per replay over `steps` replays."""
+    import torch
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(max(warmup, 1)):
+            for n in names:
+                fns[n]()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for n in names:
+            fns[n]()
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(steps):
+        g.replay()
+    t1.record()
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / steps, g
+
+
 def time_it(fns, names, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -138,33 +167,88 @@ def time_it(fns, names, steps, warmup):
     return total, per
 
 
-def rho1_flops(prob):
+def _nc_np(prob):
     nc = prob.n_compute_c_batches.astype(np.float64)
     npt = prob.n_point_batches.astype(np.float64)
     sel = (nc > 0) & (npt > 0)
-    return float(np.sum((2 * nc * nc * npt + 2 * nc * npt)[sel]))
+    return nc[sel], npt[sel]
 
 
-def adjoint_flops(prob):
-    """FLOPs of the adjoint as executed by the default kernel: 64 x 64 tiles (I <= J) of each batch's
-    nc x nc local H, 2 * 64 * 64 * np per tile, counting only the rows/columns inside nc."""
-    tot = 0.0
-    for nc, npt in zip(prob.n_compute_c_batches, prob.n_point_batches):
-        if nc <= 0 or npt <= 0:
+def work_model(prob, kernel, prec, S=None):
+    """Algorithmic work of one launch: (flops or int8 ops, HBM bytes), summed over the batches.
+
+    rho1 FP64 (full GEMM as in the reference): 2 nc^2 np + 2 nc np flop;
+        bytes 8 nc^2 (P1 gather) + 8 nc np (phi) + 4 nc + 12 np (basis list, map, output).
+    rho1 INT8 split (upper tile pairs of the symmetric part): S(S+1)/2 * 2 * nc(nc+1)/2 * np int8 ops;
+        bytes 8 nc(nc+1)/2 (packed P' digits) + S nc np (phi digits) + 8 nc np (phi, FP64 ddot)
+        + 4 nc + 12 np.
+    adjoint FP64 (upper triangle of the symmetric local H): 2 nc(nc+1)/2 np + nc np flop;
+        bytes 8 nc np (phi) + 12 np + 16 nc(nc+1)/2 (read-modify-write of the scatter into H).
+    adjoint INT8 split: S(S+1)/2 * 2 * nc(nc+1)/2 * np int8 ops; bytes as FP64 plus S nc np (phi digits).
+    """
+    nc, npt = _nc_np(prob)
+    tri = nc * (nc + 1) / 2
+    if kernel == "rho1" and S is None:
+        return float(np.sum(2 * nc * nc * npt + 2 * nc * npt)), float(np.sum(8 * nc * nc + 8 * nc * npt + 4 * nc + 12 * npt))
+    if kernel == "rho1":
+        return (float(np.sum(S * (S + 1) / 2 * 2 * tri * npt)),
+                float(np.sum(8 * tri + S * nc * npt + 8 * nc * npt + 4 * nc + 12 * npt)))
+    if kernel == "adjoint" and S is None:
+        return float(np.sum(2 * tri * npt + nc * npt)), float(np.sum(8 * nc * npt + 12 * npt + 16 * tri))
+    if kernel == "adjoint":
+        return (float(np.sum(S * (S + 1) / 2 * 2 * tri * npt)),
+                float(np.sum(8 * nc * npt + S * nc * npt + 12 * npt + 16 * tri)))
+    raise ValueError(kernel)
+
+
+PROFILED_KERNEL = {  # bench kernel (path) -> kernel name prefix in the rocprofv3 summaries
+    ("rho1", "fast"): "frpa::rho1_ozaki_kernel", ("rho1", "fp64"): "frpa::rho1_fp64_kernel",
+    ("adjoint", "fast"): "frpa::h1_ozaki_kernel", ("adjoint", "fp64"): "frpa::h1_push_kernel",
+    ("far", "fast"): "frpa::far_forward_kernel<1,", ("far", "fp64"): "frpa::far_forward_kernel<0,",
+}
+
+
+def read_counter_table(path):
+    """{(kernel, counter): mean per dispatch} from a summarize_profile counter table."""
+    out = {}
+    for ln in open(path):
+        parts = ln.split()
+        if len(parts) >= 5 and parts[-4] in ("FETCH_SIZE", "WRITE_SIZE"):
+            out[(" ".join(parts[:-4]), parts[-4])] = float(parts[-1])
+    return out
+
+
+def traffic_table(fetch_txt, write_txt, cf_txt, cw_txt, copy_bytes=2 ** 30):
+    """Markdown table of HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of each
+    frpa kernel, each scaled by copy_bytes / (the counter's reading for a plain copy of copy_bytes)."""
+    f, w = read_counter_table(fetch_txt), read_counter_table(write_txt)
+    cf, cw = read_counter_table(cf_txt), read_counter_table(cw_txt)
+    sf = copy_bytes / (cf[("__amd_rocclr_copyBuffer", "FETCH_SIZE")] * 1024)
+    sw = copy_bytes / (cw[("__amd_rocclr_copyBuffer", "WRITE_SIZE")] * 1024)
+    rows = [f"copy calibration: {copy_bytes} B copy reads FETCH_SIZE "
+            f"{cf[('__amd_rocclr_copyBuffer', 'FETCH_SIZE')]:.1f} KiB -> fetch scale {sf:.4f}; "
+            f"WRITE_SIZE {cw[('__amd_rocclr_copyBuffer', 'WRITE_SIZE')]:.1f} KiB -> write scale {sw:.4f}", "",
+            "| kernel | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | HBM bytes/launch (scaled) |",
+            "|---|---|---|---|"]
+    for (k, c), v in sorted(f.items()):
+        if not k.startswith("frpa::") or c != "FETCH_SIZE":
             continue
-        nt = (int(nc) + 63) // 64
-        sizes = [min(64, int(nc) - 64 * i) for i in range(nt)]
-        for I in range(nt):
-            for J in range(I, nt):
-                tot += 2.0 * sizes[I] * sizes[J] * npt
-    return tot
+        wv = w.get((k, "WRITE_SIZE"), 0.0)
+        rows.append(f"| {k} | {v:.1f} | {wv:.1f} | {v * 1024 * sf + wv * 1024 * sw:.4e} |")
+    return "\n".join(rows)
 
 
-def rho1_bytes(prob):
-    nc = prob.n_compute_c_batches.astype(np.float64)
-    npt = prob.n_point_batches.astype(np.float64)
-    sel = (nc > 0) & (npt > 0)
-    return float(np.sum((8 * nc * npt + 8 * nc * nc + 4 * nc + 12 * npt)[sel]))
+def load_traffic(workload):
+    """HBM bytes per launch by kernel name from profiles/hbm_traffic_<workload>.md (None if absent)."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for ln in open(path):
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0].startswith("frpa::"):
+            out[cells[0]] = float(cells[3])
+    return out
 
 
 def cpu_baseline(D, threads, rho_s=12.0, far_s=6.0):
@@ -248,6 +332,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--paths", default="fast,fp64")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default rho1 launch configuration")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: the timed step is one replay of the step captured in a HIP graph (per-kernel "
+                         "times from an eager pass); 0: eager launches")
     ap.add_argument("--kernels", default="rho1,adjoint,far",
                     help="subset of the step to time (a partial step is not a bench line)")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -256,7 +344,13 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device-to-device copy of GIB GiB five times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run) and exit")
+    ap.add_argument("--traffic-table", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="only print the HBM traffic table from four summaries made by --summarize-profile "
+                         "(kernel FETCH_SIZE, kernel WRITE_SIZE, 1 GiB copy FETCH_SIZE, copy WRITE_SIZE)")
     a = ap.parse_args()
+    if a.traffic_table:
+        print(traffic_table(*a.traffic_table))
+        return
     if a.summarize_profile:
         summarize_profile(a.summarize_profile)
         return
@@ -309,15 +403,25 @@ def main():
     names = [n for n in names if n in a.kernels.split(",")]
     res = {}
     outs = {}
+    tuned = {}
     for path in a.paths.split(","):
         prec = api.OZAKI if (path == "fast" and have_ozaki) else api.FP64
         adj_prec = api.OZAKI if (path == "fast" and have_ozaki_adj) else api.FP64
         var = api.FAR_HORNER if path == "fast" else api.FAR_TERMS
+        if not a.no_autotune and "rho1" in names:
+            t0 = time.time()
+            tuned[path] = D["plan"].autotune(D["wave"], D["P1"], precision=prec, reps=2)
+            log(f"autotune {path} ({time.time() - t0:.1f}s): {tuned[path]}")
         fns, o = step_fns(D, prec, adj_prec, var, have_adj)
-        total, per = time_it(fns, names, a.steps, a.warmup)
-        res[path] = dict(ms=total, per=per, prec=prec, adj_prec=adj_prec, var=var)
+        eager, per = time_it(fns, names, a.steps, a.warmup)
+        total, graph_ms = eager, None
+        if a.graph:
+            graph_ms, _ = time_graph(fns, names, a.steps, a.warmup)
+            total = graph_ms
+        res[path] = dict(ms=total, eager_ms=eager, graph_ms=graph_ms, per=per, prec=prec, adj_prec=adj_prec, var=var)
         outs[path] = {k: v.clone() for k, v in o.items()}
-        log(f"path {path}: {total:.3f} ms/step  " + "  ".join(f"{n} {per[n]:.3f} ms" for n in names))
+        log(f"path {path}: {total:.3f} ms/step (eager {eager:.3f}, graph {graph_ms})  " +
+            "  ".join(f"{n} {per[n]:.3f} ms" for n in names))
 
     pts_rho = int(prob.n_point_batches[prob.n_compute_c_batches > 0].sum())
     pts = pts_rho + fp.n_valid_points
@@ -338,6 +442,9 @@ def main():
         line["fp64_ms_per_step"] = res["fp64"]["ms"]
         line["speedup_vs_fp64"] = res["fp64"]["ms"] / fast["ms"]
     line["ms_per_kernel"] = {p: r["per"] for p, r in res.items()}
+    line["step_timing"] = {p: {"graph_ms": r["graph_ms"], "eager_ms": r["eager_ms"]} for p, r in res.items()}
+    if tuned:
+        line["autotune"] = tuned
     if len(names) < (3 if have_adj else 2):
         line["partial_step"] = names  # timing experiment: value does not cover the whole step
     if "fast" in outs and "fp64" in outs:
@@ -349,35 +456,45 @@ def main():
             h_f, h_d = outs["fast"]["H"], outs["fp64"]["H"]
             line["h1_rel_err_vs_fp64"] = float((h_f - h_d).abs().max() / h_d.abs().max())
 
-    # roofline of every kernel of the fast path; the dominant one is reported as "roofline"
-    pairs = far_pairs(D)
-    fl_rho = rho1_flops(prob)
-    fl_adj = adjoint_flops(prob)
+    # roofline of every kernel of the fast path: algorithmic work / measured time against the peak of
+    # whichever resource binds the kernel at its algorithmic work; the dominant kernel is "roofline"
+    traffic = load_traffic(a.workload) or {}
     rl = {}
     for name, tms in fast["per"].items():
         t = tms / 1e3
-        if name in ("rho1", "adjoint"):
-            prec = fast["prec"] if name == "rho1" else fast["adj_prec"]
-            fl = fl_rho if name == "rho1" else fl_adj
-            if prec in (api.OZAKI, api.OZAKI4):
-                npr = 15 if prec == api.OZAKI else 10
-                rl[name] = {"kernel": name, "bound": "mfma", "achieved": npr * fl / t / 1e12, "peak": PEAK_INT8,
-                            "unit": "TOP/s", "frac": npr * fl / t / 1e12 / PEAK_INT8, "traffic": None,
-                            "fp64_equivalent_TFLOPs": fl / t / 1e12}
-            else:
-                rl[name] = {"kernel": name, "bound": "mfma", "achieved": fl / t / 1e12, "peak": PEAK_FP64,
-                            "unit": "TFLOP/s", "frac": fl / t / 1e12 / PEAK_FP64, "traffic": None}
-        else:
+        path = "fast" if "fast" in res else "fp64"
+        prof = PROFILED_KERNEL[(name, path)]
+        tr = next((v for k, v in traffic.items() if k.startswith(prof)), None)
+        if name == "far":
             from fast_rpa_eval_amd.multipole_tables import horner_fma_count
+            pairs = far_pairs(D)
             per_pair = (2.0 * horner_fma_count(w["l_far"]) + 25) if fast["var"] == api.FAR_HORNER else \
                 (2.0 * fp.tables["n_cc_lm_ijk"][w["l_far"]] + 25)
             ach = pairs * per_pair / t / 1e12
-            rl[name] = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                        "frac": ach / PEAK_FP64, "traffic": None, "pairs": pairs, "flop_per_pair": per_pair}
+            rl[name] = {"kernel": prof, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                        "frac": ach / PEAK_FP64, "traffic": tr, "pairs": pairs, "flop_per_pair": per_pair,
+                        "note": "FP64 VALU (v_fma_f64); the FP64 vector and matrix peaks are equal on MI355X"}
+            continue
+        prec = fast["prec"] if name == "rho1" else fast["adj_prec"]
+        S = {api.OZAKI: 5, api.OZAKI4: 4}.get(prec)
+        ops, nbytes = work_model(prob, name, prec, S)
+        peak_ops = PEAK_INT8 if S else PEAK_FP64
+        t_ops = ops / (peak_ops * 1e12)
+        t_mem = nbytes / (PEAK_HBM * 1e9)
+        if t_mem > t_ops:
+            ach = nbytes / t / 1e9
+            rl[name] = {"kernel": prof, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM, "unit": "GB/s",
+                        "frac": ach / PEAK_HBM, "traffic": tr, "algorithmic_bytes": nbytes}
+        else:
+            ach = ops / t / 1e12
+            rl[name] = {"kernel": prof, "bound": "mfma", "achieved": ach, "peak": peak_ops,
+                        "unit": "TOP/s" if S else "TFLOP/s", "frac": ach / peak_ops, "traffic": tr,
+                        "algorithmic_bytes": nbytes}
+        rl[name]["algorithmic_ops"] = ops
+        rl[name]["other_bound_frac"] = (ops / t / 1e12 / peak_ops) if t_mem > t_ops else (nbytes / t / 1e9 / PEAK_HBM)
     dom = max(fast["per"], key=fast["per"].get)
     line["roofline"] = rl[dom]
     line["roofline_all"] = rl
-    line["rho1_algorithmic_bytes_GB"] = rho1_bytes(prob) / 1e9
 
     if not a.no_cpu:
         cb = cpu_baseline(D, a.cpu_threads)

@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfrpa.so")
+# FRPA_LIBRARY may name another build of the library (for example one made with different flags)
+LIB_PATH = os.environ.get("FRPA_LIBRARY") or os.path.join(HERE, "libfrpa.so")
 
 FRPA_OK = 0
 FRPA_PREC_FP64 = 0
@@ -37,6 +38,8 @@ def lib():
         L.frpa_rho1_forward.argtypes = [p, i, i, i, p, p, p, p, p, p, p, p]
         L.frpa_rho1_adjoint.argtypes = [p, i, i, p, p, p, p]
         L.frpa_rho1_prepare_wave.argtypes = [p, i, p, ll, p]
+        L.frpa_rho1_autotune.argtypes = [p, i, i, p, p, p, i, ctypes.c_char_p, i, p]
+        L.frpa_rho1_set_launch.argtypes = [p, i, i, i, i]
         L.frpa_synth_wave.argtypes = [p, i, p, p, p, p, p, p, p, p]
         L.frpa_far_plan_create.argtypes = [ctypes.POINTER(p), i, i, i, i, i, i, i, i] + [p] * 9
         L.frpa_far_plan_destroy.argtypes = [p]

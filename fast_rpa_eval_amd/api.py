@@ -109,6 +109,21 @@ class Rho1Plan:
               "frpa_rho1_adjoint")
         return H
 
+    def autotune(self, wave, density_matrix, precision=FP64, T=1, reps=3, out=None, stream=None):
+        """Time the launch configurations of forward() for this precision on these inputs and keep the
+        fastest in the plan; returns the per-candidate timings, the chosen one first."""
+        if out is None:
+            out = torch.empty(self.n_full_points * T, dtype=torch.float64, device=wave.device)
+        buf = ctypes.create_string_buffer(2048)
+        check(lib().frpa_rho1_autotune(self.handle, T, precision, _dp(wave), _dp(density_matrix), _dp(out), reps,
+                                       buf, len(buf), _stream(stream)), "frpa_rho1_autotune")
+        return buf.value.decode()
+
+    def set_launch(self, precision, order=0, persistent=0, points_per_wave=0):
+        """Choose the launch configuration of forward() for one precision (see frpa_rho1_set_launch)."""
+        check(lib().frpa_rho1_set_launch(self.handle, precision, order, persistent, points_per_wave),
+              "frpa_rho1_set_launch")
+
     def prepare_wave(self, wave, precision=OZAKI, stream=None):
         b = ctypes.c_longlong(0)
         check(lib().frpa_rho1_prepare_wave(self.handle, precision, _dp(wave), ctypes.byref(b), _stream(stream)),

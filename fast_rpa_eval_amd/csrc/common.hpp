@@ -37,6 +37,14 @@ struct Error {
     }                                                                                                          \
   } while (0)
 
+// Propagates the status of a nested C-ABI call (its message is already set).
+#define FRPA_CHECK_OK(call)                                                                                    \
+  do {                                                                                                         \
+    const int rc_ = (call);                                                                                    \
+    if (rc_ != FRPA_OK)                                                                                        \
+      throw ::frpa::Error{rc_};                                                                                \
+  } while (0)
+
 // Wraps a C-ABI body: translates exceptions into status codes.
 template <class F>
 int guarded(F &&f) {

@@ -30,6 +30,32 @@ __global__ void fill_f64_kernel(double *p, size_t n, double v) {
     p[i] = v;
 }
 
+__global__ void fill_i32_kernel(int *p, size_t n, int v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+void fill_i32(int *p, size_t n, int v, hipStream_t s) {
+  if (n == 0)
+    return;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  check_launch("fill_i32_kernel");
+}
+
+LaunchArgs launch_args(Rho1Plan &P, const LaunchCfg &cfg, int T, int blocks_per_cu, hipStream_t s) {
+  LaunchArgs a{nullptr, nullptr, dim3(P.n_valid, T)};
+  if (cfg.order == 1)
+    a.order = P.d_order_spatial.ptr;
+  const int grid = cfg.persistent >= N_XCD ? cfg.persistent : std::max(1, P.n_cu) * blocks_per_cu;
+  if (cfg.persistent && grid >= N_XCD && (long long)grid < (long long)P.n_valid * T) {
+    fill_i32(P.d_counters.ptr, N_XCD, 0, s);
+    a.counters = P.d_counters.ptr;
+    a.grid = dim3(grid / N_XCD * N_XCD, 1);
+  }
+  return a;
+}
+
 void fill_f64(double *p, size_t n, double v, hipStream_t s) {
   if (n == 0)
     return;
@@ -43,6 +69,8 @@ Rho1Plan::~Rho1Plan() {
     ozaki_release(ozaki);
   if (adj)
     adj_release(adj);
+  if (adjoz)
+    adjoz_release(adjoz);
   delete relabel;
 }
 
@@ -153,6 +181,27 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
   P->n_valid = (int)desc.size();
   P->h_desc = desc;
   P->d_desc.upload(desc);
+  {
+    // XCD-aware spatial order: caller order cut into N_XCD contiguous ranges, position 8 k + x
+    // takes the k-th batch of range x (range lengths differ by at most one)
+    std::vector<int> caller(P->n_valid);
+    std::iota(caller.begin(), caller.end(), 0);
+    std::stable_sort(caller.begin(), caller.end(), [&](int a, int b) { return desc[a].b < desc[b].b; });
+    std::vector<int> order(std::max(P->n_valid, 1), 0);
+    const int n = P->n_valid;
+    int start = 0;
+    for (int x = 0; x < N_XCD; x++) {
+      const int len = n / N_XCD + (x < n % N_XCD ? 1 : 0);
+      for (int k = 0; k < len; k++)
+        order[N_XCD * k + x] = caller[start + k];
+      start += len;
+    }
+    P->d_order_spatial.upload(order);
+    P->d_counters.alloc(N_XCD);
+    int dev = 0;
+    FRPA_HIP(hipGetDevice(&dev));
+    FRPA_HIP(hipDeviceGetAttribute(&P->n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
   P->d_ibasis0.upload(P->h_ibasis0);
   P->d_map.upload(map, (size_t)n_valid_points);
   return P.release();
@@ -342,6 +391,95 @@ int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, c
     FRPA_REQUIRE(plan && g && first_order_H && atom_tile_size >= 1, FRPA_ERR_INVALID, "bad adjoint arguments");
     Rho1Plan &P = *reinterpret_cast<Rho1Plan *>(plan);
     rho1_adjoint(P, atom_tile_size, precision, wave, g, first_order_H, as_stream(stream));
+  });
+}
+
+int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
+    FRPA_REQUIRE(precision == FRPA_PREC_FP64 || precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4,
+                 FRPA_ERR_INVALID, "unknown precision %d", precision);
+    FRPA_REQUIRE(order == 0 || order == 1, FRPA_ERR_INVALID, "order must be 0 or 1");
+    FRPA_REQUIRE(persistent == 0 || persistent == 1 || persistent >= N_XCD, FRPA_ERR_INVALID,
+                 "persistent must be 0, 1 or a grid of at least %d workgroups", N_XCD);
+    const bool ok = points_per_wave == 0 ||
+                    (precision == FRPA_PREC_FP64 && (points_per_wave == 4 || points_per_wave == 8 ||
+                                                     points_per_wave == 13)) ||
+                    (precision == FRPA_PREC_OZAKI && points_per_wave == 4) ||
+                    (precision == FRPA_PREC_OZAKI4 && (points_per_wave == 4 || points_per_wave == 5));
+    FRPA_REQUIRE(ok, FRPA_ERR_INVALID, "%d point groups per wave not available for precision %d", points_per_wave,
+                 precision);
+    reinterpret_cast<Rho1Plan *>(plan)->cfg[precision] = LaunchCfg{order, persistent, points_per_wave};
+  });
+}
+
+int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, const double *wave,
+                       const double *density_matrix, double *first_order_rho, int reps, char *choice,
+                       int choice_len, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && density_matrix && first_order_rho && atom_tile_size >= 1 && reps >= 1, FRPA_ERR_INVALID,
+                 "bad autotune arguments");
+    FRPA_REQUIRE(precision == FRPA_PREC_FP64 || precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4,
+                 FRPA_ERR_INVALID, "unknown precision %d", precision);
+    Rho1Plan &P = *reinterpret_cast<Rho1Plan *>(plan);
+    if (P.n_valid == 0)
+      return;
+    hipStream_t s = as_stream(stream);
+    std::vector<int> npgs;
+    if (precision == FRPA_PREC_FP64)
+      npgs = {4, 8, 13};
+    else if (precision == FRPA_PREC_OZAKI4)
+      npgs = {4, 5};
+    else
+      npgs = {0};
+    std::vector<LaunchCfg> cands;
+    for (int order = 0; order < 2; order++)
+      for (int pers = 0; pers < 2; pers++)
+        for (int npg : npgs)
+          cands.push_back(LaunchCfg{order, pers, npg});
+    hipEvent_t e0, e1;
+    FRPA_HIP(hipEventCreate(&e0));
+    FRPA_HIP(hipEventCreate(&e1));
+    std::vector<float> ms(cands.size());
+    try {
+      for (size_t i = 0; i < cands.size(); i++) {
+        P.cfg[precision] = cands[i];
+        auto run = [&] {
+          FRPA_CHECK_OK(frpa_rho1_forward(plan, atom_tile_size, atom_tile_size, precision, wave, density_matrix,
+                                          nullptr, first_order_rho, nullptr, nullptr, nullptr, stream));
+        };
+        run();
+        FRPA_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++)
+          run();
+        FRPA_HIP(hipEventRecord(e1, s));
+        FRPA_HIP(hipEventSynchronize(e1));
+        FRPA_HIP(hipEventElapsedTime(&ms[i], e0, e1));
+        ms[i] /= reps;
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const size_t best = std::min_element(ms.begin(), ms.end()) - ms.begin();
+    P.cfg[precision] = cands[best];
+    if (choice && choice_len > 0) {
+      std::string out;
+      auto desc = [&](size_t i) {
+        char buf[96];
+        snprintf(buf, sizeof(buf), "order=%d persistent=%d npg=%d ms=%.3f", cands[i].order, cands[i].persistent,
+                 cands[i].npg, ms[i]);
+        return std::string(buf);
+      };
+      out = desc(best);
+      for (size_t i = 0; i < cands.size(); i++)
+        if (i != best)
+          out += "; " + desc(i);
+      snprintf(choice, (size_t)choice_len, "%s", out.c_str());
+    }
   });
 }
 

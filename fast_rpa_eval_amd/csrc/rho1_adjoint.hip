@@ -339,22 +339,25 @@ void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const d
   FRPA_REQUIRE(wave || P.n_valid == 0, FRPA_ERR_INVALID, "null wave");
   if (P.n_valid == 0)
     return;
-  FRPA_REQUIRE(precision != FRPA_PREC_OZAKI, FRPA_ERR_UNSUPPORTED,
-               "the INT8-split adjoint is not available in this build (use FRPA_PREC_FP64)");
-  FRPA_REQUIRE(precision == FRPA_PREC_FP64, FRPA_ERR_INVALID, "unknown precision %d", precision);
+  FRPA_REQUIRE(precision == FRPA_PREC_FP64 || precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4,
+               FRPA_ERR_INVALID, "unknown precision %d", precision);
   if (!P.adj)
     P.adj = new AdjState();
   AdjState &st = *P.adj;
   const Relabel &R = get_relabel(P);
   const int n = P.n_basis;
   const char *mode = getenv("FRPA_ADJOINT");
-  if (!(mode && strcmp(mode, "tiles") == 0)) {
+  if (precision != FRPA_PREC_FP64 || !(mode && strcmp(mode, "tiles") == 0)) {
     const size_t nn = (size_t)n * n * T;
     st.d_hs.alloc(nn);
     fill_f64(st.d_hs.ptr, nn, 0.0, s);
-    hipLaunchKernelGGL(h1_push_kernel, dim3(P.n_valid, T), dim3(256), 0, s, P.d_desc.ptr, R.d_sib.ptr, R.d_srow.ptr,
-                       P.d_map.ptr, wave, g, P.n_full, n, st.d_hs.ptr);
-    check_launch("h1_push_kernel");
+    if (precision == FRPA_PREC_FP64) {
+      hipLaunchKernelGGL(h1_push_kernel, dim3(P.n_valid, T), dim3(256), 0, s, P.d_desc.ptr, R.d_sib.ptr,
+                         R.d_srow.ptr, P.d_map.ptr, wave, g, P.n_full, n, st.d_hs.ptr);
+      check_launch("h1_push_kernel");
+    } else {
+      h1_ozaki_push(P, T, precision == FRPA_PREC_OZAKI ? 5 : 4, wave, g, st.d_hs.ptr, s);
+    }
     hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, T), dim3(256), 0, s, st.d_hs.ptr,
                        R.d_pinv.ptr, n, H);
     check_launch("h1_unrelabel_kernel");

@@ -26,8 +26,24 @@ struct Relabel {
   DevBuf<int> d_pi, d_pinv, d_srow, d_sib;
 };
 
+// How a rho1 forward kernel walks the batches (chosen by frpa_rho1_autotune, or the default):
+//   order 0: descending work (the BatchDesc order); order 1: XCD-aware spatial order -- the batches in
+//   the caller's order cut into 8 contiguous ranges, launch position 8 k + x taking the k-th batch of
+//   range x, so that workgroup x (mod 8), which the dispatcher places on XCD x, works through one
+//   spatial region and its L2 sees neighbouring batches;
+//   persistent: a grid of a few workgroups per CU that claim positions 8 k + x from per-XCD
+//   counters instead of one workgroup per batch;
+//   npg: point groups per wave and pass (0 = the default for the precision).
+struct LaunchCfg {
+  int order = 0;
+  int persistent = 0;
+  int npg = 0;
+};
+constexpr int N_XCD = 8;
+
 struct OzakiState;  // INT8 split data (rho1_ozaki.hip)
 struct AdjState;    // basis-tile work lists of the adjoint (rho1_adjoint.hip)
+struct AdjOzState;  // digits of the INT8-split adjoint (rho1_adjoint_ozaki.hip)
 
 struct Rho1Plan {
   int n_batches = 0, n_basis = 0, n_full = 0, nmb = 0;
@@ -40,7 +56,12 @@ struct Rho1Plan {
   DevBuf<int> d_map;
   OzakiState *ozaki = nullptr;
   AdjState *adj = nullptr;
+  AdjOzState *adjoz = nullptr;
   Relabel *relabel = nullptr;       // built on first use (get_relabel)
+  DevBuf<int> d_order_spatial;      // launch position -> BatchDesc index, order 1
+  DevBuf<int> d_counters;           // N_XCD work counters of the persistent mode
+  int n_cu = 0;                     // compute units of the device the plan was made on
+  LaunchCfg cfg[3];                 // per precision (FP64, INT8 split S = 5, S = 4)
   ~Rho1Plan();
 };
 
@@ -53,12 +74,41 @@ struct Rho1Call {
 Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_max_batch_size,
                          const int *n_point_batches, const int *prefix, const int *map, const int *ncc,
                          const long long *ibasis_ptr, const int *ibasis, const long long *wave_offset);
-void rho1_forward_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s);
+void rho1_forward_fp64(Rho1Plan &P, const Rho1Call &c, hipStream_t s);
 void rho1_forward_ozaki(Rho1Plan &P, const Rho1Call &c, int slices, hipStream_t s);
 void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const double *g, double *H, hipStream_t s);
 const Relabel &get_relabel(Rho1Plan &P);
+// Kernel arguments of a launch configuration: the position -> BatchDesc map (nullptr = identity),
+// the zeroed work counters (nullptr = one workgroup per position), and the grid.
+struct LaunchArgs {
+  const int *order;
+  int *counters;
+  dim3 grid;
+};
+LaunchArgs launch_args(Rho1Plan &P, const LaunchCfg &cfg, int T, int blocks_per_cu, hipStream_t s);
+void fill_i32(int *p, size_t n, int v, hipStream_t s);
+
+// Persistent-mode work claiming: item k of XCD group x is launch position 8 (k % len) + x of
+// perturbation t = k / len, where len = number of positions congruent to x.
+__device__ __forceinline__ bool claim_item(int *counters, int n_pos, int T, int &pos, int &t, int *s_item) {
+  const int x = blockIdx.x % N_XCD;
+  const int len = n_pos / N_XCD + (x < n_pos % N_XCD ? 1 : 0);
+  __syncthreads();  // every thread has read the previous claim
+  if (threadIdx.x == 0)
+    *s_item = atomicAdd(&counters[x], 1);
+  __syncthreads();
+  const int k = *s_item;
+  if (len == 0 || k >= len * T)
+    return false;
+  pos = N_XCD * (k % len) + x;
+  t = k / len;
+  return true;
+}
 void ozaki_release(OzakiState *st);
 void adj_release(AdjState *st);
+void adjoz_release(AdjOzState *st);
+// INT8-split adjoint of one call: atomics into the zeroed relabelled upper-triangle scratch Hs
+void h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *g, double *Hs, hipStream_t s);
 long long ozaki_prepare_wave(Rho1Plan &P, const double *wave, int slices, hipStream_t s);
 void synth_wave(const Rho1Plan &P, int n_atoms, const double *atom_xyz, const int *basis_atom, const int *basis_l,
                 const int *basis_m, const double *basis_alpha, int n_basis, const double *coord_points, double *wave,

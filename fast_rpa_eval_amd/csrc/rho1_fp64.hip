@@ -202,7 +202,8 @@ __device__ __forceinline__ void rho1_fp64_dispatch(std::integer_sequence<int, Gs
 
 template <int MAXPG, bool GRAD>
 __global__ __launch_bounds__(256) void rho1_fp64_kernel(const BatchDesc *__restrict__ descs,
-                                                        const int *__restrict__ ibasis0,
+                                                        const int *__restrict__ order, int *__restrict__ counters,
+                                                        int n_pos, int T, const int *__restrict__ ibasis0,
                                                         const int *__restrict__ map, const double *__restrict__ wave,
                                                         const double *__restrict__ gwave,
                                                         const double *__restrict__ P1, int n_basis,
@@ -216,45 +217,61 @@ __global__ __launch_bounds__(256) void rho1_fp64_kernel(const BatchDesc *__restr
   __shared__ double Bs0[KC * PLD], Bs1[KC * PLD];
   __shared__ int jbs0[KC], jbs1[KC];
   __shared__ double red[NRED * 4 * PMAX];
+  __shared__ int s_item;
   const Fp64Smem sm{Bs0, Bs1, jbs0, jbs1, red};
-  const BatchDesc d = descs[blockIdx.x];
-  const int t = blockIdx.y;
-  const int ld = (d.nc + 3) & ~3;
-  const double *W = wave + d.wave_off;
-  const double *G = GRAD ? gwave + 3 * d.wave_off : nullptr;
-  const int *ib = ibasis0 + d.ib_off;
-  const double *Pt = P1 + (size_t)n_basis * n_basis * t;
-  (void)ld;
-  for (int pc = 0; pc < d.np; pc += PMAX) {
-    const int np = min(PMAX, d.np - pc);
-    const int npg = (np + 15) >> 4;
-    rho1_fp64_dispatch<MAXPG, GRAD>(std::make_integer_sequence<int, MAXPG>{}, npg, sm, d, t, pc, np, ib, map, W, G,
-                                    Pt, n_basis, bias, gbias, rho, grad, n_full, nmb, n_batches_all, n_tile);
+  int pos = blockIdx.x, t = blockIdx.y;
+  for (bool first = true;; first = false) {
+    if (counters) {
+      if (!claim_item(counters, n_pos, T, pos, t, &s_item))
+        break;
+    } else if (!first) {
+      break;
+    }
+    const BatchDesc d = descs[order ? order[pos] : pos];
+    const double *W = wave + d.wave_off;
+    const double *G = GRAD ? gwave + 3 * d.wave_off : nullptr;
+    const int *ib = ibasis0 + d.ib_off;
+    const double *Pt = P1 + (size_t)n_basis * n_basis * t;
+    for (int pc = 0; pc < d.np; pc += PMAX) {
+      const int np = min(PMAX, d.np - pc);
+      const int npg = (np + 15) >> 4;
+      rho1_fp64_dispatch<MAXPG, GRAD>(std::make_integer_sequence<int, MAXPG>{}, npg, sm, d, t, pc, np, ib, map, W,
+                                      G, Pt, n_basis, bias, gbias, rho, grad, n_full, nmb, n_batches_all, n_tile);
+    }
   }
 }
 
 template <int MAXPG, bool GRAD>
-void launch_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
-  dim3 grid(P.n_valid, c.T);
-  hipLaunchKernelGGL((rho1_fp64_kernel<MAXPG, GRAD>), grid, dim3(256), 0, s, P.d_desc.ptr, P.d_ibasis0.ptr,
-                     P.d_map.ptr, c.wave, c.gwave, c.P1, P.n_basis, c.bias, c.gbias, c.rho, c.grad, P.n_full,
-                     P.nmb, P.n_batches, c.n_tile);
+void launch_fp64(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s) {
+  static int per_cu = -1;
+  if (per_cu < 0) {
+    FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_fp64_kernel<MAXPG, GRAD>, 256, 0));
+    per_cu = std::max(per_cu, 1);
+  }
+  const LaunchArgs la = launch_args(P, cfg, c.T, per_cu, s);
+  hipLaunchKernelGGL((rho1_fp64_kernel<MAXPG, GRAD>), la.grid, dim3(256), 0, s, P.d_desc.ptr, la.order, la.counters,
+                     P.n_valid, c.T, P.d_ibasis0.ptr, P.d_map.ptr, c.wave, c.gwave, c.P1, P.n_basis, c.bias,
+                     c.gbias, c.rho, c.grad, P.n_full, P.nmb, P.n_batches, c.n_tile);
   check_launch("rho1_fp64_kernel");
 }
 
 template <bool GRAD>
-void dispatch_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
-  if (P.max_np <= 64)
-    launch_fp64<4, GRAD>(P, c, s);
-  else if (P.max_np <= 128)
-    launch_fp64<8, GRAD>(P, c, s);
+void dispatch_fp64(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
+  const LaunchCfg &cfg = P.cfg[FRPA_PREC_FP64];
+  // points per pass 16 MAXPG: by default the smallest of 64 / 128 / 208 that holds the largest
+  // batch (208: larger batches loop over chunks)
+  const int mp = cfg.npg ? cfg.npg : (P.max_np <= 64 ? 4 : P.max_np <= 128 ? 8 : 13);
+  if (mp == 4)
+    launch_fp64<4, GRAD>(P, c, cfg, s);
+  else if (mp == 8)
+    launch_fp64<8, GRAD>(P, c, cfg, s);
   else
-    launch_fp64<13, GRAD>(P, c, s);  // 208 points per pass; larger batches loop over chunks
+    launch_fp64<13, GRAD>(P, c, cfg, s);
 }
 
 } // namespace
 
-void rho1_forward_fp64(const Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
+void rho1_forward_fp64(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   if (P.n_valid == 0)
     return;
   if (c.grad)

@@ -393,31 +393,42 @@ __device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>
 
 template <int S, int NPG>
 __global__ __launch_bounds__(OZ_THREADS) void rho1_ozaki_kernel(
-    const BatchDesc *__restrict__ descs, const int *__restrict__ sib_all, const int *__restrict__ srow_all,
-    const int *__restrict__ map, const double *__restrict__ wave, const long long *__restrict__ adig,
-    const double *__restrict__ ainv, const signed char *__restrict__ bdig, const long long *__restrict__ boff,
-    const double *__restrict__ binv, int n_basis, const double *__restrict__ bias, double *__restrict__ rho,
-    int n_full, int nmb, int n_batches_all, int n_tile) {
+    const BatchDesc *__restrict__ descs, const int *__restrict__ order, int *__restrict__ counters, int n_pos, int T,
+    const int *__restrict__ sib_all, const int *__restrict__ srow_all, const int *__restrict__ map,
+    const double *__restrict__ wave, const long long *__restrict__ adig, const double *__restrict__ ainv,
+    const signed char *__restrict__ bdig, const long long *__restrict__ boff, const double *__restrict__ binv,
+    int n_basis, const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all,
+    int n_tile) {
   constexpr int PCH = 32 * NPG;
   __shared__ __attribute__((aligned(16))) signed char Bs[2 * S * PCH * BROW];
   __shared__ int jbs[128];
   __shared__ double red[4 * PCH];
-  const BatchDesc d = descs[blockIdx.x];
-  const int t = blockIdx.y;
-  const double *W = wave + d.wave_off;
-  const int *sib = sib_all + d.ib_off;
-  const int *srow = srow_all + d.ib_off;
-  const long long *At = adig + (size_t)n_basis * n_basis * t;
-  const double *ai = ainv + (size_t)n_basis * t;
-  const signed char *Bg = bdig + boff[blockIdx.x];
+  __shared__ int s_item;
   const int half = threadIdx.x >> 8;
-  for (int pc = 0; pc < d.np; pc += PCH) {
-    const int npc = min(PCH, d.np - pc);
-    const int ngt = (npc + 15) >> 4;       // groups of 16 points in this chunk
-    const int n0 = (ngt + 1) >> 1;         // the first half takes the extra group
-    const int ng = half ? ngt - n0 : n0, gp0 = half ? n0 : 0;
-    ozaki_dispatch<S, NPG>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc, gp0, sib,
-                           srow, map, W, At, ai, Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all, n_tile);
+  int pos = blockIdx.x, t = blockIdx.y;
+  for (bool first = true;; first = false) {
+    if (counters) {
+      if (!claim_item(counters, n_pos, T, pos, t, &s_item))
+        break;
+    } else if (!first) {
+      break;
+    }
+    const int di = order ? order[pos] : pos;
+    const BatchDesc d = descs[di];
+    const double *W = wave + d.wave_off;
+    const int *sib = sib_all + d.ib_off;
+    const int *srow = srow_all + d.ib_off;
+    const long long *At = adig + (size_t)n_basis * n_basis * t;
+    const double *ai = ainv + (size_t)n_basis * t;
+    const signed char *Bg = bdig + boff[di];
+    for (int pc = 0; pc < d.np; pc += PCH) {
+      const int npc = min(PCH, d.np - pc);
+      const int ngt = (npc + 15) >> 4;       // groups of 16 points in this chunk
+      const int n0 = (ngt + 1) >> 1;         // the first half takes the extra group
+      const int ng = half ? ngt - n0 : n0, gp0 = half ? n0 : 0;
+      ozaki_dispatch<S, NPG>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc, gp0, sib,
+                             srow, map, W, At, ai, Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all, n_tile);
+    }
   }
 }
 
@@ -445,12 +456,19 @@ void prepare(Rho1Plan &P, OzakiState &st, const double *wave, hipStream_t s) {
 }
 
 template <int S, int NPG>
-void launch(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
+void launch(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s) {
   OzakiState &st = *P.ozaki;
   const Relabel &R = get_relabel(P);
-  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG>), dim3(P.n_valid, c.T), dim3(OZ_THREADS), 0, s, P.d_desc.ptr,
-                     R.d_sib.ptr, R.d_srow.ptr, P.d_map.ptr, c.wave, st.d_adig.ptr, st.d_ainv.ptr, st.d_bdig.ptr,
-                     st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full, P.nmb, P.n_batches, c.n_tile);
+  static int per_cu = -1;
+  if (per_cu < 0) {
+    FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_ozaki_kernel<S, NPG>, OZ_THREADS, 0));
+    per_cu = std::max(per_cu, 1);
+  }
+  const LaunchArgs la = launch_args(P, cfg, c.T, per_cu, s);
+  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG>), la.grid, dim3(OZ_THREADS), 0, s, P.d_desc.ptr, la.order,
+                     la.counters, P.n_valid, c.T, R.d_sib.ptr, R.d_srow.ptr, P.d_map.ptr, c.wave, st.d_adig.ptr,
+                     st.d_ainv.ptr, st.d_bdig.ptr, st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full,
+                     P.nmb, P.n_batches, c.n_tile);
   check_launch("rho1_ozaki_kernel");
 }
 
@@ -477,15 +495,18 @@ void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   hipLaunchKernelGGL(p1_digits_kernel<S>, dim3((n + 63) / 64, (n + 63) / 64, c.T), dim3(256), 0, s, c.P1,
                      R.d_pinv.ptr, st.d_ainv.ptr, n, st.d_adig.ptr);
   check_launch("p1_digits_kernel");
-  // point-chunk width 32 NPG: the widest whose accumulators, prefetched digits and staged B fit
-  // 256 registers per lane (two waves per SIMD) without spilling: 128 points for 5 digits, 160 for 4
+  // point-chunk width 32 NPG: at most the widest whose accumulators, prefetched digits and staged B
+  // fit 256 registers per lane (two waves per SIMD) without spilling: 128 points for 5 digits, 160
+  // for 4 (the default when some batch has more than 128 points)
+  const LaunchCfg &cfg = P.cfg[S == 5 ? FRPA_PREC_OZAKI : FRPA_PREC_OZAKI4];
   if constexpr (S == 5) {
-    launch<S, 4>(P, c, s);
+    launch<S, 4>(P, c, cfg, s);
   } else {
-    if (P.max_np <= 128)
-      launch<S, 4>(P, c, s);
+    const int npg = cfg.npg ? cfg.npg : (P.max_np <= 128 ? 4 : 5);
+    if (npg == 4)
+      launch<S, 4>(P, c, cfg, s);
     else
-      launch<S, 5>(P, c, s);
+      launch<S, 5>(P, c, cfg, s);
   }
 }
 

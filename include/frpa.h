@@ -95,6 +95,27 @@ int frpa_rho1_forward(frpa_rho1_plan *plan, int atom_tile_size, int n_tile, int 
 int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, const double *wave, const double *g,
                       double *first_order_H, void *stream);
 
+/* Autotuner of the rho1 forward launch: times every launch configuration valid for the plan and
+ * precision (batch order: descending work or XCD-aware spatial; one workgroup per batch or a
+ * persistent grid with per-XCD work counters; points per pass) on the given inputs, `reps` calls
+ * each after one warm-up, with HIP events on `stream`; keeps the fastest in the plan for every later
+ * frpa_rho1_forward of that precision (the GGA gradient launch included) and writes a description
+ * ("order=1 persistent=0 npg=4 ms=..." per candidate, the chosen one first) to `choice`.  The
+ * configurations give bit-identical results; first_order_rho is overwritten as by
+ * frpa_rho1_forward (no bias).  No counterpart in the reference, whose batch tiling is fixed at
+ * compile time (N_BATCHES_TILE, fast_rap_src_dfpt_gpu_eval/evaluate_first_order_rho.cu:188-371). */
+int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, const double *wave,
+                       const double *density_matrix, double *first_order_rho, int reps, char *choice,
+                       int choice_len, void *stream);
+
+/* Sets the launch configuration of frpa_rho1_forward for one precision explicitly (what the
+ * autotuner chooses): order 0 = descending work, 1 = XCD-aware spatial; persistent 0 = one
+ * workgroup per batch, 1 = a persistent grid sized from the occupancy, k >= 8 = a persistent grid of
+ * k workgroups (rounded down to a multiple of 8); points_per_wave = groups of 16 points per wave and
+ * pass (FP64: 4, 8, 13; INT8 split S = 5: 4; S = 4: 4, 5; 0 = default).  Results do not depend on
+ * it.  No counterpart in the reference (compile-time tiling). */
+int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave);
+
 /* One-off conversion of wave_batches_compress into the INT8 digits used by the INT8-split
  * precisions (kept in the plan and reused while the same wave pointer is passed; call again after
  * changing wave in place).  Optional: the first INT8-split forward converts on demand.

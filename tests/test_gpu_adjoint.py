@@ -59,3 +59,31 @@ def test_h1_adjoint_identity_with_gpu_forward(cuda):
     lhs, rhs = float(np.dot(g, rho)), float(np.sum(H * P))
     print(f"[adjoint identity] {lhs:.15e} {rhs:.15e}", flush=True)
     assert abs(lhs - rhs) <= 1e-11 * max(1.0, abs(lhs))
+
+
+@pytest.mark.parametrize("precision,tol", [(api.OZAKI, 1e-10), (api.OZAKI4, 1e-8)])
+@pytest.mark.parametrize("k,args", [(1, (8, 10.0, 64, 100, None)), (3, (40, 16.0, 120, (1, 300), (1, 260)))])
+def test_h1_int8_split_against_oracle(cuda, oracle, k, args, precision, tol):
+    """INT8-split adjoint (per-row digit scales, point index contracted on INT8 MFMA): max|dH| / max|H|
+    within the stated bound of the FP64 oracle; T = 2, batches over one 256-point pass, nc = 0."""
+    prob = synth.make_rho1_problem(*args, config_k=k, zero_nc=(2,) if k == 3 else ())
+    T = 2
+    rng = np.random.default_rng(10 + k)
+    g = np.asfortranarray(rng.standard_normal((prob.n_full_points, T)))
+    ref = _oracle_h1(oracle, prob, g, T)
+    plan = api.Rho1Plan.from_problem(prob)
+    H0 = rng.standard_normal(prob.n_basis * prob.n_basis * T)
+    Hd = api.to_device(H0)
+    wave = api.to_device(prob.wave)
+    plan.adjoint(wave, api.to_device(g), T=T, H=Hd, precision=precision)
+    got = Hd.cpu().numpy().reshape((prob.n_basis, prob.n_basis, T), order="F") - H0.reshape(
+        (prob.n_basis, prob.n_basis, T), order="F")
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"[h1 int8 split config {k} precision {precision}] rel {err:.3e}", flush=True)
+    assert err <= tol
+    # a second call with another g reuses the phi digits
+    g2 = np.asfortranarray(rng.standard_normal((prob.n_full_points, T)))
+    H2 = plan.adjoint(wave, api.to_device(g2), T=T, precision=precision).cpu().numpy().reshape(
+        (prob.n_basis, prob.n_basis, T), order="F")
+    ref2 = _oracle_h1(oracle, prob, g2, T)
+    assert np.abs(H2 - ref2).max() <= tol * np.abs(ref2).max()

@@ -1,0 +1,59 @@
+"""The whole step (INT8-split rho1, H1 adjoint, far field) captured once in a HIP graph and replayed:
+every replay, with eager work and synchronisations between replays, must give the eager results
+(rho1 and the far field bit for bit; H1, summed with atomics in a run-dependent order, to 1e-13)."""
+import numpy as np
+import pytest
+import torch
+
+from fast_rpa_eval_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_step_graph_replays_match_eager(cuda):
+    prob = synth.make_rho1_problem(16, 12.0, 150, (30, 200), (20, 230), config_k=12)
+    fp = synth.make_far_problem(16, 12.0, 5000, 6, config_k=12, xyz=prob.atom_xyz)
+    plan = api.Rho1Plan.from_problem(prob)
+    fplan = api.FarPlan.from_problem(fp)
+    finp = api.FarInputs(fp)
+    wave = api.to_device(prob.wave)
+    P1 = api.to_device(synth.make_p1(prob.atom_xyz, prob.basis, 3012))
+    g = api.to_device(np.random.default_rng(1).standard_normal(prob.n_full_points))
+    rho = torch.empty(prob.n_full_points, dtype=torch.float64, device="cuda")
+    H = torch.empty(prob.n_basis * prob.n_basis, dtype=torch.float64, device="cuda")
+    dv = torch.empty(fp.n_full_points, dtype=torch.float64, device="cuda")
+
+    def step():
+        plan.forward(wave, P1, precision=api.OZAKI, out=rho)
+        H.zero_()
+        plan.adjoint(wave, g, H=H)
+        dv.zero_()
+        fplan.forward(dv, *finp.args(), variant=api.FAR_HORNER)
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()  # first calls build the per-plan device state outside the capture
+        step()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    ref = [rho.clone(), H.clone(), dv.clone()]
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for rep in range(4):
+        rho.fill_(np.nan)
+        H.fill_(np.nan)
+        dv.fill_(np.nan)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(rho, ref[0]), f"replay {rep}: rho1 differs"
+        assert torch.equal(dv, ref[2]), f"replay {rep}: far field differs"
+        hmax = float(ref[1].abs().max())
+        herr = float((H - ref[1]).abs().max())
+        print(f"[graph replay {rep}] H1 max|diff| / max {herr / hmax:.2e}", flush=True)
+        assert herr <= 1e-13 * hmax
+        step()  # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(rho, ref[0])

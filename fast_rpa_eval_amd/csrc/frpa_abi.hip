@@ -160,13 +160,11 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
     }
     desc.push_back(BatchDesc{wave_offset[b], 0, nc, np, prefix[b], b});
   }
-  // processing order: descending work (nc^2 np), or the caller's batch order (neighbouring batches
-  // of a spatial grid partition share basis functions, so concurrent blocks reuse P1 in the caches)
-  const char *order = getenv("FRPA_BATCH_ORDER");
-  if (!(order && strcmp(order, "caller") == 0))
-    std::stable_sort(desc.begin(), desc.end(), [](const BatchDesc &x, const BatchDesc &y) {
-      return (double)x.nc * x.nc * x.np > (double)y.nc * y.nc * y.np;
-    });
+  // BatchDesc order: descending work nc^2 np (longest first); the XCD-aware spatial order is a
+  // separate position map (d_order_spatial)
+  std::stable_sort(desc.begin(), desc.end(), [](const BatchDesc &x, const BatchDesc &y) {
+    return (double)x.nc * x.nc * x.np > (double)y.nc * y.nc * y.np;
+  });
   // compact 0-based basis lists, in processing order
   long long off = 0;
   for (auto &d : desc) {

@@ -18,6 +18,9 @@
 //    u = 1/r^2 and X = x u, Y = y u, Z = z u the pair sum is
 //        (1/r) sum_q u^q Q_q(X, Y, Z),   Q_q of degree l - 2q in X, Y, Z (X degree <= 1),
 //    evaluated by nested Horner: one FMA per merged coefficient (165 at l = 8 vs 425 terms).
+#include <cstdlib>
+#include <utility>
+
 #include "farfield_common.hpp"
 
 namespace frpa {
@@ -226,6 +229,179 @@ __global__ __launch_bounds__(256) void far_forward_kernel(
 }
 
 // ---------------------------------------------------------------------------------------------
+// Horner forward with several points per lane: one scalar load of a coefficient feeds PPL
+// independent FMA chains (the coefficient block of one centre, 165 doubles at l = 8, does not fit
+// the SGPRs, so the single-point kernel waits on scalar loads between FMAs).
+// ---------------------------------------------------------------------------------------------
+
+template <int N>
+struct DV {
+  double v[N];
+};
+template <int N>
+__device__ __forceinline__ DV<N> splat(double x) {
+  DV<N> r;
+#pragma unroll
+  for (int q = 0; q < N; q++)
+    r.v[q] = x;
+  return r;
+}
+template <int N>
+__device__ __forceinline__ DV<N> fmav(const DV<N> &a, const DV<N> &b, const DV<N> &c) {
+  DV<N> r;
+#pragma unroll
+  for (int q = 0; q < N; q++)
+    r.v[q] = fma(a.v[q], b.v[q], c.v[q]);
+  return r;
+}
+template <int N>
+__device__ __forceinline__ DV<N> fmas(const DV<N> &a, const DV<N> &b, double c) {
+  DV<N> r;
+#pragma unroll
+  for (int q = 0; q < N; q++)
+    r.v[q] = fma(a.v[q], b.v[q], c);
+  return r;
+}
+
+template <int N, int L, int Q, int I, int J>
+__device__ __forceinline__ DV<N> hzv(const double *__restrict__ c, const DV<N> &Z) {
+  constexpr int K = L - 2 * Q - I - J;
+  DV<N> s = splat<N>(c[horner_index(L, Q, I, J, K)]);
+#pragma unroll
+  for (int k = K - 1; k >= 0; k--)
+    s = fmas(s, Z, c[horner_index(L, Q, I, J, k)]);
+  return s;
+}
+template <int N, int L, int Q, int I, int J>
+__device__ __forceinline__ DV<N> hyv(const double *__restrict__ c, const DV<N> &Y, const DV<N> &Z) {
+  if constexpr (J == L - 2 * Q - I)
+    return hzv<N, L, Q, I, J>(c, Z);
+  else
+    return fmav(hyv<N, L, Q, I, J + 1>(c, Y, Z), Y, hzv<N, L, Q, I, J>(c, Z));
+}
+template <int N, int L, int Q>
+__device__ __forceinline__ DV<N> hqv(const double *__restrict__ c, const DV<N> &X, const DV<N> &Y, const DV<N> &Z) {
+  if constexpr (L - 2 * Q == 0)
+    return hyv<N, L, Q, 0, 0>(c, Y, Z);
+  else
+    return fmav(hyv<N, L, Q, 1, 0>(c, Y, Z), X, hyv<N, L, Q, 0, 0>(c, Y, Z));
+}
+template <int N, int L, int Q>
+__device__ __forceinline__ DV<N> huv(const double *__restrict__ c, const DV<N> &X, const DV<N> &Y, const DV<N> &Z,
+                                     const DV<N> &u) {
+  if constexpr (2 * (Q + 1) > L)
+    return hqv<N, L, Q>(c, X, Y, Z);
+  else
+    return fmav(huv<N, L, Q + 1>(c, X, Y, Z, u), u, hqv<N, L, Q>(c, X, Y, Z));
+}
+
+template <int N, int L>
+__device__ __forceinline__ DV<N> pair_horner_v(const double *__restrict__ c, const DV<N> &mx, const DV<N> &my,
+                                               const DV<N> &mz, const DV<N> &d2) {
+  DV<N> u, X, Y, Z, rinv;
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    u.v[q] = 1.0 / d2.v[q];
+    rinv.v[q] = sqrt(u.v[q]);
+    X.v[q] = mx.v[q] * u.v[q];
+    Y.v[q] = my.v[q] * u.v[q];
+    Z.v[q] = mz.v[q] * u.v[q];
+  }
+  DV<N> s = huv<N, L, 0>(c + horner_prefix(L), X, Y, Z, u);
+#pragma unroll
+  for (int q = 0; q < N; q++)
+    s.v[q] *= rinv.v[q];
+  return s;
+}
+
+template <int N, int... Ls>
+__device__ __forceinline__ DV<N> pair_switch_v(std::integer_sequence<int, Ls...>, int L, const double *__restrict__ c,
+                                               const DV<N> &mx, const DV<N> &my, const DV<N> &mz, const DV<N> &d2) {
+  DV<N> v = splat<N>(0.0);
+  ((L == Ls ? (void)(v = pair_horner_v<N, Ls>(c, mx, my, mz, d2)) : (void)0), ...);
+  return v;
+}
+
+// Point q of lane `tid` of block b is valid point b * 256 N + 256 q + tid.
+template <int LMAXK, int N>
+__global__ __launch_bounds__(256) void far_horner_kernel(
+    const CenterRec *__restrict__ centers, int n_hc, const double *__restrict__ cp, const int *__restrict__ map,
+    int n_valid, const double *__restrict__ opr, int lp1, const double *__restrict__ mrsq, int n_atoms,
+    const double *__restrict__ coefs, int cstride, double *__restrict__ dv, int n_full) {
+  const int t = blockIdx.y;
+  bool active[N];
+  int p[N];
+  double rx[N], ry[N], rz[N], V[N];
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    const int iv = blockIdx.x * 256 * N + 256 * q + threadIdx.x;
+    active[q] = iv < n_valid;
+    p[q] = active[q] ? map[iv] : 0;
+    rx[q] = active[q] ? cp[3 * (size_t)p[q]] : 0.0;
+    ry[q] = active[q] ? cp[3 * (size_t)p[q] + 1] : 0.0;
+    rz[q] = active[q] ? cp[3 * (size_t)p[q] + 2] : 0.0;
+    V[q] = 0.0;
+  }
+  for (int ic = 0; ic < n_hc; ic++) {
+    const CenterRec c = centers[ic];
+    const double *op = opr + (size_t)lp1 * (c.atom + (size_t)n_atoms * t);
+    const double rsq = mrsq[c.atom + (size_t)n_atoms * t];
+    DV<N> d2;
+    int l[N];
+    bool far[N];
+    unsigned long long pend[N], any = 0;
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+      const double ax = rx[q] - c.dx, ay = ry[q] - c.dy, az = rz[q] - c.dz;
+      d2.v[q] = ax * ax + ay * ay + az * az;
+      l[q] = c.l_const;
+      while (op[l[q]] < d2.v[q] && l[q] > 0)
+        l[q]--;
+      far[q] = active[q] && !(d2.v[q] < rsq) && d2.v[q] < 1e8;
+      pend[q] = __ballot(far[q]);
+      any |= pend[q];
+    }
+    if (!any)
+      continue;
+    DV<N> mx, my, mz;
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+      mx.v[q] = rx[q] - c.mx;
+      my.v[q] = ry[q] - c.my;
+      mz.v[q] = rz[q] - c.mz;
+    }
+    const double *cb = coefs + (size_t)cstride * (c.mc_atom + (size_t)n_atoms * t);
+    for (;;) {
+      int L = -1;
+#pragma unroll
+      for (int q = 0; q < N; q++)
+        if (L < 0 && pend[q])
+          L = __builtin_amdgcn_readlane(l[q], __ffsll((long long)pend[q]) - 1);
+      if (L < 0)
+        break;
+      bool mine[N], anymine = false;
+#pragma unroll
+      for (int q = 0; q < N; q++) {
+        mine[q] = far[q] && l[q] == L;
+        pend[q] &= ~__ballot(mine[q]);
+        anymine |= mine[q];
+      }
+      if (anymine) {
+        const DV<N> r = pair_switch_v<N>(std::make_integer_sequence<int, LMAXK + 1>{}, L, cb, mx, my, mz, d2);
+#pragma unroll
+        for (int q = 0; q < N; q++)
+          if (mine[q])
+            V[q] += r.v[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < N; q++)
+    if (active[q])
+      dv[(size_t)p[q] + (size_t)n_full * t] += V[q];
+}
+
+// ---------------------------------------------------------------------------------------------
 // Adjoint: d/dM of sum_p gV(p, t) delta_v(p, t)
 // ---------------------------------------------------------------------------------------------
 
@@ -325,6 +501,27 @@ __global__ void far_adjoint_reduce_kernel(int n_atoms, int n_hc, int nbx, int nl
   }
 }
 
+template <int N>
+void launch_horner(FarPlan &P, const FarCall &c, double *dv, hipStream_t s) {
+  dim3 grid((c.n_valid + 256 * N - 1) / (256 * N), c.n_tile);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, P.d_centers.ptr, P.n_hc, c.coord_points, c.map, c.n_valid, c.opr,
+                       P.l_pot_max + 1, c.mrsq, P.n_atoms, P.d_coef.ptr, P.coef_stride, dv, c.n_full);
+  };
+  switch (P.l_pot_max) {
+#define FRPA_CASE(LL)                                                                                          \
+  case LL:                                                                                                     \
+    go(far_horner_kernel<LL, N>);                                                                              \
+    break;
+    FRPA_CASE(0) FRPA_CASE(1) FRPA_CASE(2) FRPA_CASE(3) FRPA_CASE(4) FRPA_CASE(5) FRPA_CASE(6) FRPA_CASE(7)
+    FRPA_CASE(8) FRPA_CASE(9) FRPA_CASE(10)
+#undef FRPA_CASE
+  default:
+    FRPA_REQUIRE(false, FRPA_ERR_UNSUPPORTED, "l_pot_max %d > %d", P.l_pot_max, FAR_LMAX);
+  }
+  check_launch("far_horner_kernel");
+}
+
 template <int VARIANT>
 void launch_forward(FarPlan &P, const FarCall &c, double *dv, hipStream_t s) {
   dim3 grid((c.n_valid + 255) / 256, c.n_tile);
@@ -369,7 +566,16 @@ void far_forward(FarPlan &P, const FarCall &c, int variant, double *dv, hipStrea
                        P.coef_stride, P.dim0, P.l_pot_max, lama, P.d_index_cc.ptr, P.d_ncc.ptr, c.lmfd, c.moments,
                        P.d_hc_ptr.ptr, P.d_hc_term.ptr, P.d_hc_w.ptr, P.d_coef.ptr);
     check_launch("far_init_horner_kernel");
-    launch_forward<FRPA_FAR_HORNER>(P, c, dv, s);
+    // points per lane of the Horner kernel (FRPA_FAR_POINTS_PER_LANE = 1, 2 or 4; default 4: l = 8
+    // over 6M points x 1024 centres runs 73 / 46 / 45 ms)
+    const char *e = getenv("FRPA_FAR_POINTS_PER_LANE");
+    const int ppl = e ? atoi(e) : 4;
+    if (ppl == 1)
+      launch_forward<FRPA_FAR_HORNER>(P, c, dv, s);
+    else if (ppl == 2)
+      launch_horner<2>(P, c, dv, s);
+    else
+      launch_horner<4>(P, c, dv, s);
   } else {
     FRPA_REQUIRE(false, FRPA_ERR_INVALID, "unknown far-field variant %d", variant);
   }

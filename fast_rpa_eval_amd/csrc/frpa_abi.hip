@@ -100,18 +100,22 @@ const Relabel &get_relabel(Rho1Plan &P) {
     R->h_pinv[R->h_pi[mu]] = mu;
   R->h_srow.resize(P.h_ibasis0.size());
   R->h_sib.resize(P.h_ibasis0.size());
+  R->h_spos.resize(P.h_ibasis0.size());
   for (const BatchDesc &d : P.h_desc) {
     const int *ib = P.h_ibasis0.data() + d.ib_off;
     int *sr = R->h_srow.data() + d.ib_off;
     std::iota(sr, sr + d.nc, 0);
     std::sort(sr, sr + d.nc, [&](int a, int b) { return R->h_pi[ib[a]] < R->h_pi[ib[b]]; });
-    for (int k = 0; k < d.nc; k++)
+    for (int k = 0; k < d.nc; k++) {
       R->h_sib[d.ib_off + k] = R->h_pi[ib[sr[k]]];
+      R->h_spos[d.ib_off + sr[k]] = k;
+    }
   }
   R->d_pi.upload(R->h_pi);
   R->d_pinv.upload(R->h_pinv);
   R->d_srow.upload(R->h_srow.empty() ? std::vector<int>{0} : R->h_srow);
   R->d_sib.upload(R->h_sib.empty() ? std::vector<int>{0} : R->h_sib);
+  R->d_spos.upload(R->h_spos.empty() ? std::vector<int>{0} : R->h_spos);
   P.relabel = R.release();
   return *P.relabel;
 }

@@ -3,18 +3,20 @@
 //   H(i, k) += sum_p g(map(p)) phi_i(p) phi_k(p)   for the batch's compute rows i, k.
 // Split (radix-254 signed digits, round to nearest, as in the forward) with one scale per ROW, the
 // point index being the contraction index:
-//   Y(i, p) = g_p phi_i(p):  Y sy_i = sum_s y_s 254^-s,   sy_i = 127 / max_p |Y(i, p)|   (per call)
+//   Y(i, p) = g_p phi_i(p):  Y sy_i = sum_s y_s 254^-s,   sy_i = 127 / max_p |Y(i, p)|   (per call
+//                            and pass of at most 256 points, made while staging tile I)
 //   phi(k, p):               phi sf_k = sum_u f_u 254^-u,  sf_k = 127 / max_p |phi(k, p)| (per wave)
 //   H(i, k) = sum_l 254^-l C_l(i, k) / (sy_i sf_k),  C_l = sum_{s+u=l} sum_p y_s(i, p) f_u(k, p)
 // with the levels s + u <= S - 1 (exact int32: |C_l| <= 5 np 127^2).
 // Reference: evaluate_first_order_H.cpp:144-215 (contract = 0.5 w v1 phi, dsyr2k, scatter-add).
 //
-// Layout: per batch, S digit planes of (nc rounded to 64) rows in the batch's relabelled order
-// (get_relabel) times (np rounded to 64) points, point index contiguous, so that a lane's MFMA
-// fragment (16 consecutive points of one row) is one 16-byte load.  The kernel keeps the Y digits
-// of one 64-row tile I for all points in LDS and streams the phi digits of the tiles J >= I from
-// global memory (register prefetch, no barrier inside the J loop); the upper triangle of each tile
-// pair is added with FP64 atomics into the relabelled scratch H (see rho1_adjoint.hip).
+// Layout: per batch, S digit planes of phi with (nc rounded to 64) rows in the batch's relabelled
+// order (get_relabel) times (np rounded to 64) points, point index contiguous, so that a lane's MFMA
+// fragment (16 consecutive points of one row) is one 16-byte load.  The kernel computes the Y digits
+// of one 64-row tile I for all points of the pass straight from the FP64 phi and g into LDS, and
+// streams the phi digits of the tiles J >= I from global memory (two steps of register prefetch, no
+// barrier inside the J loop); the upper triangle of each tile pair is added with FP64 atomics into
+// the relabelled scratch H (see rho1_adjoint.hip).
 #include <algorithm>
 #include <vector>
 
@@ -28,12 +30,10 @@ struct AdjOzState {
   int S = 0;
   const double *wave = nullptr;  // the wave the phi digits were made from
   int npk_max = 0;               // max over batches of np rounded to 64
-  long long per_t = 0;           // bytes of one perturbation's Y digits
+  long long per_t = 0;           // bytes of the phi digits
   DevBuf<long long> d_off;       // per BatchDesc: byte offset of its digit planes
   DevBuf<signed char> d_fdig;    // phi digits (per wave)
   DevBuf<double> d_finv;         // 1 / sf per sorted row (compact, at ib_off)
-  DevBuf<signed char> d_ydig;    // Y digits (per call), (per_t bytes) x T
-  DevBuf<double> d_yinv;         // 1 / sy per sorted row and perturbation
 };
 
 void adjoz_release(AdjOzState *st) { delete st; }
@@ -46,16 +46,18 @@ constexpr int PMAX = 256;  // points held in LDS per pass (larger batches take s
 
 __device__ __forceinline__ int ceil64(int x) { return (x + 63) & ~63; }
 
-// Digit planes of one batch: rows in sorted order, 64-row tiles; WITH_G multiplies by g(map(p), t).
+// Digit planes of one batch: rows in sorted order; WITH_G multiplies by g(map(p), t).  Threads run
+// over the caller's rows (coalesced reads down each point's column, twice: row maximum, then
+// digits) and write each row's digits at its sorted position spos[r]; positions nc .. nc64-1 are
+// zero rows.
 template <int S, bool WITH_G>
 __global__ __launch_bounds__(256) void h1_digits_kernel(const BatchDesc *__restrict__ descs,
                                                         const long long *__restrict__ off,
-                                                        const int *__restrict__ srow_all,
+                                                        const int *__restrict__ spos_all,
                                                         const int *__restrict__ map, const double *__restrict__ wave,
                                                         const double *__restrict__ g, int n_full, long long per_t,
                                                         size_t inv_per_t, signed char *__restrict__ dig,
                                                         double *__restrict__ inv) {
-  __shared__ double red[4][64];
   __shared__ double gs[64];
   const BatchDesc d = descs[blockIdx.x];
   const int t = blockIdx.y;
@@ -63,57 +65,61 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const BatchDesc *__restr
   const int nc64 = ceil64(nc), npk = ceil64(np);
   const size_t plane = (size_t)nc64 * npk;
   const double *W = wave + d.wave_off;
-  const int *srow = srow_all + d.ib_off;
+  const int *spos = spos_all + d.ib_off;
   const double *gt = WITH_G ? g + (size_t)n_full * t : nullptr;
   signed char *out = dig + off[blockIdx.x] + (WITH_G ? per_t * t : 0);
   double *iv = inv + d.ib_off + (WITH_G ? inv_per_t * t : 0);
-  const int a = threadIdx.x & 63, q = threadIdx.x >> 6;
-  for (int i0 = 0; i0 < nc64; i0 += 64) {
-    const int i = i0 + a;
-    const double *Wr = i < nc ? W + srow[i] : nullptr;
+  auto load_g = [&](int p0) {
+    __syncthreads();
+    if (threadIdx.x < 64)
+      gs[threadIdx.x] = (WITH_G && p0 + (int)threadIdx.x < np) ? gt[map[d.pstart + p0 + threadIdx.x]] : 1.0;
+    __syncthreads();
+  };
+  for (int r0 = 0; r0 < nc64; r0 += 256) {
+    const int r = r0 + threadIdx.x;
+    const bool real = r < nc;
+    const double *Wr = W + (real ? r : 0);
     double mx = 0.0;
     for (int p0 = 0; p0 < np; p0 += 64) {
-      __syncthreads();
-      if (threadIdx.x < 64)
-        gs[threadIdx.x] = (WITH_G && p0 + (int)threadIdx.x < np) ? gt[map[d.pstart + p0 + threadIdx.x]] : 1.0;
-      __syncthreads();
-      if (Wr)
-        for (int j = q; j < 64 && p0 + j < np; j += 4)
+      load_g(p0);
+      if (real) {
+        const int pe = min(64, np - p0);
+        for (int j = 0; j < pe; j++)
           mx = fmax(mx, fabs(Wr[(size_t)ld * (p0 + j)] * gs[j]));
-    }
-    red[q][a] = mx;
-    __syncthreads();
-    mx = fmax(fmax(red[0][a], red[1][a]), fmax(red[2][a], red[3][a]));
-    const double sc = mx > 0.0 ? 127.0 / mx : 1.0;
-    if (q == 0 && i < nc)
-      iv[i] = 1.0 / sc;
-    for (int p0 = 0; p0 < npk; p0 += 64) {
-      __syncthreads();
-      if (threadIdx.x < 64)
-        gs[threadIdx.x] = (WITH_G && p0 + (int)threadIdx.x < np) ? gt[map[d.pstart + p0 + threadIdx.x]] : 1.0;
-      __syncthreads();
-      // this thread: row i, points p0 + 16 q .. + 15
-      unsigned int w4[S][4];
-#pragma unroll
-      for (int sl = 0; sl < S; sl++)
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-          w4[sl][u] = 0u;
-#pragma unroll
-      for (int j = 0; j < 16; j++) {
-        const int p = p0 + 16 * q + j;
-        double r = (Wr && p < np) ? Wr[(size_t)ld * p] * gs[16 * q + j] * sc : 0.0;
-#pragma unroll
-        for (int sl = 0; sl < S; sl++) {
-          const double dg = rint(r);
-          w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
-          r = (r - dg) * RADIX;
-        }
       }
+    }
+    const double sc = mx > 0.0 ? 127.0 / mx : 1.0;
+    const int row = real ? spos[r] : r;  // padding positions keep their own index
+    if (real)
+      iv[row] = 1.0 / sc;
+    for (int p0 = 0; p0 < npk; p0 += 64) {
+      load_g(p0);
+      if (r >= nc64)
+        continue;
+#pragma unroll 1
+      for (int q = 0; q < 4; q++) {
+        unsigned int w4[S][4];
 #pragma unroll
-      for (int sl = 0; sl < S; sl++)
-        *reinterpret_cast<v4i *>(out + sl * plane + (size_t)npk * i + p0 + 16 * q) =
-            v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
+        for (int sl = 0; sl < S; sl++)
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            w4[sl][u] = 0u;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const int p = p0 + 16 * q + j;
+          double x = (real && p < np) ? Wr[(size_t)ld * p] * gs[16 * q + j] * sc : 0.0;
+#pragma unroll
+          for (int sl = 0; sl < S; sl++) {
+            const double dg = rint(x);
+            w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
+            x = (x - dg) * RADIX;
+          }
+        }
+#pragma unroll
+        for (int sl = 0; sl < S; sl++)
+          *reinterpret_cast<v4i *>(out + sl * plane + (size_t)npk * row + p0 + 16 * q) =
+              v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
+      }
     }
   }
 }
@@ -125,23 +131,29 @@ template <int S>
 __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *__restrict__ descs,
                                                               const long long *__restrict__ off,
                                                               const int *__restrict__ sib_all,
+                                                              const int *__restrict__ srow_all,
+                                                              const int *__restrict__ map,
+                                                              const double *__restrict__ wave,
+                                                              const double *__restrict__ g, int n_full,
                                                               const signed char *__restrict__ fdig,
-                                                              const double *__restrict__ finv,
-                                                              const signed char *__restrict__ ydig,
-                                                              const double *__restrict__ yinv, long long per_t,
-                                                              size_t inv_per_t, int n_basis, double *__restrict__ Hs) {
+                                                              const double *__restrict__ finv, int n_basis,
+                                                              double *__restrict__ Hs) {
   constexpr int YROW = PMAX + 16;  // LDS bytes per (plane, row) of the staged Y tile
   __shared__ __attribute__((aligned(16))) signed char Ys[S * 64 * YROW];
+  __shared__ double gs[PMAX];      // g(map(p), t) of the pass's points
+  __shared__ double red[8][64];
+  __shared__ double ysc_s[64];     // 1 / sy of the tile's rows
   const int di = blockIdx.x, t = blockIdx.y;
   const BatchDesc d = descs[di];
-  const int nc = d.nc, np = d.np;
+  const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
   const int nc64 = ceil64(nc), npk = ceil64(np);
   const size_t plane = (size_t)nc64 * npk;
+  const double *W = wave + d.wave_off;
   const signed char *F = fdig + off[di];
-  const signed char *Y = ydig + off[di] + per_t * t;
   const double *fi = finv + d.ib_off;
-  const double *yi = yinv + d.ib_off + inv_per_t * t;
   const int *sib = sib_all + d.ib_off;
+  const int *srow = srow_all + d.ib_off;
+  const double *gt = g + (size_t)n_full * t;
   double *Ht = Hs + (size_t)n_basis * n_basis * t;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -151,79 +163,125 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
   for (int pp = 0; pp < npk; pp += PMAX) {
     const int npp = min(PMAX, npk - pp);  // points of this pass (multiple of 64)
     const int nck = npp / 64;
+    __syncthreads();
+    for (int j = tid; j < npp; j += AZ_THREADS)
+      gs[j] = pp + j < np ? gt[map[d.pstart + pp + j]] : 0.0;
     for (int I = 0; I < nT; I++) {
-      __syncthreads();  // the previous tile's Y is no longer read
-      {
-        const int pieces = S * 64 * (npp / 16);
-        for (int e = tid; e < pieces; e += AZ_THREADS) {
-          const int c16 = e % (npp / 16), row = (e / (npp / 16)) % 64, sl = e / (npp / 16) / 64;
-          *reinterpret_cast<v4i *>(&Ys[(sl * 64 + row) * YROW + 16 * c16]) =
-              *reinterpret_cast<const v4i *>(Y + sl * plane + (size_t)npk * (64 * I + row) + pp + 16 * c16);
+      // Y = g phi for the tile's 64 sorted rows and the pass's points, as S digit planes in LDS with
+      // one scale per row (row a = tid % 64; the 8 waves split the points)
+      const int a = tid & 63;
+      const int ia = 64 * I + a;
+      const double *Wa = ia < nc ? W + srow[ia] : nullptr;
+      const int pe = min(npp, np - pp);  // real points of the pass
+      __syncthreads();                   // gs written; the previous tile's Y and scales are no longer read
+      double mx = 0.0;
+      if (Wa)
+        for (int j = w; j < pe; j += 8)
+          mx = fmax(mx, fabs(Wa[(size_t)ld * (pp + j)] * gs[j]));
+      red[w][a] = mx;
+      __syncthreads();
+      if (w == 0) {
+        double m = red[0][a];
+#pragma unroll
+        for (int q = 1; q < 8; q++)
+          m = fmax(m, red[q][a]);
+        red[0][a] = m > 0.0 ? 127.0 / m : 1.0;
+        ysc_s[a] = m > 0.0 ? m / 127.0 : 0.0;
+      }
+      __syncthreads();
+      const double sc = red[0][a];
+      for (int c16 = w; c16 < npp / 16; c16 += 8) {
+        unsigned int w4[S][4];
+#pragma unroll
+        for (int sl = 0; sl < S; sl++)
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            w4[sl][u] = 0u;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const int pj = 16 * c16 + j;
+          double x = (Wa && pj < pe) ? Wa[(size_t)ld * (pp + pj)] * gs[pj] * sc : 0.0;
+#pragma unroll
+          for (int sl = 0; sl < S; sl++) {
+            const double dg = rint(x);
+            w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
+            x = (x - dg) * RADIX;
+          }
         }
+#pragma unroll
+        for (int sl = 0; sl < S; sl++)
+          *reinterpret_cast<v4i *>(&Ys[(sl * 64 + a) * YROW + 16 * c16]) =
+              v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
       }
       __syncthreads();
       double ysc[2];
 #pragma unroll
-      for (int gi = 0; gi < 2; gi++) {
-        const int i = 64 * I + 16 * (2 * iw + gi) + lr;
-        ysc[gi] = i < nc ? yi[i] : 0.0;
-      }
-      // A fragments of (J, c): phi digits of row 64 J + 16 kw + lr, points pp + 64 c + 16 lk .. + 15
-      auto load_a = [&](int J, int c, v4i (&a)[S]) {
+      for (int gi = 0; gi < 2; gi++)
+        ysc[gi] = ysc_s[16 * (2 * iw + gi) + lr];
+      // A fragments of step s = (J, c) = (I + s / nck, s % nck): phi digits of row 64 J + 16 kw + lr,
+      // points pp + 64 c + 16 lk .. + 15; the fragments of step s + 2 are in flight while step s runs
+      auto load_a = [&](int st, v4i (&a)[S]) {
+        const int J = I + st / nck, c = st % nck;
         const signed char *src = F + (size_t)npk * (64 * J + 16 * kw + lr) + pp + 64 * c + 16 * lk;
 #pragma unroll
         for (int sl = 0; sl < S; sl++)
           a[sl] = *reinterpret_cast<const v4i *>(src + sl * plane);
       };
-      v4i a[S], an[S];
-      load_a(I, 0, a);
-      for (int J = I; J < nT; J++) {
-        v4i acc[S][2];
+      const int nsteps = (nT - I) * nck;
+      v4i a0[S], a1[S], a2[S];
+      load_a(0, a0);
+      if (nsteps > 1)
+        load_a(1, a1);
+      v4i acc[S][2];
+      for (int st = 0, J = I, c = 0; st < nsteps; st++) {
+        if (c == 0) {
 #pragma unroll
-        for (int l = 0; l < S; l++)
+          for (int l = 0; l < S; l++)
 #pragma unroll
-          for (int gi = 0; gi < 2; gi++)
-            acc[l][gi] = v4i{0, 0, 0, 0};
-        for (int c = 0; c < nck; c++) {
-          // prefetch the next chunk (of this pair or of the next one)
-          const bool last = c + 1 == nck;
-          if (!last)
-            load_a(J, c + 1, an);
-          else if (J + 1 < nT)
-            load_a(J + 1, 0, an);
-#pragma unroll
-          for (int gi = 0; gi < 2; gi++) {
-            v4i b[S];
-#pragma unroll
-            for (int sl = 0; sl < S; sl++)
-              b[sl] = *reinterpret_cast<const v4i *>(
-                  &Ys[(sl * 64 + 16 * (2 * iw + gi) + lr) * YROW + 64 * c + 16 * lk]);
-#pragma unroll
-            for (int sa = 0; sa < S; sa++)
-#pragma unroll
-              for (int sb = 0; sb < S - sa; sb++)
-                acc[sa + sb][gi] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][gi], 0, 0, 0);
-          }
-#pragma unroll
-          for (int sl = 0; sl < S; sl++)
-            a[sl] = an[sl];
+            for (int gi = 0; gi < 2; gi++)
+              acc[l][gi] = v4i{0, 0, 0, 0};
         }
-        // acc[l][gi][r] = C_l(k = 64 J + 16 kw + 4 lk + r, i = 64 I + 16 (2 iw + gi) + lr)
+        if (st + 2 < nsteps)
+          load_a(st + 2, a2);
 #pragma unroll
         for (int gi = 0; gi < 2; gi++) {
-          const int i = 64 * I + 16 * (2 * iw + gi) + lr;
-          const size_t x = i < nc ? (size_t)sib[i] : 0;
+          v4i b[S];
 #pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int k = 64 * J + 16 * kw + 4 * lk + r;
-            if (i < nc && k < nc && i <= k) {
-              double v = (double)acc[S - 1][gi][r];
+          for (int sl = 0; sl < S; sl++)
+            b[sl] = *reinterpret_cast<const v4i *>(&Ys[(sl * 64 + 16 * (2 * iw + gi) + lr) * YROW + 64 * c + 16 * lk]);
 #pragma unroll
-              for (int l = S - 2; l >= 0; l--)
-                v = v * (1.0 / RADIX) + (double)acc[l][gi][r];
-              atomicAdd(&Ht[x + (size_t)n_basis * sib[k]], v * ysc[gi] * fi[k]);
+          for (int sa = 0; sa < S; sa++)
+#pragma unroll
+            for (int sb = 0; sb < S - sa; sb++)
+              acc[sa + sb][gi] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[sa], b[sb], acc[sa + sb][gi], 0, 0, 0);
+        }
+        if (c == nck - 1) {
+          // acc[l][gi][r] = C_l(k = 64 J + 16 kw + 4 lk + r, i = 64 I + 16 (2 iw + gi) + lr)
+#pragma unroll
+          for (int gi = 0; gi < 2; gi++) {
+            const int i = 64 * I + 16 * (2 * iw + gi) + lr;
+            const size_t x = i < nc ? (size_t)sib[i] : 0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              const int k = 64 * J + 16 * kw + 4 * lk + r;
+              if (i < nc && k < nc && i <= k) {
+                double v = (double)acc[S - 1][gi][r];
+#pragma unroll
+                for (int l = S - 2; l >= 0; l--)
+                  v = v * (1.0 / RADIX) + (double)acc[l][gi][r];
+                atomicAdd(&Ht[x + (size_t)n_basis * sib[k]], v * ysc[gi] * fi[k]);
+              }
             }
           }
+        }
+#pragma unroll
+        for (int sl = 0; sl < S; sl++) {
+          a0[sl] = a1[sl];
+          a1[sl] = a2[sl];
+        }
+        if (++c == nck) {
+          c = 0;
+          J++;
         }
       }
     }
@@ -247,7 +305,7 @@ void prepare_f(Rho1Plan &P, AdjOzState &st, const double *wave, hipStream_t s) {
   st.d_fdig.alloc(st.per_t);
   st.d_finv.alloc(std::max<size_t>(P.h_ibasis0.size(), 1));
   hipLaunchKernelGGL((h1_digits_kernel<S, false>), dim3(P.n_valid, 1), dim3(256), 0, s, P.d_desc.ptr, st.d_off.ptr,
-                     R.d_srow.ptr, P.d_map.ptr, wave, nullptr, P.n_full, st.per_t, (size_t)0, st.d_fdig.ptr,
+                     R.d_spos.ptr, P.d_map.ptr, wave, nullptr, P.n_full, st.per_t, (size_t)0, st.d_fdig.ptr,
                      st.d_finv.ptr);
   check_launch("h1_digits_kernel");
   st.S = S;
@@ -257,15 +315,8 @@ void prepare_f(Rho1Plan &P, AdjOzState &st, const double *wave, hipStream_t s) {
 template <int S>
 void run(Rho1Plan &P, AdjOzState &st, int T, const double *wave, const double *g, double *Hs, hipStream_t s) {
   const Relabel &R = get_relabel(P);
-  const size_t inv_per_t = std::max<size_t>(P.h_ibasis0.size(), 1);
-  st.d_ydig.alloc((size_t)st.per_t * T);
-  st.d_yinv.alloc(inv_per_t * T);
-  hipLaunchKernelGGL((h1_digits_kernel<S, true>), dim3(P.n_valid, T), dim3(256), 0, s, P.d_desc.ptr, st.d_off.ptr,
-                     R.d_srow.ptr, P.d_map.ptr, wave, g, P.n_full, st.per_t, inv_per_t, st.d_ydig.ptr,
-                     st.d_yinv.ptr);
-  check_launch("h1_digits_kernel");
   hipLaunchKernelGGL((h1_ozaki_kernel<S>), dim3(P.n_valid, T), dim3(AZ_THREADS), 0, s, P.d_desc.ptr, st.d_off.ptr,
-                     R.d_sib.ptr, st.d_fdig.ptr, st.d_finv.ptr, st.d_ydig.ptr, st.d_yinv.ptr, st.per_t, inv_per_t,
+                     R.d_sib.ptr, R.d_srow.ptr, P.d_map.ptr, wave, g, P.n_full, st.d_fdig.ptr, st.d_finv.ptr,
                      P.n_basis, Hs);
   check_launch("h1_ozaki_kernel");
 }

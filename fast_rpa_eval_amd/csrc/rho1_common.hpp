@@ -22,8 +22,8 @@ static_assert(sizeof(BatchDesc) == 32, "BatchDesc layout");
 // new label: srow[k] = local row of the caller's list, sib[k] = pi of that row's basis function
 // (ascending), both compact with the same offsets as Rho1Plan::d_ibasis0.
 struct Relabel {
-  std::vector<int> h_pi, h_pinv, h_srow, h_sib;
-  DevBuf<int> d_pi, d_pinv, d_srow, d_sib;
+  std::vector<int> h_pi, h_pinv, h_srow, h_sib, h_spos;  // spos: inverse of srow per batch
+  DevBuf<int> d_pi, d_pinv, d_srow, d_sib, d_spos;
 };
 
 // How a rho1 forward kernel walks the batches (chosen by frpa_rho1_autotune, or the default):

@@ -41,7 +41,7 @@ struct OzakiState {
   DevBuf<signed char> d_bdig;       // phi digits
   DevBuf<double> d_binv;            // 1 / sb_p per valid point (indexed by pstart + p)
   // per call
-  DevBuf<long long> d_adig;         // P' digits in the new labels, (n_basis, n_basis, T)
+  DevBuf<unsigned long long> d_adig;  // P' digits (byte s = digit s) in the new labels, (n, n, T)
   DevBuf<double> d_ainv;            // 1 / sa in the new labels, (n_basis, T)
   DevBuf<unsigned long long> d_rowbits;  // bound_r = max(max_c |P1(r, c)|, max_c |P1(c, r)|), IEEE bits
   int n_valid_points = 0;
@@ -146,12 +146,13 @@ __global__ void p1_rowscale_kernel(const unsigned long long *__restrict__ rowbit
   ainv[i] = mx > 0.0 ? mx / 127.0 : 1.0;
 }
 
-// adig[x + n y] = digits of P'(pinv[x], pinv[y]) / ainv[x]: one 64 x 64 tile of the new labels per
-// block; the transposed tile P1(pinv[y], pinv[x]) goes through LDS so both reads run down columns.
+// Digits of P'(pinv[x], pinv[y]) / ainv[x] at x + n y, digit s in byte s of one 8-byte word (one
+// gather load per element); one 64 x 64 tile of the new labels per block; the transposed tile
+// P1(pinv[y], pinv[x]) goes through LDS so both reads run down columns.
 template <int S>
 __global__ __launch_bounds__(256) void p1_digits_kernel(const double *__restrict__ P1, const int *__restrict__ pinv,
                                                         const double *__restrict__ ainv, int n_basis,
-                                                        long long *__restrict__ adig) {
+                                                        unsigned long long *__restrict__ adig) {
   __shared__ double tr[64][65];
   const int x0 = 64 * blockIdx.x, y0 = 64 * blockIdx.y, t = blockIdx.z;
   const size_t nn = (size_t)n_basis * n_basis;
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(256) void p1_digits_kernel(const double *__restrict
       packed |= (unsigned long long)(unsigned char)(signed char)dg << (8 * sl);
       z = (z - dg) * RADIX;
     }
-    adig[x + (size_t)n_basis * y + nn * t] = (long long)packed;
+    adig[x + (size_t)n_basis * y + nn * t] = packed;
   }
 }
 
@@ -221,20 +222,22 @@ __device__ __forceinline__ void repack_a(const unsigned long long (&e)[16], v4i 
 
 // One point chunk; this wave multiplies NG (exact, compile-time) groups of 16 points starting at
 // group gp0 of the chunk.  Every wave of the block runs the same barriers whatever its NG.
+// LDS: three staged phi-digit chunks -- buffer 2 holds the diagonal chunk J = I for the whole row
+// tile (its digits also give phi(i, p) for the ddot, so the FP64 wave is not read), buffers 0 and 1
+// alternate for J > I.
 template <int S, int NPG, int NG>
 __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *red, const BatchDesc &d, int t,
                                             int pc, int npc, int gp0, const int *__restrict__ sib,
-                                            const int *__restrict__ srow, const int *__restrict__ map,
-                                            const double *__restrict__ W, const long long *__restrict__ At,
-                                            const double *__restrict__ ai, const signed char *__restrict__ Bg,
-                                            const double *__restrict__ binv, int n_basis,
-                                            const double *__restrict__ bias, double *__restrict__ rho, int n_full,
-                                            int nmb, int n_batches_all, int n_tile) {
+                                            const int *__restrict__ map, const unsigned long long *__restrict__ At,
+                                            const double *__restrict__ ai,
+                                            const signed char *__restrict__ Bg, const double *__restrict__ binv,
+                                            int n_basis, const double *__restrict__ bias, double *__restrict__ rho,
+                                            int n_full, int nmb, int n_batches_all, int n_tile) {
   constexpr int PCH = 32 * NPG;                 // point slots per chunk
   constexpr int BUF = S * PCH * BROW;           // bytes per LDS buffer
   constexpr int NQ = S * PCH * 4;               // 16-byte pieces of one staged chunk
   constexpr int QPT = (NQ + OZ_THREADS - 1) / OZ_THREADS;
-  const int nc = d.nc, ld = (nc + 3) & ~3, ncK = ceil64(nc);
+  const int nc = d.nc, ncK = ceil64(nc);
   const size_t plane = (size_t)ncK * d.np;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rg = w & 3;
   const int lr = lane & 15, lk = lane >> 4;
@@ -288,19 +291,19 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
 #pragma unroll
       for (int q = 0; q < 16; q++) {
         const int jb = jbs[64 * buf + 16 * lk + q];
-        const bool upper = c > I || 64 * c + 16 * lk + q >= irow;
-        e[q] = (iok && jb >= 0 && upper) ? (unsigned long long)At[arow + (size_t)n_basis * jb] : 0ull;
+        const bool ok = iok && jb >= 0 && (c > I || 64 * c + 16 * lk + q >= irow);
+        e[q] = ok ? At[arow + (size_t)n_basis * jb] : 0ull;
       }
     };
-    __syncthreads();  // the previous tile is done with both buffers
+    __syncthreads();  // the previous tile is done with all buffers
     load_b(I);
-    store_b(I, 0);
+    store_b(I, 2);
     __syncthreads();
     v4i a[S];
-    load_a(I, 0);
+    load_a(I, 2);
     repack_a<S>(e, a);
     for (int c = I; c < nT; c++) {
-      const int cur = (c - I) & 1;
+      const int cur = c == I ? 2 : (c - I - 1) & 1, nxt = (c - I) & 1;
       const bool more = c + 1 < nT;
       if (more)
         load_b(c + 1);
@@ -318,41 +321,48 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
             acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
       }
       if (more)
-        store_b(c + 1, cur ^ 1);
+        store_b(c + 1, nxt);
       __syncthreads();
       if (more) {
-        load_a(c + 1, cur ^ 1);
+        load_a(c + 1, nxt);
         repack_a<S>(e, a);
       }
     }
 
     // epilogue: v_mfma_i32_16x16x64_i8 leaves D(4 (lane / 16) + r, lane % 16) in acc[r], so
-    // acc[l][g][r] = C_l(64 I + 16 rg + 4 lk + r, pc + 16 (gp0 + g) + lr)
+    // acc[l][g][r] = C_l(i = 64 I + 16 rg + 4 lk + r, p = pc + 16 (gp0 + g) + lr); phi(i, p) sb_p is
+    // rebuilt from the diagonal chunk's digits (rows 16 rg + 4 lk .. + 3 are one dword per plane):
+    //   rho(p) += W(i, p) phi(i, p) = (sum_l 254^-l C_l / sa_i) (sum_s b_s 254^-s) / sb_p^2
     double rinv[4];
-    int rrow[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int i = 64 * I + 16 * rg + 4 * lk + r;
       rinv[r] = i < nc ? ai[sib[i]] : 0.0;
-      rrow[r] = i < nc ? srow[i] : 0;
     }
+    const signed char *BD = Bs + 2 * BUF;
 #pragma unroll
     for (int g = 0; g < NG; g++) {
       const int p = 16 * (gp0 + g) + lr;
       double part = 0.0;
       if (p < npc) {
-        const double bi = binv[d.pstart + pc + p];
+        unsigned dw[S];
+#pragma unroll
+        for (int sl = 0; sl < S; sl++)
+          dw[sl] = *reinterpret_cast<const unsigned *>(&BD[(sl * PCH + p) * BROW + 16 * rg + 4 * lk]);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int i = 64 * I + 16 * rg + 4 * lk + r;
-          if (i < nc) {
-            double wv = (double)acc[S - 1][g][r];
+          double wv = (double)acc[S - 1][g][r];
 #pragma unroll
-            for (int l = S - 2; l >= 0; l--)
-              wv = wv * (1.0 / RADIX) + (double)acc[l][g][r];
-            part += wv * rinv[r] * bi * W[rrow[r] + (size_t)ld * (pc + p)];
-          }
+          for (int l = S - 2; l >= 0; l--)
+            wv = wv * (1.0 / RADIX) + (double)acc[l][g][r];
+          double ph = (double)(signed char)(dw[S - 1] >> (8 * r));
+#pragma unroll
+          for (int sl = S - 2; sl >= 0; sl--)
+            ph = ph * (1.0 / RADIX) + (double)(signed char)(dw[sl] >> (8 * r));
+          part += wv * rinv[r] * ph;
         }
+        const double bi = binv[d.pstart + pc + p];
+        part *= bi * bi;
       }
       part += __shfl_xor(part, 16);
       part += __shfl_xor(part, 32);
@@ -381,11 +391,11 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
 template <int S, int NPG, int... Gs>
 __device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>, int ng, signed char *Bs, int *jbs,
                                                double *red, const BatchDesc &d, int t, int pc, int npc, int gp0,
-                                               const int *sib, const int *srow, const int *map, const double *W,
-                                               const long long *At, const double *ai, const signed char *Bg,
+                                               const int *sib, const int *map, const unsigned long long *At,
+                                               const double *ai, const signed char *Bg,
                                                const double *binv, int n_basis, const double *bias, double *rho,
                                                int n_full, int nmb, int n_batches_all, int n_tile) {
-  ((ng == Gs ? ozaki_chunk<S, NPG, Gs>(Bs, jbs, red, d, t, pc, npc, gp0, sib, srow, map, W, At, ai, Bg, binv, n_basis,
+  ((ng == Gs ? ozaki_chunk<S, NPG, Gs>(Bs, jbs, red, d, t, pc, npc, gp0, sib, map, At, ai, Bg, binv, n_basis,
                                        bias, rho, n_full, nmb, n_batches_all, n_tile)
              : (void)0),
    ...);
@@ -394,14 +404,13 @@ __device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>
 template <int S, int NPG>
 __global__ __launch_bounds__(OZ_THREADS) void rho1_ozaki_kernel(
     const BatchDesc *__restrict__ descs, const int *__restrict__ order, int *__restrict__ counters, int n_pos, int T,
-    const int *__restrict__ sib_all, const int *__restrict__ srow_all, const int *__restrict__ map,
-    const double *__restrict__ wave, const long long *__restrict__ adig, const double *__restrict__ ainv,
-    const signed char *__restrict__ bdig, const long long *__restrict__ boff, const double *__restrict__ binv,
-    int n_basis, const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all,
-    int n_tile) {
+    const int *__restrict__ sib_all, const int *__restrict__ map, const unsigned long long *__restrict__ adig,
+    const double *__restrict__ ainv, const signed char *__restrict__ bdig,
+    const long long *__restrict__ boff, const double *__restrict__ binv, int n_basis,
+    const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all, int n_tile) {
   constexpr int PCH = 32 * NPG;
-  __shared__ __attribute__((aligned(16))) signed char Bs[2 * S * PCH * BROW];
-  __shared__ int jbs[128];
+  __shared__ __attribute__((aligned(16))) signed char Bs[3 * S * PCH * BROW];
+  __shared__ int jbs[192];
   __shared__ double red[4 * PCH];
   __shared__ int s_item;
   const int half = threadIdx.x >> 8;
@@ -415,10 +424,8 @@ __global__ __launch_bounds__(OZ_THREADS) void rho1_ozaki_kernel(
     }
     const int di = order ? order[pos] : pos;
     const BatchDesc d = descs[di];
-    const double *W = wave + d.wave_off;
     const int *sib = sib_all + d.ib_off;
-    const int *srow = srow_all + d.ib_off;
-    const long long *At = adig + (size_t)n_basis * n_basis * t;
+    const size_t nn = (size_t)n_basis * n_basis * t;
     const double *ai = ainv + (size_t)n_basis * t;
     const signed char *Bg = bdig + boff[di];
     for (int pc = 0; pc < d.np; pc += PCH) {
@@ -427,7 +434,7 @@ __global__ __launch_bounds__(OZ_THREADS) void rho1_ozaki_kernel(
       const int n0 = (ngt + 1) >> 1;         // the first half takes the extra group
       const int ng = half ? ngt - n0 : n0, gp0 = half ? n0 : 0;
       ozaki_dispatch<S, NPG>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc, gp0, sib,
-                             srow, map, W, At, ai, Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all, n_tile);
+                             map, adig + nn, ai, Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all, n_tile);
     }
   }
 }
@@ -466,9 +473,9 @@ void launch(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s)
   }
   const LaunchArgs la = launch_args(P, cfg, c.T, per_cu, s);
   hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG>), la.grid, dim3(OZ_THREADS), 0, s, P.d_desc.ptr, la.order,
-                     la.counters, P.n_valid, c.T, R.d_sib.ptr, R.d_srow.ptr, P.d_map.ptr, c.wave, st.d_adig.ptr,
-                     st.d_ainv.ptr, st.d_bdig.ptr, st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full,
-                     P.nmb, P.n_batches, c.n_tile);
+                     la.counters, P.n_valid, c.T, R.d_sib.ptr, P.d_map.ptr, st.d_adig.ptr, st.d_ainv.ptr,
+                     st.d_bdig.ptr, st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full, P.nmb,
+                     P.n_batches, c.n_tile);
   check_launch("rho1_ozaki_kernel");
 }
 

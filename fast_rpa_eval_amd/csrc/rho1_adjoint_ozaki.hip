@@ -124,9 +124,9 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const BatchDesc *__restr
   }
 }
 
-// Block: 8 waves; wave w multiplies phi rows 16 (w & 3) .. +15 of tile J (MFMA rows) against the Y
-// columns of 2 of the 4 16-row groups of tile I (w >> 2), so D(k, i) lands with i along lane % 16
-// (consecutive relabelled rows of H for the atomics).
+// Block: 8 waves; wave w multiplies phi rows 16 (w & 3) .. +15 of a tile J (MFMA rows) against the
+// 64 Y columns of tile I, the two halves w >> 2 taking alternate J, so D(k, i) lands with i along
+// lane % 16 (consecutive relabelled rows of H for the atomics).
 template <int S>
 __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *__restrict__ descs,
                                                               const long long *__restrict__ off,
@@ -214,41 +214,45 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
               v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
       }
       __syncthreads();
-      double ysc[2];
+      double ysc[4];
 #pragma unroll
-      for (int gi = 0; gi < 2; gi++)
-        ysc[gi] = ysc_s[16 * (2 * iw + gi) + lr];
-      // A fragments of step s = (J, c) = (I + s / nck, s % nck): phi digits of row 64 J + 16 kw + lr,
-      // points pp + 64 c + 16 lk .. + 15; the fragments of step s + 2 are in flight while step s runs
+      for (int gi = 0; gi < 4; gi++)
+        ysc[gi] = ysc_s[16 * gi + lr];
+      // The two halves of the block (iw) take alternate column tiles J = I + iw, I + iw + 2, ...; a
+      // wave multiplies phi rows 16 kw .. + 15 of its J against all 64 Y rows of tile I.
+      // A fragments of step s = (J, c) = (I + iw + 2 (s / nck), s % nck): phi digits of row
+      // 64 J + 16 kw + lr, points pp + 64 c + 16 lk .. + 15; step s + 2 is in flight while s runs.
       auto load_a = [&](int st, v4i (&a)[S]) {
-        const int J = I + st / nck, c = st % nck;
+        const int J = I + iw + 2 * (st / nck), c = st % nck;
         const signed char *src = F + (size_t)npk * (64 * J + 16 * kw + lr) + pp + 64 * c + 16 * lk;
 #pragma unroll
         for (int sl = 0; sl < S; sl++)
           a[sl] = *reinterpret_cast<const v4i *>(src + sl * plane);
       };
-      const int nsteps = (nT - I) * nck;
+      const int npairs = nT - I > iw ? (nT - I - iw + 1) / 2 : 0;
+      const int nsteps = npairs * nck;
       v4i a0[S], a1[S], a2[S];
-      load_a(0, a0);
+      if (nsteps > 0)
+        load_a(0, a0);
       if (nsteps > 1)
         load_a(1, a1);
-      v4i acc[S][2];
-      for (int st = 0, J = I, c = 0; st < nsteps; st++) {
+      v4i acc[S][4];
+      for (int st = 0, J = I + iw, c = 0; st < nsteps; st++) {
         if (c == 0) {
 #pragma unroll
           for (int l = 0; l < S; l++)
 #pragma unroll
-            for (int gi = 0; gi < 2; gi++)
+            for (int gi = 0; gi < 4; gi++)
               acc[l][gi] = v4i{0, 0, 0, 0};
         }
         if (st + 2 < nsteps)
           load_a(st + 2, a2);
 #pragma unroll
-        for (int gi = 0; gi < 2; gi++) {
+        for (int gi = 0; gi < 4; gi++) {
           v4i b[S];
 #pragma unroll
           for (int sl = 0; sl < S; sl++)
-            b[sl] = *reinterpret_cast<const v4i *>(&Ys[(sl * 64 + 16 * (2 * iw + gi) + lr) * YROW + 64 * c + 16 * lk]);
+            b[sl] = *reinterpret_cast<const v4i *>(&Ys[(sl * 64 + 16 * gi + lr) * YROW + 64 * c + 16 * lk]);
 #pragma unroll
           for (int sa = 0; sa < S; sa++)
 #pragma unroll
@@ -256,10 +260,10 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
               acc[sa + sb][gi] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[sa], b[sb], acc[sa + sb][gi], 0, 0, 0);
         }
         if (c == nck - 1) {
-          // acc[l][gi][r] = C_l(k = 64 J + 16 kw + 4 lk + r, i = 64 I + 16 (2 iw + gi) + lr)
+          // acc[l][gi][r] = C_l(k = 64 J + 16 kw + 4 lk + r, i = 64 I + 16 gi + lr)
 #pragma unroll
-          for (int gi = 0; gi < 2; gi++) {
-            const int i = 64 * I + 16 * (2 * iw + gi) + lr;
+          for (int gi = 0; gi < 4; gi++) {
+            const int i = 64 * I + 16 * gi + lr;
             const size_t x = i < nc ? (size_t)sib[i] : 0;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
         }
         if (++c == nck) {
           c = 0;
-          J++;
+          J += 2;
         }
       }
     }

@@ -13,7 +13,7 @@
 // Layout: per batch, S digit planes of phi with (nc rounded to 64) rows in the batch's relabelled
 // order (get_relabel) times (np rounded to 64) points, point index contiguous, so that a lane's MFMA
 // fragment (16 consecutive points of one row) is one 16-byte load.  The kernel computes the Y digits
-// of one 64-row tile I for all points of the pass straight from the FP64 phi and g into LDS, and
+// of one 64-row tile I for all points of the pass from the phi digits and g into LDS, and
 // streams the phi digits of the tiles J >= I from global memory (two steps of register prefetch, no
 // barrier inside the J loop); the upper triangle of each tile pair is added with FP64 atomics into
 // the relabelled scratch H (see rho1_adjoint.hip).
@@ -131,9 +131,7 @@ template <int S>
 __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *__restrict__ descs,
                                                               const long long *__restrict__ off,
                                                               const int *__restrict__ sib_all,
-                                                              const int *__restrict__ srow_all,
                                                               const int *__restrict__ map,
-                                                              const double *__restrict__ wave,
                                                               const double *__restrict__ g, int n_full,
                                                               const signed char *__restrict__ fdig,
                                                               const double *__restrict__ finv, int n_basis,
@@ -145,14 +143,12 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
   __shared__ double ysc_s[64];     // 1 / sy of the tile's rows
   const int di = blockIdx.x, t = blockIdx.y;
   const BatchDesc d = descs[di];
-  const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
+  const int nc = d.nc, np = d.np;
   const int nc64 = ceil64(nc), npk = ceil64(np);
   const size_t plane = (size_t)nc64 * npk;
-  const double *W = wave + d.wave_off;
   const signed char *F = fdig + off[di];
   const double *fi = finv + d.ib_off;
   const int *sib = sib_all + d.ib_off;
-  const int *srow = srow_all + d.ib_off;
   const double *gt = g + (size_t)n_full * t;
   double *Ht = Hs + (size_t)n_basis * n_basis * t;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -168,16 +164,38 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
       gs[j] = pp + j < np ? gt[map[d.pstart + pp + j]] : 0.0;
     for (int I = 0; I < nT; I++) {
       // Y = g phi for the tile's 64 sorted rows and the pass's points, as S digit planes in LDS with
-      // one scale per row (row a = tid % 64; the 8 waves split the points)
+      // one scale per row.  phi comes from its own digits (x = phi sf, 16 points of one row per
+      // 16-byte load per plane): z = g x, held in registers for the row maximum m; the Y digits are
+      // those of 127 z / m and 1 / sy = m / (127 sf).  Row a = tid % 64; the 8 waves take the
+      // 16-point groups w and w + 8 of the pass.
       const int a = tid & 63;
       const int ia = 64 * I + a;
-      const double *Wa = ia < nc ? W + srow[ia] : nullptr;
-      const int pe = min(npp, np - pp);  // real points of the pass
-      __syncthreads();                   // gs written; the previous tile's Y and scales are no longer read
+      __syncthreads();  // gs written; the previous tile's Y and scales are no longer read
+      double z[2][16];
       double mx = 0.0;
-      if (Wa)
-        for (int j = w; j < pe; j += 8)
-          mx = fmax(mx, fabs(Wa[(size_t)ld * (pp + j)] * gs[j]));
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int c16 = w + 8 * h;
+        if (c16 < npp / 16) {
+          v4i fr[S];
+#pragma unroll
+          for (int sl = 0; sl < S; sl++)
+            fr[sl] = *reinterpret_cast<const v4i *>(F + sl * plane + (size_t)npk * ia + pp + 16 * c16);
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            double x = (double)(signed char)((unsigned)fr[S - 1][j >> 2] >> (8 * (j & 3)));
+#pragma unroll
+            for (int sl = S - 2; sl >= 0; sl--)
+              x = x * (1.0 / RADIX) + (double)(signed char)((unsigned)fr[sl][j >> 2] >> (8 * (j & 3)));
+            z[h][j] = x * gs[16 * c16 + j];
+            mx = fmax(mx, fabs(z[h][j]));
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < 16; j++)
+            z[h][j] = 0.0;
+        }
+      }
       red[w][a] = mx;
       __syncthreads();
       if (w == 0) {
@@ -186,32 +204,35 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
         for (int q = 1; q < 8; q++)
           m = fmax(m, red[q][a]);
         red[0][a] = m > 0.0 ? 127.0 / m : 1.0;
-        ysc_s[a] = m > 0.0 ? m / 127.0 : 0.0;
+        ysc_s[a] = ia < nc && m > 0.0 ? m / 127.0 * fi[ia] : 0.0;
       }
       __syncthreads();
       const double sc = red[0][a];
-      for (int c16 = w; c16 < npp / 16; c16 += 8) {
-        unsigned int w4[S][4];
 #pragma unroll
-        for (int sl = 0; sl < S; sl++)
+      for (int h = 0; h < 2; h++) {
+        const int c16 = w + 8 * h;
+        if (c16 < npp / 16) {
+          unsigned int w4[S][4];
 #pragma unroll
-          for (int u = 0; u < 4; u++)
-            w4[sl][u] = 0u;
+          for (int sl = 0; sl < S; sl++)
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-          const int pj = 16 * c16 + j;
-          double x = (Wa && pj < pe) ? Wa[(size_t)ld * (pp + pj)] * gs[pj] * sc : 0.0;
+            for (int u = 0; u < 4; u++)
+              w4[sl][u] = 0u;
 #pragma unroll
-          for (int sl = 0; sl < S; sl++) {
-            const double dg = rint(x);
-            w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
-            x = (x - dg) * RADIX;
+          for (int j = 0; j < 16; j++) {
+            double x = z[h][j] * sc;
+#pragma unroll
+            for (int sl = 0; sl < S; sl++) {
+              const double dg = rint(x);
+              w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
+              x = (x - dg) * RADIX;
+            }
           }
-        }
 #pragma unroll
-        for (int sl = 0; sl < S; sl++)
-          *reinterpret_cast<v4i *>(&Ys[(sl * 64 + a) * YROW + 16 * c16]) =
-              v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
+          for (int sl = 0; sl < S; sl++)
+            *reinterpret_cast<v4i *>(&Ys[(sl * 64 + a) * YROW + 16 * c16]) =
+                v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
+        }
       }
       __syncthreads();
       double ysc[4];
@@ -320,7 +341,7 @@ template <int S>
 void run(Rho1Plan &P, AdjOzState &st, int T, const double *wave, const double *g, double *Hs, hipStream_t s) {
   const Relabel &R = get_relabel(P);
   hipLaunchKernelGGL((h1_ozaki_kernel<S>), dim3(P.n_valid, T), dim3(AZ_THREADS), 0, s, P.d_desc.ptr, st.d_off.ptr,
-                     R.d_sib.ptr, R.d_srow.ptr, P.d_map.ptr, wave, g, P.n_full, st.d_fdig.ptr, st.d_finv.ptr,
+                     R.d_sib.ptr, P.d_map.ptr, g, P.n_full, st.d_fdig.ptr, st.d_finv.ptr,
                      P.n_basis, Hs);
   check_launch("h1_ozaki_kernel");
 }

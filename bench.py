@@ -204,7 +204,7 @@ def work_model(prob, kernel, prec, S=None):
 PROFILED_KERNEL = {  # bench kernel (path) -> kernel name prefix in the rocprofv3 summaries
     ("rho1", "fast"): "frpa::rho1_ozaki_kernel", ("rho1", "fp64"): "frpa::rho1_fp64_kernel",
     ("adjoint", "fast"): "frpa::h1_ozaki_kernel", ("adjoint", "fp64"): "frpa::h1_push_kernel",
-    ("far", "fast"): "frpa::far_forward_kernel<1,", ("far", "fp64"): "frpa::far_forward_kernel<0,",
+    ("far", "fast"): "frpa::far_horner_kernel", ("far", "fp64"): "frpa::far_forward_kernel<0,",
 }
 
 
@@ -333,6 +333,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--paths", default="fast,fp64")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default rho1 launch configuration")
+    ap.add_argument("--launch-fast", metavar="ORDER,PERSISTENT,NPG",
+                    help="fixed rho1 launch configuration of the fast path instead of the autotuner (profiles)")
+    ap.add_argument("--launch-fp64", metavar="ORDER,PERSISTENT,NPG",
+                    help="fixed rho1 launch configuration of the FP64 path instead of the autotuner")
     ap.add_argument("--graph", type=int, default=1,
                     help="1: the timed step is one replay of the step captured in a HIP graph (per-kernel "
                          "times from an eager pass); 0: eager launches")
@@ -408,7 +412,12 @@ def main():
         prec = api.OZAKI if (path == "fast" and have_ozaki) else api.FP64
         adj_prec = api.OZAKI if (path == "fast" and have_ozaki_adj) else api.FP64
         var = api.FAR_HORNER if path == "fast" else api.FAR_TERMS
-        if not a.no_autotune and "rho1" in names:
+        fixed = a.launch_fast if path == "fast" else a.launch_fp64
+        if fixed:
+            o, pers, npg = (int(x) for x in fixed.split(","))
+            D["plan"].set_launch(prec, o, pers, npg)
+            tuned[path] = f"fixed order={o} persistent={pers} npg={npg}"
+        elif not a.no_autotune and "rho1" in names:
             t0 = time.time()
             tuned[path] = D["plan"].autotune(D["wave"], D["P1"], precision=prec, reps=2)
             log(f"autotune {path} ({time.time() - t0:.1f}s): {tuned[path]}")

@@ -22,6 +22,7 @@ from . import _lib
 from ._lib import check, lib
 
 FP64, OZAKI, OZAKI4 = _lib.FRPA_PREC_FP64, _lib.FRPA_PREC_OZAKI, _lib.FRPA_PREC_OZAKI4
+FP32, BF16X3 = _lib.FRPA_PREC_FP32, _lib.FRPA_PREC_BF16X3
 FAR_TERMS, FAR_HORNER = _lib.FRPA_FAR_TERMS, _lib.FRPA_FAR_HORNER
 
 

@@ -381,6 +381,8 @@ int frpa_rho1_forward(frpa_rho1_plan *plan, int atom_tile_size, int n_tile, int 
       rho1_forward_fp64(P, c, s);
     } else if (precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4) {
       rho1_forward_ozaki(P, c, precision == FRPA_PREC_OZAKI ? 5 : 4, s);
+    } else if (precision == FRPA_PREC_FP32 || precision == FRPA_PREC_BF16X3) {
+      rho1_forward_lowp(P, c, precision, s);
     } else {
       FRPA_REQUIRE(false, FRPA_ERR_INVALID, "unknown precision %d", precision);
     }

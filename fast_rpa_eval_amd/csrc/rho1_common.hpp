@@ -76,6 +76,7 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
                          const long long *ibasis_ptr, const int *ibasis, const long long *wave_offset);
 void rho1_forward_fp64(Rho1Plan &P, const Rho1Call &c, hipStream_t s);
 void rho1_forward_ozaki(Rho1Plan &P, const Rho1Call &c, int slices, hipStream_t s);
+void rho1_forward_lowp(Rho1Plan &P, const Rho1Call &c, int precision, hipStream_t s);
 void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const double *g, double *H, hipStream_t s);
 const Relabel &get_relabel(Rho1Plan &P);
 // Kernel arguments of a launch configuration: the position -> BatchDesc map (nullptr = identity),

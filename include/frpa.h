@@ -50,6 +50,8 @@ typedef struct frpa_rho1_plan frpa_rho1_plan;
 #define FRPA_PREC_OZAKI 1  /* INT8-MFMA split, 5 radix-254 digits per operand, 15 exact INT8     */
                            /* GEMMs, FP64 recombination (max|d rho1|/max|rho1| ~ 1e-12)          */
 #define FRPA_PREC_OZAKI4 2 /* INT8-MFMA split with 4 digits, 10 INT8 GEMMs (~ 5e-10)            */
+#define FRPA_PREC_FP32 3   /* rho1 forward only, for comparison: FP32 MFMA, FP32 chunk sums      */
+#define FRPA_PREC_BF16X3 4 /* rho1 forward only, for comparison: bf16 hi/lo split, 3 products    */
 
 /*
  * Batch metadata (host arrays, copied into the plan):

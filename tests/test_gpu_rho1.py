@@ -178,3 +178,15 @@ def test_rho1_launch_configs_bit_identical(cuda, oracle, precision, npgs):
     print(f"[autotune precision {precision}] {desc}", flush=True)
     assert desc.count("order=") == 2 * 2 * len(npgs)
     assert torch.equal(plan.forward(wave, P1, T=T, precision=precision), ref)
+
+
+@pytest.mark.parametrize("precision,tol", [(api.FP32, 1e-5), (api.BF16X3, 1e-3)])
+def test_rho1_reduced_precision_comparison_paths(cuda, oracle, precision, tol):
+    """FP32-MFMA and BF16x3 rho1 (the comparison paths): their max|d rho1| / max|rho1| against the FP64
+    oracle is printed and bounded loosely; it stays far above the 1e-9 the INT8 split meets."""
+    prob = synth.make_rho1_problem(24, 14.0, 90, (1, 230), (1, 180), config_k=3, zero_nc=(3,))
+    P1 = synth.make_p1(prob.atom_xyz, prob.basis, 3003)
+    bias, _ = _bias(prob, 1, 4)
+    ref, _ = oracle.rho1(prob, P1, T=1, bias=bias)
+    got, _ = _run_gpu(prob, P1, 1, 1, bias=bias, precision=precision)
+    _check(got, ref, rel=tol, what=f"rho1 comparison path (precision {precision})")

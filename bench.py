@@ -251,6 +251,42 @@ def load_traffic(workload):
     return out
 
 
+def compare_precisions(D, rho_fp64, steps):
+    """rho1 of every precision on the same inputs: ms per call and the error against the FP64 GPU path,
+    max over points of |d rho1(p)| / max|rho1| and the median over points with |rho1(p)| > 1e-3 max of
+    the pointwise |d rho1(p)| / |rho1(p)|."""
+    import torch
+
+    from fast_rpa_eval_amd import api
+    from fast_rpa_eval_amd._lib import FrpaError
+    prob = D["prob"]
+    valid = torch.from_numpy(prob.i_valid_point_2_i_full_points_map[:prob.n_point_batches_prefix_sum[-1]]).cuda().long()
+    ref = rho_fp64[valid]
+    big = ref.abs() > 1e-3 * ref.abs().max()
+    out = torch.empty_like(rho_fp64)
+    res = {}
+    for name, prec in (("int8_split_s5", api.OZAKI), ("int8_split_s4", api.OZAKI4), ("fp32_mfma", api.FP32),
+                       ("bf16x3", api.BF16X3)):
+        try:
+            D["plan"].forward(D["wave"], D["P1"], precision=prec, out=out)
+        except FrpaError as e:
+            res[name] = {"error": str(e)}
+            continue
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(steps):
+            D["plan"].forward(D["wave"], D["P1"], precision=prec, out=out)
+        t1.record()
+        torch.cuda.synchronize()
+        d = (out[valid] - ref).abs()
+        res[name] = {"ms": t0.elapsed_time(t1) / steps, "max_rel_err": float(d.max() / ref.abs().max()),
+                     "median_pointwise_rel_err": float((d[big] / ref[big].abs()).median())}
+        log(f"precision {name}: {res[name]}")
+    return res
+
+
 def cpu_baseline(D, threads, rho_s=12.0, far_s=6.0):
     """CPU FP64 oracle on a bounded sample of the same inputs: random batches (rho1) and the first
     far-field points, each sized from a short pilot run to take about rho_s / far_s seconds."""
@@ -333,6 +369,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--paths", default="fast,fp64")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default rho1 launch configuration")
+    ap.add_argument("--compare-precisions", type=int, default=1,
+                    help="1: also time rho1 in every precision (INT8 split S = 5 / 4, FP32 MFMA, BF16x3) and state "
+                         "its error against the FP64 path")
     ap.add_argument("--launch-fast", metavar="ORDER,PERSISTENT,NPG",
                     help="fixed rho1 launch configuration of the fast path instead of the autotuner (profiles)")
     ap.add_argument("--launch-fp64", metavar="ORDER,PERSISTENT,NPG",
@@ -464,6 +503,9 @@ def main():
         if have_adj:
             h_f, h_d = outs["fast"]["H"], outs["fp64"]["H"]
             line["h1_rel_err_vs_fp64"] = float((h_f - h_d).abs().max() / h_d.abs().max())
+
+    if a.compare_precisions and "fp64" in outs and "rho1" in names:
+        line["rho1_precision_comparison"] = compare_precisions(D, outs["fp64"]["rho"], a.steps)
 
     # roofline of every kernel of the fast path: algorithmic work / measured time against the peak of
     # whichever resource binds the kernel at its algorithmic work; the dominant kernel is "roofline"

@@ -189,4 +189,7 @@ def test_rho1_reduced_precision_comparison_paths(cuda, oracle, precision, tol):
     bias, _ = _bias(prob, 1, 4)
     ref, _ = oracle.rho1(prob, P1, T=1, bias=bias)
     got, _ = _run_gpu(prob, P1, 1, 1, bias=bias, precision=precision)
-    _check(got, ref, rel=tol, what=f"rho1 comparison path (precision {precision})")
+    # the reference's 1e-6 absolute FP64 check does not apply to these paths: only the relative bound
+    rel = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"[rho1 comparison path (precision {precision})] max|err| / max|ref| {rel:.3e}", flush=True)
+    assert np.all(np.isfinite(got)) and rel <= tol

@@ -2,7 +2,8 @@
 
 * ``Rho1Function``: P1 -> first_order_rho; backward is the first-order H integration
   d L / d P1(mu, nu, t) = sum_p dL/drho1(map(p), t) phi_mu(p) phi_nu(p)
-  (frpa_rho1_adjoint; evaluate_first_order_H.cpp:144-215 with the GGA terms zero).
+  (frpa_rho1_adjoint; evaluate_first_order_H.cpp:144-215 with the GGA terms zero), in FP64 or the
+  INT8 split.
 * ``FarFieldFunction``: multipole_moments -> delta_v_hartree (far field); backward
   d L / d M(lm, a, t) = sum_p dL/dV(p, t) d V(p, t) / d M(lm, a, t) (frpa_far_adjoint).
 
@@ -16,15 +17,15 @@ from . import api
 
 class Rho1Function(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, density_matrix, plan, wave, T=1, precision=api.FP64):
+    def forward(ctx, density_matrix, plan, wave, T=1, precision=api.FP64, adjoint_precision=api.FP64):
         rho = plan.forward(wave, density_matrix.detach().contiguous(), T=T, precision=precision)
-        ctx.plan, ctx.wave, ctx.T = plan, wave, T
+        ctx.plan, ctx.wave, ctx.T, ctx.adjoint_precision = plan, wave, T, adjoint_precision
         return rho
 
     @staticmethod
     def backward(ctx, grad_rho):
-        H = ctx.plan.adjoint(ctx.wave, grad_rho.contiguous(), T=ctx.T, precision=api.FP64)
-        return H, None, None, None, None
+        H = ctx.plan.adjoint(ctx.wave, grad_rho.contiguous(), T=ctx.T, precision=ctx.adjoint_precision)
+        return H, None, None, None, None, None
 
 
 class FarFieldFunction(torch.autograd.Function):
@@ -47,8 +48,10 @@ class FarFieldFunction(torch.autograd.Function):
         return gM, None, None, None
 
 
-def rho1(density_matrix, plan, wave, T=1, precision=api.FP64):
-    return Rho1Function.apply(density_matrix, plan, wave, T, precision)
+def rho1(density_matrix, plan, wave, T=1, precision=api.FP64, adjoint_precision=api.FP64):
+    """first_order_rho as a differentiable function of the density matrix; the backward runs the
+    adjoint in `adjoint_precision` (FP64 or the INT8 split)."""
+    return Rho1Function.apply(density_matrix, plan, wave, T, precision, adjoint_precision)
 
 
 def far_field(multipole_moments, plan, inputs, variant=api.FAR_HORNER):

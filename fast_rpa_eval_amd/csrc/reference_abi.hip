@@ -50,9 +50,22 @@ void run_or_die(const char *where, F &&f) {
     die(where);
 }
 
+// FRPA_RHO1_PRECISION: fp64 (default), ozaki (INT8 split, 5 digits), ozaki4 (4 digits).
+// FRPA_RHO1_GRADIENT=0: first_order_gradient_rho is not computed (left zero; LDA callers).  The
+// GGA gradient needs W in FP64, so with the gradient requested the FP64 kernel runs whatever the
+// precision.
 int rho1_precision_from_env() {
   const char *p = getenv("FRPA_RHO1_PRECISION");
-  return (p && strcmp(p, "ozaki") == 0) ? FRPA_PREC_OZAKI : FRPA_PREC_FP64;
+  if (p && strcmp(p, "ozaki") == 0)
+    return FRPA_PREC_OZAKI;
+  if (p && strcmp(p, "ozaki4") == 0)
+    return FRPA_PREC_OZAKI4;
+  return FRPA_PREC_FP64;
+}
+
+bool rho1_gradient_from_env() {
+  const char *g = getenv("FRPA_RHO1_GRADIENT");
+  return !(g && strcmp(g, "0") == 0);
 }
 
 } // namespace
@@ -189,10 +202,14 @@ void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
     FRPA_HIP(hipMemcpyAsync(S.bias.ptr, first_order_rho_bias_part2_ptr, nbt * 8, hipMemcpyHostToDevice, s));
     FRPA_HIP(hipMemcpyAsync(S.gbias.ptr, first_order_gradient_rho_bias_batches_atoms_ptr, 3 * nbt * 8,
                             hipMemcpyHostToDevice, s));
-    const int rc = frpa_rho1_forward(reinterpret_cast<frpa_rho1_plan *>(S.rho1.get()), S.T, n_tile,
-                                     rho1_precision_from_env(), S.wave.ptr, S.P1.ptr, S.bias.ptr, S.rho.ptr,
-                                     S.gwave.ptr, S.gbias.ptr, S.grad.ptr, s);
+    const bool grad = rho1_gradient_from_env();
+    const int prec = grad ? FRPA_PREC_FP64 : rho1_precision_from_env();
+    const int rc = frpa_rho1_forward(reinterpret_cast<frpa_rho1_plan *>(S.rho1.get()), S.T, n_tile, prec,
+                                     S.wave.ptr, S.P1.ptr, S.bias.ptr, S.rho.ptr, grad ? S.gwave.ptr : nullptr,
+                                     grad ? S.gbias.ptr : nullptr, grad ? S.grad.ptr : nullptr, s);
     FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
+    if (!grad)  // evaluate_first_order_rho.cpp:113-114 zeroes both outputs
+      fill_f64(S.grad.ptr, 3 * nbt, 0.0, s);
     FRPA_HIP(hipMemcpyAsync(first_order_rho_ptr, S.rho.ptr, S.rho.n * 8, hipMemcpyDeviceToHost, s));
     FRPA_HIP(hipMemcpyAsync(first_order_gradient_rho_ptr, S.grad.ptr, 3 * nbt * 8, hipMemcpyDeviceToHost, s));
     FRPA_HIP(hipStreamSynchronize(s));

@@ -225,7 +225,8 @@ void free_dfpt_device_data(void);
  * batch data of init_dfpt_device_data; uploads first_order_density_matrix and the bias arrays from
  * the given host pointers and downloads first_order_rho and first_order_gradient_rho into them
  * (the reference's non-ALL_KERNELS_ON_GPU mode, evaluate_first_order_rho.cu:240-244,362-366).
- * Precision: FP64, or the INT8 split when FRPA_RHO1_PRECISION=ozaki. */
+ * first_order_gradient_rho is computed (GGA, FP64) unless FRPA_RHO1_GRADIENT=0 (then left zero);
+ * without it FRPA_RHO1_PRECISION=ozaki / ozaki4 selects the INT8 split for first_order_rho. */
 void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
     int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
     int *n_basis_, int *n_atoms_, int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_,

@@ -38,11 +38,13 @@ def lib():
 
 
 def _p(a, dtype):
-    """Pointer to a contiguous copy-free view (Fortran arrays are flattened in memory order)."""
+    """Pointer to the memory of a copy-free view; arrays of two or more dimensions must be in
+    Fortran order (the C restatement indexes them column-major; a C-order array would be read
+    transposed and outputs written to the wrong elements)."""
     a = np.asarray(a)
     assert a.dtype == dtype, (a.dtype, dtype)
-    if not (a.flags.f_contiguous or a.flags.c_contiguous):
-        raise ValueError("array must be contiguous")
+    if not a.flags.f_contiguous:
+        raise ValueError(f"array of shape {a.shape} must be Fortran-contiguous")
     return a.ctypes.data_as(ctypes.c_void_p)
 
 

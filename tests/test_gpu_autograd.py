@@ -67,3 +67,63 @@ def test_far_grad_matches_fd_and_closed_form(cuda):
         for l in range(7):
             ref = (g[:, None] * Y[:, l * l:(l + 1) ** 2] / r[:, None] ** (l + 1)).sum(axis=0)
             np.testing.assert_allclose(Gn[l * l:(l + 1) ** 2, a, 0], ref, rtol=1e-10, atol=1e-14)
+
+
+def test_adjoint_check_config_128_atoms(cuda, oracle):
+    """The adjoint-check configuration at its own size: 128 atoms (1792 basis functions), 2000 batches
+    of 100-150 points with natural compute sets, v1 ~ N(0, 1); d/dP1 against FP64 central differences
+    (h = 1e-6) and the oracle H integration, in FP64 and in the INT8 split; far field l = 6 over
+    50k points, d/dM against central differences and the closed form Y_lm / r^(l+1)."""
+    prob = synth.make_rho1_problem(128, 20.0, 2000, (100, 150), None, config_k=4)
+    plan = api.Rho1Plan.from_problem(prob)
+    wave = api.to_device(prob.wave)
+    rng = np.random.default_rng(54)
+    v1 = rng.standard_normal(prob.n_full_points)
+    vd = api.to_device(v1)
+    from tests.test_gpu_adjoint import _oracle_h1
+    ref = _oracle_h1(oracle, prob, np.asfortranarray(v1[:, None]), 1)[:, :, 0]
+    P0 = api.to_device(synth.make_p1(prob.atom_xyz, prob.basis, 3004))
+    dP = api.to_device(rng.standard_normal(prob.n_basis * prob.n_basis))
+    for prec, tol in ((api.FP64, 1e-12), (api.OZAKI, 1e-10)):
+        P = P0.clone().requires_grad_(True)
+        L = torch.dot(vd, autograd.rho1(P, plan, wave, precision=prec, adjoint_precision=prec))
+        L.backward()
+        G = P.grad.detach()
+        got = G.cpu().numpy().reshape((prob.n_basis, prob.n_basis), order="F")
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        h = 1e-6
+        with torch.no_grad():
+            fd = float((torch.dot(vd, plan.forward(wave, P0 + h * dP)) -
+                        torch.dot(vd, plan.forward(wave, P0 - h * dP))) / (2 * h))
+        an = float(torch.dot(G, dP))
+        print(f"[adjoint check cfg, precision {prec}] H1 vs oracle {err:.3e}; fd {fd:.10e} analytic {an:.10e}",
+              flush=True)
+        assert err <= tol
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
+    fp = synth.make_far_problem(128, 20.0, 50000, 6, config_k=4, all_far=True, xyz=prob.atom_xyz)
+    fplan = api.FarPlan.from_problem(fp)
+    inp = api.FarInputs(fp)
+    M = inp.moments.clone().requires_grad_(True)
+    gV = api.to_device(rng.standard_normal(fp.n_full_points))
+    Lf = torch.dot(gV, autograd.far_field(M, fplan, inp))
+    Lf.backward()
+    GM = M.grad.detach()
+    dM = api.to_device(rng.standard_normal(M.numel()))
+    h = 1e-6
+    with torch.no_grad():
+        fd = float((torch.dot(gV, autograd.far_field(M.detach() + h * dM, fplan, inp)) -
+                    torch.dot(gV, autograd.far_field(M.detach() - h * dM, fplan, inp))) / (2 * h))
+    an = float(torch.dot(GM, dM))
+    print(f"[adjoint check cfg, far field] fd {fd:.10e} analytic {an:.10e}", flush=True)
+    assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
+    mp = fp.i_valid_point_2_i_full_points_map
+    pts = fp.coord_points[:, mp].T
+    g = gV.cpu().numpy()[mp]
+    Gn = GM.cpu().numpy().reshape(fp.multipole_moments.shape, order="F")
+    for a in (0, 77):
+        dvec = pts - fp.coords_center[:, a]
+        r = np.linalg.norm(dvec, axis=1)
+        Y = mt.real_sph_harm(6, dvec / r[:, None])
+        for l in range(7):
+            refm = (g[:, None] * Y[:, l * l:(l + 1) ** 2] / r[:, None] ** (l + 1)).sum(axis=0)
+            np.testing.assert_allclose(Gn[l * l:(l + 1) ** 2, a, 0], refm, rtol=1e-10, atol=1e-14)

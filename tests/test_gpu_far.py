@@ -51,6 +51,20 @@ def test_far_l_reduction_near_pairs_and_tile(cuda, oracle, variant):
     _check(got, ref, f"far l-reduction variant={variant}")
 
 
+@pytest.mark.parametrize("variant", [api.FAR_TERMS, api.FAR_HORNER])
+def test_far_full_tile_two_perturbations(cuda, oracle, variant):
+    """T = 2 with both perturbations in the tile (n_tile = 2), l reduction and near pairs, accumulated
+    into a nonzero delta_v_hartree."""
+    fp = synth.make_far_problem(10, 11.0, 4000, 6, config_k=4, all_far=False, n_tile=2, outer_radius="decreasing")
+    dv0 = np.asfortranarray(np.random.default_rng(3).uniform(-1, 1, size=(fp.n_full_points, 2)))
+    ref = oracle.far(fp, delta_v=dv0.copy(order="F"), j_atom_begin=1, j_atom_end=2)
+    got = _gpu_far(fp, variant, dv0=dv0, n_tile=2)
+    for t in range(2):
+        d = np.abs(got[:, t] - ref[:, t]).max() / np.abs(ref[:, t] - dv0[:, t]).max()
+        print(f"[far full tile variant={variant}] perturbation {t}: rel {d:.3e}", flush=True)
+        assert d <= 1e-12
+
+
 def test_far_l8_many_atoms(cuda, oracle):
     fp = synth.make_far_problem(128, 25.0, 4000, 8, config_k=5, all_far=False)
     ref = oracle.far(fp)

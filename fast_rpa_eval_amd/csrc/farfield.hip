@@ -295,14 +295,26 @@ __device__ __forceinline__ DV<N> huv(const double *__restrict__ c, const DV<N> &
     return fmav(huv<N, L, Q + 1>(c, X, Y, Z, u), u, hqv<N, L, Q>(c, X, Y, Z));
 }
 
+// 1 / sqrt(x) from v_rsq_f64 and two Newton steps r += r (1 - x r^2) / 2 (quadratic convergence:
+// within a few ulp of the correctly rounded value, for the x > 0 of far pairs)
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double e = fma(-x * r, r, 1.0);
+    r = fma(0.5 * r, e, r);
+  }
+  return r;
+}
+
 template <int N, int L>
 __device__ __forceinline__ DV<N> pair_horner_v(const double *__restrict__ c, const DV<N> &mx, const DV<N> &my,
                                                const DV<N> &mz, const DV<N> &d2) {
   DV<N> u, X, Y, Z, rinv;
 #pragma unroll
   for (int q = 0; q < N; q++) {
-    u.v[q] = 1.0 / d2.v[q];
-    rinv.v[q] = sqrt(u.v[q]);
+    rinv.v[q] = rsqrt_f64(d2.v[q]);
+    u.v[q] = rinv.v[q] * rinv.v[q];
     X.v[q] = mx.v[q] * u.v[q];
     Y.v[q] = my.v[q] * u.v[q];
     Z.v[q] = mz.v[q] * u.v[q];
@@ -346,6 +358,7 @@ __global__ __launch_bounds__(256) void far_horner_kernel(
     const CenterRec c = centers[ic];
     const double *op = opr + (size_t)lp1 * (c.atom + (size_t)n_atoms * t);
     const double rsq = mrsq[c.atom + (size_t)n_atoms * t];
+    const double op_top = op[c.l_const];  // wave-uniform: one scalar load per centre
     DV<N> d2;
     int l[N];
     bool far[N];
@@ -354,9 +367,13 @@ __global__ __launch_bounds__(256) void far_horner_kernel(
     for (int q = 0; q < N; q++) {
       const double ax = rx[q] - c.dx, ay = ry[q] - c.dy, az = rz[q] - c.dz;
       d2.v[q] = ax * ax + ay * ay + az * az;
+      // sum_up_whole_potential.cpp:404-407: l = l_hartree; while (opr(l) < d^2 && l > 0) l--
       l[q] = c.l_const;
-      while (op[l[q]] < d2.v[q] && l[q] > 0)
+      if (op_top < d2.v[q] && l[q] > 0) {
         l[q]--;
+        while (op[l[q]] < d2.v[q] && l[q] > 0)
+          l[q]--;
+      }
       far[q] = active[q] && !(d2.v[q] < rsq) && d2.v[q] < 1e8;
       pend[q] = __ballot(far[q]);
       any |= pend[q];

@@ -29,8 +29,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 struct AdjOzState {
   int S = 0;
   const double *wave = nullptr;  // the wave the phi digits were made from
-  int npk_max = 0;               // max over batches of np rounded to 64
-  long long per_t = 0;           // bytes of the phi digits
+  long long bytes = 0;           // bytes of the phi digits
   DevBuf<long long> d_off;       // per BatchDesc: byte offset of its digit planes
   DevBuf<signed char> d_fdig;    // phi digits (per wave)
   DevBuf<double> d_finv;         // 1 / sf per sorted row (compact, at ib_off)
@@ -46,80 +45,60 @@ constexpr int PMAX = 256;  // points held in LDS per pass (larger batches take s
 
 __device__ __forceinline__ int ceil64(int x) { return (x + 63) & ~63; }
 
-// Digit planes of one batch: rows in sorted order; WITH_G multiplies by g(map(p), t).  Threads run
-// over the caller's rows (coalesced reads down each point's column, twice: row maximum, then
+// Phi digit planes of one batch (once per wave): rows in sorted order, one scale per row.  Threads
+// run over the caller's rows (coalesced reads down each point's column, twice: row maximum, then
 // digits) and write each row's digits at its sorted position spos[r]; positions nc .. nc64-1 are
 // zero rows.
-template <int S, bool WITH_G>
+template <int S>
 __global__ __launch_bounds__(256) void h1_digits_kernel(const BatchDesc *__restrict__ descs,
                                                         const long long *__restrict__ off,
                                                         const int *__restrict__ spos_all,
-                                                        const int *__restrict__ map, const double *__restrict__ wave,
-                                                        const double *__restrict__ g, int n_full, long long per_t,
-                                                        size_t inv_per_t, signed char *__restrict__ dig,
-                                                        double *__restrict__ inv) {
-  __shared__ double gs[64];
+                                                        const double *__restrict__ wave,
+                                                        signed char *__restrict__ dig, double *__restrict__ inv) {
   const BatchDesc d = descs[blockIdx.x];
-  const int t = blockIdx.y;
   const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
   const int nc64 = ceil64(nc), npk = ceil64(np);
   const size_t plane = (size_t)nc64 * npk;
   const double *W = wave + d.wave_off;
   const int *spos = spos_all + d.ib_off;
-  const double *gt = WITH_G ? g + (size_t)n_full * t : nullptr;
-  signed char *out = dig + off[blockIdx.x] + (WITH_G ? per_t * t : 0);
-  double *iv = inv + d.ib_off + (WITH_G ? inv_per_t * t : 0);
-  auto load_g = [&](int p0) {
-    __syncthreads();
-    if (threadIdx.x < 64)
-      gs[threadIdx.x] = (WITH_G && p0 + (int)threadIdx.x < np) ? gt[map[d.pstart + p0 + threadIdx.x]] : 1.0;
-    __syncthreads();
-  };
+  signed char *out = dig + off[blockIdx.x];
+  double *iv = inv + d.ib_off;
   for (int r0 = 0; r0 < nc64; r0 += 256) {
     const int r = r0 + threadIdx.x;
+    if (r >= nc64)
+      continue;
     const bool real = r < nc;
     const double *Wr = W + (real ? r : 0);
     double mx = 0.0;
-    for (int p0 = 0; p0 < np; p0 += 64) {
-      load_g(p0);
-      if (real) {
-        const int pe = min(64, np - p0);
-        for (int j = 0; j < pe; j++)
-          mx = fmax(mx, fabs(Wr[(size_t)ld * (p0 + j)] * gs[j]));
-      }
-    }
+    if (real)
+      for (int p = 0; p < np; p++)
+        mx = fmax(mx, fabs(Wr[(size_t)ld * p]));
     const double sc = mx > 0.0 ? 127.0 / mx : 1.0;
     const int row = real ? spos[r] : r;  // padding positions keep their own index
     if (real)
       iv[row] = 1.0 / sc;
-    for (int p0 = 0; p0 < npk; p0 += 64) {
-      load_g(p0);
-      if (r >= nc64)
-        continue;
-#pragma unroll 1
-      for (int q = 0; q < 4; q++) {
-        unsigned int w4[S][4];
+    for (int p0 = 0; p0 < npk; p0 += 16) {
+      unsigned int w4[S][4];
 #pragma unroll
-        for (int sl = 0; sl < S; sl++)
+      for (int sl = 0; sl < S; sl++)
 #pragma unroll
-          for (int u = 0; u < 4; u++)
-            w4[sl][u] = 0u;
+        for (int u = 0; u < 4; u++)
+          w4[sl][u] = 0u;
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-          const int p = p0 + 16 * q + j;
-          double x = (real && p < np) ? Wr[(size_t)ld * p] * gs[16 * q + j] * sc : 0.0;
+      for (int j = 0; j < 16; j++) {
+        const int p = p0 + j;
+        double x = (real && p < np) ? Wr[(size_t)ld * p] * sc : 0.0;
 #pragma unroll
-          for (int sl = 0; sl < S; sl++) {
-            const double dg = rint(x);
-            w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
-            x = (x - dg) * RADIX;
-          }
+        for (int sl = 0; sl < S; sl++) {
+          const double dg = rint(x);
+          w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
+          x = (x - dg) * RADIX;
         }
-#pragma unroll
-        for (int sl = 0; sl < S; sl++)
-          *reinterpret_cast<v4i *>(out + sl * plane + (size_t)npk * row + p0 + 16 * q) =
-              v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
       }
+#pragma unroll
+      for (int sl = 0; sl < S; sl++)
+        *reinterpret_cast<v4i *>(out + sl * plane + (size_t)npk * row + p0) =
+            v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
     }
   }
 }
@@ -317,21 +296,17 @@ template <int S>
 void prepare_f(Rho1Plan &P, AdjOzState &st, const double *wave, hipStream_t s) {
   const Relabel &R = get_relabel(P);
   std::vector<long long> off(P.n_valid + 1, 0);
-  int npk_max = 0;
   for (int i = 0; i < P.n_valid; i++) {
     const BatchDesc &d = P.h_desc[i];
     const int nc64 = (d.nc + 63) / 64 * 64, npk = (d.np + 63) / 64 * 64;
     off[i + 1] = off[i] + (long long)S * nc64 * npk;
-    npk_max = std::max(npk_max, npk);
   }
   st.d_off.upload(off);
-  st.per_t = std::max<long long>(off[P.n_valid], 16);
-  st.npk_max = npk_max;
-  st.d_fdig.alloc(st.per_t);
+  st.bytes = std::max<long long>(off[P.n_valid], 16);
+  st.d_fdig.alloc(st.bytes);
   st.d_finv.alloc(std::max<size_t>(P.h_ibasis0.size(), 1));
-  hipLaunchKernelGGL((h1_digits_kernel<S, false>), dim3(P.n_valid, 1), dim3(256), 0, s, P.d_desc.ptr, st.d_off.ptr,
-                     R.d_spos.ptr, P.d_map.ptr, wave, nullptr, P.n_full, st.per_t, (size_t)0, st.d_fdig.ptr,
-                     st.d_finv.ptr);
+  hipLaunchKernelGGL((h1_digits_kernel<S>), dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, st.d_off.ptr,
+                     R.d_spos.ptr, wave, st.d_fdig.ptr, st.d_finv.ptr);
   check_launch("h1_digits_kernel");
   st.S = S;
   st.wave = wave;

@@ -16,6 +16,7 @@ sample of the same inputs).
     python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 """
 import argparse
+import copy
 import json
 import os
 import sys
@@ -87,6 +88,73 @@ def make_inputs(w):
         f"({time.time() - t0:.1f}s)")
     torch.cuda.synchronize()
     return dict(prob=prob, plan=plan, wave=wave, P1=P1, g=gfull, fp=fp, fplan=fplan, finp=finp)
+
+
+NEAR_GRID = 200        # log-grid points per species of the near-field spline tile
+NEAR_RADIUS = 8.0      # multipole radius (bohr): pairs closer than this take the spline branch
+
+
+def make_near_inputs(D, w):
+    """Near-field branch of sum_up_whole_potential over the rho1 integration grid (its batch-ordered
+    points), two species with their own log grids, l_hartree = l_far / l_far - 2, smooth radial tiles."""
+    from fast_rpa_eval_amd import api, synth
+    t0 = time.time()
+    prob = D["prob"]
+    fpn = synth.make_far_problem(w["n_atoms"], w["box_ang"], 0, w["l_far"], config_k=w["config_k"],
+                                 xyz=prob.atom_xyz, min_dist_bohr=NEAR_RADIUS,
+                                 grid=(prob.coord_points, prob.valid_map()))
+    synth.add_near_field(fpn, NEAR_GRID, "smooth", n_species=2, seed=w["config_k"])
+    plan = api.FarPlan.from_problem(fpn)
+    inp = api.FarInputs(fpn)
+    log(f"near-field inputs ready: {fpn.n_valid_points} points, tile "
+        f"{fpn.near['spl'].nbytes / 1e9:.2f} GB ({time.time() - t0:.1f}s)")
+    return dict(fp=fpn, plan=plan, inp=inp)
+
+
+def near_pairs(fpn):
+    """(number of near pairs, sum over them of (l+1)^2) for the near-field inputs (device count)."""
+    import torch
+    pts = torch.from_numpy(fpn.coord_points[:, fpn.i_valid_point_2_i_full_points_map].T.copy()).cuda()
+    cen = torch.from_numpy(fpn.coords_center.T.copy()).cuda()
+    lh = torch.from_numpy(fpn.l_hartree[fpn.species - 1].astype(np.int64)).cuda()
+    opr = torch.from_numpy(np.ascontiguousarray(fpn.outer_potential_radius[:, :, 0].T)).cuda()  # (atoms, l)
+    mr = torch.from_numpy(fpn.multipole_radius_sq[:, 0].copy()).cuda()
+    n, nlm = 0, 0.0
+    for s in range(0, pts.shape[0], 200000):
+        d2 = torch.cdist(pts[s:s + 200000], cen) ** 2
+        near = d2 < mr[None, :]
+        l = lh[None, :].expand_as(d2).clone()
+        for _ in range(int(lh.max())):
+            low = (torch.gather(opr, 1, l.T).T < d2) & (l > 0)
+            l = l - low.long()
+        n += int(near.sum())
+        nlm += float((((l + 1) ** 2) * near).sum())
+    return n, nlm
+
+
+def time_near(N, steps, warmup):
+    """Near-field launch timed with HIP events on torch's current stream (the stream it runs on)."""
+    import torch
+    fpn = N["fp"]
+    dv = torch.zeros(fpn.n_full_points, dtype=torch.float64, device="cuda")
+
+    def run():
+        dv.zero_()
+        N["plan"].near_forward(dv, *N["inp"].near_args())
+    for _ in range(max(warmup, 1)):
+        run()
+    torch.cuda.synchronize()
+    t = []
+    for _ in range(steps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        dv.zero_()
+        a.record()
+        N["plan"].near_forward(dv, *N["inp"].near_args())
+        b.record()
+        t.append((a, b))
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in t])), dv
 
 
 def step_fns(D, prec, adj_prec, far_variant, have_adj):
@@ -379,6 +447,9 @@ def main():
     ap.add_argument("--graph", type=int, default=1,
                     help="1: the timed step is one replay of the step captured in a HIP graph (per-kernel "
                          "times from an eager pass); 0: eager launches")
+    ap.add_argument("--near", type=int, default=1,
+                    help="1: also time the near-field (spline) branch of the Hartree sum over the rho1 grid "
+                         "(reported as near_field, outside the step)")
     ap.add_argument("--kernels", default="rho1,adjoint,far",
                     help="subset of the step to time (a partial step is not a bench line)")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -546,6 +617,47 @@ def main():
     dom = max(fast["per"], key=fast["per"].get)
     line["roofline"] = rl[dom]
     line["roofline_all"] = rl
+
+    if a.near:
+        N = make_near_inputs(D, w)
+        ms, dvn = time_near(N, a.steps, a.warmup)
+        fpn = N["fp"]
+        npairs, nlm = near_pairs(fpn)
+        L = w["l_far"]
+        # algorithmic work per near pair: 4 FMAs per lm for the Hermite dot products (8 flop), ~3 flop per
+        # lm for Y_lm by its recurrence, ~40 for distance, log and weights; (l+1)^2 per pair after the
+        # outer-radius reduction
+        flops = 11.0 * nlm + 40.0 * npairs
+        coef_bytes = 32.0 * nlm
+        ach = flops / (ms / 1e3) / 1e12
+        near = {"workload": f"near field over the rho1 grid: {fpn.n_valid_points} points, {fpn.n_atoms} centres, "
+                            f"l_hartree {L}/{max(L - 2, 0)} (2 species), multipole radius {NEAR_RADIUS} bohr, "
+                            f"{NEAR_GRID}-point log grids",
+                "ms": ms, "points_per_s": fpn.n_valid_points / (ms / 1e3), "near_pairs": npairs,
+                "pairs_per_s": npairs / (ms / 1e3),
+                "roofline": {"kernel": "frpa::near_kernel", "bound": "mfma", "achieved": ach, "peak": PEAK_FP64,
+                             "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "flop_per_pair_lm": 11,
+                             "coef_bytes_per_s": coef_bytes / (ms / 1e3),
+                             "note": "FP64 VALU; coefficient gathers (32 B per lm per pair) are served by L2/MALL"}}
+        if not a.no_cpu:
+            from oracle import oracle
+            sel = np.sort(np.random.default_rng(11).choice(fpn.n_valid_points, size=min(20000, fpn.n_valid_points),
+                                                           replace=False))
+            sub = copy.copy(fpn)
+            sub.i_valid_point_2_i_full_points_map = fpn.i_valid_point_2_i_full_points_map[sel]
+            sub.n_valid_points = len(sel)
+            oracle.set_threads(a.cpu_threads)
+            t0 = time.time()
+            ref = oracle.near(sub)[:, 0]
+            cpu_s = time.time() - t0
+            mp = sub.i_valid_point_2_i_full_points_map
+            got = dvn.cpu().numpy()[mp]
+            near["vs_oracle_rel"] = float(np.abs(got - ref[mp]).max() / np.abs(ref[mp]).max())
+            near["cpu_baseline"] = {"value": len(sel) / cpu_s, "unit": "points/s", "cores": a.cpu_threads,
+                                    "kind": "port", "sample": f"{len(sel)} random points of the grid"}
+        line["near_field"] = near
+        log(f"near field: {ms:.3f} ms, {npairs} pairs, {near['pairs_per_s'] / 1e9:.2f} G pairs/s, "
+            f"{ach:.1f} TF/s ({100 * ach / PEAK_FP64:.0f}% FP64)")
 
     if not a.no_cpu:
         cb = cpu_baseline(D, a.cpu_threads)

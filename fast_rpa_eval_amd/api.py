@@ -171,9 +171,32 @@ class FarPlan:
     @classmethod
     def from_problem(cls, fp):
         t = fp.tables
-        return cls(fp.l_pot_max, fp.l_max_analytic_multipole, t["index_cc"], t["cc"], t["n_cc_lm_ijk"],
+        plan = cls(fp.l_pot_max, fp.l_max_analytic_multipole, t["index_cc"], t["cc"], t["n_cc_lm_ijk"],
                    t["index_ijk_max_cc"], fp.species, fp.l_hartree, fp.centers_hartree_potential, fp.center_to_atom,
                    fp.coords_center, fp.hartree_force_l_add)
+        if fp.near is not None:
+            nf = fp.near
+            plan.set_near(nf["species_center"], nf["n_grid"], nf["r_grid_min"], nf["log_r_grid_inc"])
+        return plan
+
+    def set_near(self, species_center, n_grid, r_grid_min, log_r_grid_inc):
+        """Near-field log grids: species_center (n_centers) 1-based, per-species n_grid, r_grid_min,
+        log_r_grid_inc (host arrays, copied)."""
+        a = [np.ascontiguousarray(species_center, np.int32), np.ascontiguousarray(n_grid, np.int32),
+             np.ascontiguousarray(r_grid_min, np.float64), np.ascontiguousarray(log_r_grid_inc, np.float64)]
+        check(lib().frpa_far_plan_set_near(self.handle, len(a[1]), _hp(a[0], np.int32), _hp(a[1], np.int32),
+                                           _hp(a[2], np.float64), _hp(a[3], np.float64)), "frpa_far_plan_set_near")
+
+    def near_forward(self, delta_v_hartree, coord_points, i_valid_point_2_i_full_points_map, multipole_radius_sq,
+                     outer_potential_radius, spl_tile, n_full_points, n_hartree_grid, T=1, n_tile=None, stream=None):
+        """delta_v_hartree (n_full_points * T,) += near-field spline sum (in place; returned).
+        spl_tile: current_delta_v_hart_part_spl_tile ((l_pot_max+1)^2, 2, n_hartree_grid, n_atoms, T)."""
+        n_tile = T if n_tile is None else n_tile
+        check(lib().frpa_near_forward(
+            self.handle, T, n_tile, n_full_points, i_valid_point_2_i_full_points_map.numel(), n_hartree_grid,
+            _dp(coord_points), _dp(i_valid_point_2_i_full_points_map), _dp(multipole_radius_sq),
+            _dp(outer_potential_radius), _dp(spl_tile), _dp(delta_v_hartree), _stream(stream)), "frpa_near_forward")
+        return delta_v_hartree
 
     def __del__(self):
         if getattr(self, "handle", None) and lib is not None:
@@ -221,6 +244,11 @@ class FarInputs:
         self.mrsq = to_device(fp.multipole_radius_sq)
         self.opr = to_device(fp.outer_potential_radius)
         self.moments = to_device(fp.multipole_moments)
+        self.spl = to_device(fp.near["spl"]) if fp.near is not None else None
+        self.n_hartree_grid = fp.near["n_hartree_grid"] if fp.near is not None else 0
 
     def args(self):
         return (self.coord_points, self.map, self.lmfd, self.mrsq, self.opr, self.moments, self.n_full_points)
+
+    def near_args(self):
+        return (self.coord_points, self.map, self.mrsq, self.opr, self.spl, self.n_full_points, self.n_hartree_grid)

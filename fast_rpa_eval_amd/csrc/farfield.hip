@@ -297,16 +297,6 @@ __device__ __forceinline__ DV<N> huv(const double *__restrict__ c, const DV<N> &
 
 // 1 / sqrt(x) from v_rsq_f64 and two Newton steps r += r (1 - x r^2) / 2 (quadratic convergence:
 // within a few ulp of the correctly rounded value, for the x > 0 of far pairs)
-__device__ __forceinline__ double rsqrt_f64(double x) {
-  double r = __builtin_amdgcn_rsq(x);
-#pragma unroll
-  for (int it = 0; it < 2; it++) {
-    const double e = fma(-x * r, r, 1.0);
-    r = fma(0.5 * r, e, r);
-  }
-  return r;
-}
-
 template <int N, int L>
 __device__ __forceinline__ DV<N> pair_horner_v(const double *__restrict__ c, const DV<N> &mx, const DV<N> &my,
                                                const DV<N> &mz, const DV<N> &d2) {

@@ -19,9 +19,28 @@ struct CenterRec {
   int atom;           // center_to_atom(centers_hartree_potential(ic) - 1) - 1
   int mc_atom;        // center_to_atom(ic) - 1: row of multipole_c used by the term sum
   int l_const;        // l_hartree(species(atom) - 1)
-  int pad;
+  int spc;            // species_center(centers_hartree_potential(ic)) - 1: log grid of the near field
 };
 static_assert(sizeof(CenterRec) == 64, "CenterRec layout");
+
+// Log grid of one species for the near-field spline: i_r_log = 1 + (log r - log_rmin) * inv_inc.
+struct NearSpecies {
+  double log_rmin;  // log(r_grid_min)
+  double inv_inc;   // 1 / log_r_grid_inc
+  int n_points;     // n_grid: the spline interval index is clamped to [1, n_points - 1]
+  int pad;
+};
+
+// 1 / sqrt(x) in FP64: hardware estimate refined by two Newton steps (error ~1 ulp).
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double e = fma(-x * r, r, 1.0);
+    r = fma(0.5 * r, e, r);
+  }
+  return r;
+}
 
 // Number of merged (i, j, k, q) coefficients of the nested-Horner form for one l:
 // sum_{q=0}^{l/2} (l - 2q + 1)^2.
@@ -66,6 +85,13 @@ struct FarPlan {
   DevBuf<int> d_hc_ptr, d_hc_term;
   DevBuf<double> d_hc_w;    // (-1)^nn (2nn-1)!! cc(n) per listed term
   DevBuf<int> d_atom_ic_ptr, d_atom_ic;  // hartree centres ic whose multipole_c row is atom a
+  // near field: per-species log grids (set_near_tables); centre records carry the species
+  std::vector<CenterRec> h_centers;
+  std::vector<int> h_chp;   // centers_hartree_potential (1-based)
+  bool near_ok = false;
+  int near_max_points = 0;  // largest n_grid over the species used by the centres
+  DevBuf<NearSpecies> d_nspec;
+  std::vector<NearSpecies> h_nspec;
   // per-call scratch
   DevBuf<double> d_mc;      // (mc_dim0, n_atoms, T) reference multipole_c
   DevBuf<double> d_coef;    // (coef_stride, n_atoms, T)
@@ -77,6 +103,9 @@ struct FarCall {
   const double *coord_points = nullptr;
   const int *map = nullptr, *lmfd = nullptr;
   const double *mrsq = nullptr, *opr = nullptr, *moments = nullptr;
+  // near field: current_delta_v_hart_part_spl_tile (lm, 2, n_hartree_grid, n_atoms, T)
+  const double *spl = nullptr;
+  int n_hgrid = 0;
 };
 
 FarPlan *make_far_plan(int l_pot_max, int lama, int dim0, int n_centers, int n_hc, int n_atoms, int n_species,
@@ -85,5 +114,8 @@ FarPlan *make_far_plan(int l_pot_max, int lama, int dim0, int n_centers, int n_h
                        const int *index_ijk_max_cc);
 void far_forward(FarPlan &P, const FarCall &c, int variant, double *dv, hipStream_t s);
 void far_adjoint(FarPlan &P, const FarCall &c, const double *gV, double *gM, hipStream_t s);
+void set_near_tables(FarPlan &P, int n_species, const int *species_center, const int *n_grid,
+                     const double *r_grid_min, const double *log_r_grid_inc);
+void near_forward(FarPlan &P, const FarCall &c, double *dv, hipStream_t s);
 
 } // namespace frpa

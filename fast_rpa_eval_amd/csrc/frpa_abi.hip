@@ -272,7 +272,7 @@ FarPlan *make_far_plan(int l_pot_max, int lama, int dim0, int n_centers, int n_h
     FRPA_REQUIRE(sp >= 0 && sp < n_species, FRPA_ERR_INVALID, "species(%d) out of range", atom + 1);
     cen[ic] = CenterRec{coords_center[3 * cur], coords_center[3 * cur + 1], coords_center[3 * cur + 2],
                         coords_center[3 * ic], coords_center[3 * ic + 1], coords_center[3 * ic + 2],
-                        atom, mca, l_hartree[sp], 0};
+                        atom, mca, l_hartree[sp], -1};
     atom_ic[mca].push_back(ic);
   }
   // merged nested-Horner gather plan: for each l and each (i, j, k, q) the terms n < n_cc(l) feeding it
@@ -308,6 +308,8 @@ FarPlan *make_far_plan(int l_pot_max, int lama, int dim0, int n_centers, int n_h
     aip.push_back((int)ai.size());
   }
   P->d_centers.upload(cen);
+  P->h_centers = cen;
+  P->h_chp.assign(chp, chp + n_hc);
   P->d_index_cc.upload(index_cc, (size_t)dim0 * 6);
   std::vector<double> ccp(std::max<size_t>(dim0, FAR_NCC[FAR_LMAX]), 0.0);
   std::copy(cc, cc + dim0, ccp.begin());
@@ -577,6 +579,43 @@ int frpa_far_adjoint(frpa_far_plan *plan, int atom_tile_size, int n_tile, int n_
                          i_valid_point_2_i_full_points_map, l_hartree_max_far_distance, multipole_radius_sq,
                          outer_potential_radius, multipole_moments);
     far_adjoint(*reinterpret_cast<FarPlan *>(plan), c, gV, grad_moments, as_stream(stream));
+  });
+}
+
+int frpa_far_plan_set_near(frpa_far_plan *plan, int n_species, const int *species_center, const int *n_grid,
+                           const double *r_grid_min, const double *log_r_grid_inc) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
+    set_near_tables(*reinterpret_cast<FarPlan *>(plan), n_species, species_center, n_grid, r_grid_min,
+                    log_r_grid_inc);
+  });
+}
+
+int frpa_near_forward(frpa_far_plan *plan, int atom_tile_size, int n_tile, int n_full_points, int n_valid_points,
+                      int n_hartree_grid, const double *coord_points, const int *i_valid_point_2_i_full_points_map,
+                      const double *multipole_radius_sq, const double *outer_potential_radius,
+                      const double *current_delta_v_hart_part_spl_tile, double *delta_v_hartree, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && delta_v_hartree, FRPA_ERR_INVALID, "null argument");
+    FRPA_REQUIRE(atom_tile_size >= 1 && n_tile >= 0 && n_tile <= atom_tile_size && n_full_points >= 0 &&
+                     n_valid_points >= 0 && n_hartree_grid >= 0,
+                 FRPA_ERR_INVALID, "near: bad sizes (T %d, n_tile %d, n_full %d, n_valid %d, n_hartree_grid %d)",
+                 atom_tile_size, n_tile, n_full_points, n_valid_points, n_hartree_grid);
+    FRPA_REQUIRE(n_valid_points == 0 || (coord_points && i_valid_point_2_i_full_points_map && multipole_radius_sq &&
+                                         outer_potential_radius && current_delta_v_hart_part_spl_tile),
+                 FRPA_ERR_INVALID, "near: null array");
+    FarCall c;
+    c.T = atom_tile_size;
+    c.n_tile = n_tile;
+    c.n_full = n_full_points;
+    c.n_valid = n_valid_points;
+    c.coord_points = coord_points;
+    c.map = i_valid_point_2_i_full_points_map;
+    c.mrsq = multipole_radius_sq;
+    c.opr = outer_potential_radius;
+    c.spl = current_delta_v_hart_part_spl_tile;
+    c.n_hgrid = n_hartree_grid;
+    near_forward(*reinterpret_cast<FarPlan *>(plan), c, delta_v_hartree, as_stream(stream));
   });
 }
 

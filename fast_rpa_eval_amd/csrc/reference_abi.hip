@@ -30,7 +30,7 @@ struct RefState {
   DevBuf<double> wave, gwave, P1, bias, gbias, rho, grad;
   DevBuf<double> coord_points;
   // far per-call buffers
-  DevBuf<double> f_cp, f_mrsq, f_opr, f_M, f_dv;
+  DevBuf<double> f_cp, f_mrsq, f_opr, f_M, f_dv, f_spl;
   DevBuf<int> f_map, f_lmfd;
 };
 
@@ -159,6 +159,7 @@ void free_dfpt_device_data(void) {
   RefState &S = ref();
   S.rho1.reset();
   S.far.reset();
+  S.f_spl.release();
   S.wave.release();
   S.gwave.release();
   S.P1.release();
@@ -257,6 +258,14 @@ void sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_(
     FRPA_HIP(hipMemcpyAsync(S.f_opr.ptr, outer_potential_radius_ptr, S.f_opr.n * 8, hipMemcpyHostToDevice, s));
     FRPA_HIP(hipMemcpyAsync(S.f_M.ptr, multipole_moments_ptr, S.f_M.n * 8, hipMemcpyHostToDevice, s));
     FRPA_HIP(hipMemcpyAsync(S.f_dv.ptr, delta_v_hartree_ptr, S.f_dv.n * 8, hipMemcpyHostToDevice, s));
+    // near field (sum_up_close_part_cu): this call's per-species log grids and spline tile
+    const size_t n_spl = (size_t)lp1 * lp1 * 2 * n_hartree_grid * n_atoms * T;
+    FRPA_REQUIRE(species_center_ptr && n_grid_ptr && r_grid_min_ptr && log_r_grid_inc_ptr &&
+                     current_delta_v_hart_part_spl_tile_ptr && n_hartree_grid >= 2 && n_species == P.n_species,
+                 FRPA_ERR_INVALID, "near field: grid arrays, spline tile and n_hartree_grid >= 2 are required");
+    set_near_tables(P, n_species, species_center_ptr, n_grid_ptr, r_grid_min_ptr, log_r_grid_inc_ptr);
+    S.f_spl.alloc(n_spl);
+    FRPA_HIP(hipMemcpyAsync(S.f_spl.ptr, current_delta_v_hart_part_spl_tile_ptr, n_spl * 8, hipMemcpyHostToDevice, s));
     const char *v = getenv("FRPA_FAR_VARIANT");
     const int variant = (v && strcmp(v, "terms") == 0 && P.terms_ok) ? FRPA_FAR_TERMS : FRPA_FAR_HORNER;
     FarCall c;
@@ -270,6 +279,9 @@ void sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_(
     c.mrsq = S.f_mrsq.ptr;
     c.opr = S.f_opr.ptr;
     c.moments = S.f_M.ptr;
+    c.spl = S.f_spl.ptr;
+    c.n_hgrid = n_hartree_grid;
+    near_forward(P, c, S.f_dv.ptr, s);
     far_forward(P, c, variant, S.f_dv.ptr, s);
     FRPA_HIP(hipMemcpyAsync(delta_v_hartree_ptr, S.f_dv.ptr, S.f_dv.n * 8, hipMemcpyDeviceToHost, s));
     FRPA_HIP(hipStreamSynchronize(s));

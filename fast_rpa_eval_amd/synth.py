@@ -23,6 +23,8 @@ reference ABI do not scale to them; ``dense_i_basis_batches`` expands on demand.
 """
 from dataclasses import dataclass, field
 
+import math
+
 import numpy as np
 
 from .multipole_tables import far_tables, sh_norm, solid_harmonic_terms
@@ -186,6 +188,11 @@ class Rho1Problem:
             out[1:, b] = np.cumsum(cnt)
         return out
 
+    def valid_map(self):
+        """i_valid_point_2_i_full_points_map restricted to the valid points (its first sum(np)
+        entries; the rho1 path reads it only through n_point_batches_prefix_sum)."""
+        return self.i_valid_point_2_i_full_points_map[:int(self.n_point_batches_prefix_sum[-1])]
+
     def wave_offset_i32(self):
         if self.wave_offset[-1] >= 2 ** 31:
             raise OverflowError("wave_batches_compress exceeds the int32 offsets of the reference ABI")
@@ -342,6 +349,7 @@ class FarProblem:
     outer_potential_radius: np.ndarray        # (l_pot_max+1, n_atoms, T) F
     multipole_moments: np.ndarray             # ((l_pot_max+1)^2, n_atoms, T) F
     tables: dict = field(default_factory=dict)
+    near: dict = None                         # near-field inputs (add_near_field)
 
     @property
     def n_centers(self):
@@ -359,18 +367,23 @@ def moments(l_max, l_pot_max, n_atoms, n_tile, seed):
 
 
 def make_far_problem(n_atoms, box_ang, n_points, l_max, config_k=2, min_dist_bohr=8.0, all_far=True,
-                     n_tile=1, outer_radius=None, xyz=None, point_margin_bohr=8.0):
+                     n_tile=1, outer_radius=None, xyz=None, point_margin_bohr=8.0, grid=None):
     """Far-field inputs.
 
     all_far: points rejected unless >= min_dist_bohr from every atom (the far-field-only case);
     otherwise uniform in the padded box, pairs closer than min_dist_bohr are near-field and skipped.
     outer_radius: None -> 1e30 (no l reduction); 'decreasing' -> a per-atom table that lowers l
     with distance, exercising the reference's l-reduction loop.
+    grid: (coord_points (3, n_full) F, i_valid_point_2_i_full_points_map) of an existing integration
+    grid (e.g. a Rho1Problem's batches) used instead of generated points; n_points is then ignored.
     """
     sd = seeds(config_k)
     if xyz is None:
         xyz = make_geometry(n_atoms, box_ang * ANG, 1.5 * ANG, sd["geometry"])
     rng = np.random.default_rng(sd["grid"])
+    if grid is not None:
+        return _far_problem(xyz, np.asfortranarray(grid[0]), np.ascontiguousarray(grid[1], np.int32), l_max,
+                            sd, rng, min_dist_bohr, n_tile, outer_radius)
     lo = xyz.min(axis=0) - point_margin_bohr
     hi = xyz.max(axis=0) + point_margin_bohr
     pts = []
@@ -391,6 +404,13 @@ def make_far_problem(n_atoms, box_ang, n_points, l_max, config_k=2, min_dist_boh
     full_idx = np.sort(rng.choice(n_full, size=n_points, replace=False)).astype(np.int32)
     coord_points = np.zeros((3, n_full), order="F")
     coord_points[:, full_idx] = pts.T
+    return _far_problem(xyz, coord_points, full_idx, l_max, sd, rng, min_dist_bohr, n_tile, outer_radius)
+
+
+def _far_problem(xyz, coord_points, full_idx, l_max, sd, rng, min_dist_bohr, n_tile, outer_radius):
+    n_atoms = len(xyz)
+    n_full = coord_points.shape[1]
+    n_points = len(full_idx)
     l_pot_max = l_max
     lama = l_pot_max + 1
     tabs = far_tables(lama)
@@ -411,3 +431,55 @@ def make_far_problem(n_atoms, box_ang, n_points, l_max, config_k=2, min_dist_boh
         l_hartree_max_far_distance=np.full((n_atoms, n_tile), l_max, np.int32, order="F"),
         multipole_radius_sq=np.full((n_atoms, n_tile), min_dist_bohr ** 2, order="F"),
         outer_potential_radius=opr, multipole_moments=M, tables=tabs)
+
+
+def add_near_field(fp, n_hartree_grid=80, kind="smooth", n_species=2, r_max_bohr=40.0, seed=0):
+    """Near-field inputs of sum_up_whole_potential on a FarProblem (in place; returns fp).
+
+    Atoms alternate over n_species species (1 or 2); species 2 has l_hartree lowered by 2, a larger
+    r_grid_min and n_grid = n_hartree_grid - 7, so species indexing and the per-species clamp of the
+    spline interval are exercised.  Log grid of species s: r_i = r_grid_min(s) exp((i-1) log_r_grid_inc(s)),
+    i = 1..n_grid(s), r_{n_grid} = r_max_bohr.  current_delta_v_hart_part_spl_tile (lm, 2, grid, atom, t):
+      kind 'smooth': f_lm(r) = A_lm r^l / (1 + r)^(2l+1) with the derivative d f / d i = f'(r) r log_inc;
+      kind 'cubic' : f_lm(i) = a + b i + c i^2 + d i^3 in the grid index (the cubic Hermite spline is then
+                     exact, also outside the grid, which gives the closed form of the near sum).
+    Grid entries beyond a species' n_grid are NaN (the spline must never read them).  fp.near['coef']
+    keeps the cubic coefficients (lm, 4, atom, t) for the closed form.
+    """
+    assert n_species in (1, 2) and n_hartree_grid >= 9
+    rng = np.random.default_rng(seed)
+    n_atoms, T, L = fp.n_atoms, fp.n_tile, fp.l_pot_max
+    L2 = (L + 1) ** 2
+    if n_species == 2:
+        fp.species = (1 + np.arange(n_atoms) % 2).astype(np.int32)
+        fp.l_hartree = np.array([L, max(L - 2, 0)], np.int32)
+    species_center = fp.species[fp.center_to_atom - 1].astype(np.int32)
+    n_grid = np.array([n_hartree_grid, n_hartree_grid - 7][:n_species], np.int32)
+    r_grid_min = np.array([1e-4, 3e-4][:n_species])
+    log_inc = np.log(r_max_bohr / r_grid_min) / (n_grid - 1)
+    spl = np.full((L2, 2, n_hartree_grid, n_atoms, T), np.nan, order="F")
+    coef = np.zeros((L2, 4, n_atoms, T), order="F")
+    lvec = np.array([l for l in range(L + 1) for _ in range(2 * l + 1)])
+    for a in range(n_atoms):
+        s = fp.species[a] - 1
+        ng = n_grid[s]
+        i = np.arange(1, ng + 1, dtype=np.float64)
+        for t in range(T):
+            if kind == "smooth":
+                A = rng.standard_normal(L2) / np.array([math.factorial(l + 1) for l in lvec])
+                r = r_grid_min[s] * np.exp((i - 1) * log_inc[s])
+                lr = lvec[:, None]
+                f = r[None, :] ** lr / (1.0 + r[None, :]) ** (2 * lr + 1)
+                df = f * (lr / r[None, :] - (2 * lr + 1) / (1.0 + r[None, :]))
+                spl[:, 0, :ng, a, t] = A[:, None] * f
+                spl[:, 1, :ng, a, t] = A[:, None] * df * r[None, :] * log_inc[s]
+            elif kind == "cubic":
+                c = rng.standard_normal((L2, 4)) / np.array([1.0, ng, ng ** 2, ng ** 3])
+                coef[:, :, a, t] = c
+                spl[:, 0, :ng, a, t] = c[:, 0:1] + c[:, 1:2] * i + c[:, 2:3] * i ** 2 + c[:, 3:4] * i ** 3
+                spl[:, 1, :ng, a, t] = c[:, 1:2] + 2 * c[:, 2:3] * i + 3 * c[:, 3:4] * i ** 2
+            else:
+                raise ValueError(kind)
+    fp.near = dict(n_hartree_grid=int(n_hartree_grid), species_center=species_center, n_grid=n_grid,
+                   r_grid_min=r_grid_min, log_r_grid_inc=log_inc, spl=spl, kind=kind, coef=coef)
+    return fp

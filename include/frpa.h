@@ -1,7 +1,8 @@
 /*
  * fast_rpa_eval_amd: MI355X (gfx950) implementation of the two DFPT grid hot paths of
- * fast_rpa_eval -- the first-order density rho1 (evaluate_first_order_rho) and the far-field
- * Hartree multipole sum (far branch of sum_up_whole_potential) -- plus their adjoints.
+ * fast_rpa_eval -- the first-order density rho1 (evaluate_first_order_rho) and the Hartree
+ * potential sum (sum_up_whole_potential: far-field multipoles and near-field splines) -- plus the
+ * adjoints of rho1 and of the far field.
  *
  * Two layers, both C linkage, plain pointers and sizes:
  *
@@ -178,6 +179,33 @@ int frpa_far_adjoint(frpa_far_plan *plan, int atom_tile_size, int n_tile, int n_
                      const double *outer_potential_radius, const double *multipole_moments, const double *gV,
                      double *grad_moments, void *stream);
 
+/*
+ * Near-field (spline) tables of the hartree centres, host arrays, copied: species_center[n_centers]
+ * (1-based), and per species n_grid[n_species], r_grid_min[n_species], log_r_grid_inc[n_species]
+ * (the arrays of the same names in sum_up_whole_potential.h:3-48).  Required before
+ * frpa_near_forward; n_grid >= 2, r_grid_min > 0, log_r_grid_inc > 0.
+ */
+int frpa_far_plan_set_near(frpa_far_plan *plan, int n_species, const int *species_center, const int *n_grid,
+                           const double *r_grid_min, const double *log_r_grid_inc);
+
+/*
+ * delta_v_hartree(p, t) += sum over hartree centres c with dist^2 < multipole_radius_sq(atom, t) of
+ *   sum_{lm < (l+1)^2} Y_lm(dir) S_lm(i_r_log), i_r_log = 1 + log(dist / r_grid_min) / log_r_grid_inc,
+ * S_lm the cubic Hermite spline of current_delta_v_hart_part_spl_tile on the log-grid index, l the
+ * atom's l_hartree lowered by outer_potential_radius (near branch of sum_up_whole_potential.cpp:382-477;
+ * sum_up_close_part_cu, sum_up_whole_potential.cu:168-371).  Device arrays:
+ *   coord_points (3, n_full_points), i_valid_point_2_i_full_points_map[n_valid_points],
+ *   multipole_radius_sq (n_atoms, T), outer_potential_radius (l_pot_max + 1, n_atoms, T),
+ *   current_delta_v_hart_part_spl_tile ((l_pot_max+1)^2, 2, n_hartree_grid, n_atoms, T),
+ *   delta_v_hartree (n_full_points, T) ACCUMULATED into.
+ * n_hartree_grid >= n_grid of every centre's species.  l_pot_max <= 10 (the reference's device code
+ * stops at 6).  frpa_near_forward followed by frpa_far_forward is the whole sum_up_whole_potential.
+ */
+int frpa_near_forward(frpa_far_plan *plan, int atom_tile_size, int n_tile, int n_full_points, int n_valid_points,
+                      int n_hartree_grid, const double *coord_points, const int *i_valid_point_2_i_full_points_map,
+                      const double *multipole_radius_sq, const double *outer_potential_radius,
+                      const double *current_delta_v_hart_part_spl_tile, double *delta_v_hartree, void *stream);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Synthetic wave generation on the device (benchmark inputs too large to build on the host)   */
 /* ------------------------------------------------------------------------------------------ */
@@ -239,8 +267,10 @@ void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
     double *density_matrix_ptr);
 
 /* fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.h:50-95 (C++ linkage there; C linkage here).
- * Far-field branch only: uploads the per-call arrays, accumulates into delta_v_hartree and
- * synchronises, as the reference (sum_up_whole_potential.cu:641-769). */
+ * Both branches, as the reference (sum_up_whole_potential.cu:641-769): uploads the per-call arrays
+ * (including current_delta_v_hart_part_spl_tile), runs the near-field spline sum and the far-field
+ * multipole sum, accumulates into delta_v_hartree and synchronises.  species_center, n_grid,
+ * r_grid_min and log_r_grid_inc must describe valid log grids (n_grid >= 2, n_grid <= n_hartree_grid). */
 void sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_(
     const int j_atom_begin, const int j_atom_end, const int n_full_points, const int l_max_analytic_multipole,
     const int index_cc_dim_0, const int n_valid_points, const int l_pot_max, const int n_max_radial,

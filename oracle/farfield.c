@@ -21,7 +21,8 @@
  * centers_hartree_potential is the identity, as in every cluster run.
  * Entries of multipole_c beyond n_cc_lm_ijk(lmfd(a, t)) are left NaN here (the reference leaves
  * them uninitialised); reading them would poison the result.
- * delta_v_hartree is ACCUMULATED into (+=).  The near branch (:411-477) is not restated.
+ * delta_v_hartree is ACCUMULATED into (+=).  The near branch (:411-477) is oracle_sum_up_near
+ * (nearfield.c); the whole sum_up_whole_potential is the two applied one after the other.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -90,7 +91,7 @@ void oracle_sum_up_far(
         while (outer_potential_radius[l + lp1 * (atom + (size_t)n_atoms * t)] < dist_sq && l > 0)
           l--;
         if (dist_sq < multipole_radius_sq[atom + (size_t)n_atoms * t])
-          continue; /* near field: not restated */
+          continue; /* near field: oracle_sum_up_near */
         if (!(dist_sq < 1e8))
           continue;
         /* far_distance_real_hartree_potential_single_atom_p2_fused_fp_c(l, ..., ic + 1, sqrt(d2)) */

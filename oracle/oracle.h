@@ -16,6 +16,11 @@
  *   oracle_sum_up_far                closed form sum_lm M_lm Y_lm / r^(l+1) (scipy + SHEval
  *                                    coefficients), reference AOT term table fixture, branch and
  *                                    l-reduction special cases.
+ *   oracle_ylm                       scipy's spherical harmonics to l = 10 and the SHEval
+ *                                    polynomial fixture (l <= 3).
+ *   oracle_sum_up_near               closed form sum_lm f_lm(i_r_log) Y_lm for tiles cubic in the
+ *                                    grid index (exact for a cubic Hermite spline, clamped ends
+ *                                    included), node values, near/far split and l reduction.
  *   oracle_evaluate_first_order_h    adjoint identity <v, rho1(P)> = <H1(v), P>, exact
  *                                    d rho1 / d P_ij = phi_i phi_j, GGA terms against an independent
  *                                    numpy restatement on tiny inputs.
@@ -39,8 +44,8 @@ void oracle_evaluate_first_order_rho(
     double *first_order_gradient_rho, const double *first_order_gradient_rho_bias_batches_atoms);
 
 /* Far branch of fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.cpp:195-529 (with :91-187);
- * the near branch (:411-477) is not restated: pairs with dist^2 < multipole_radius_sq add nothing,
- * as in the far-only reference of task3_jit/version5_sumup_far_part_with_jit_or_aot. */
+ * pairs with dist^2 < multipole_radius_sq are the near branch (oracle_sum_up_near) and add nothing
+ * here, as in the far-only reference of task3_jit/version5_sumup_far_part_with_jit_or_aot. */
 void oracle_sum_up_far(
     int atom_tile_size, int j_atom_begin, int j_atom_end, int n_full_points, int l_max_analytic_multipole,
     int index_cc_dim_0, int n_valid_points, int l_pot_max, int n_centers, int n_centers_hartree_potential,
@@ -50,6 +55,22 @@ void oracle_sum_up_far(
     const double *multipole_moments, const int *l_hartree, const int *n_cc_lm_ijk,
     const int *centers_hartree_potential, const int *center_to_atom, const int *index_cc,
     const int *index_ijk_max_cc, const double *cc, const double *coords_center, double *delta_v_hartree);
+
+/* Near branch of fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.cpp:382-477 (with :12-49 and
+ * sheval.hpp): pairs with dist^2 < multipole_radius_sq, cubic-Hermite spline of the per-atom
+ * (lm, 2, n_hartree_grid) tile on the log grid, dotted with Y_lm; accumulated into delta_v_hartree. */
+void oracle_sum_up_near(
+    int atom_tile_size, int j_atom_begin, int j_atom_end, int n_full_points, int n_valid_points, int l_pot_max,
+    int n_hartree_grid, int n_centers_hartree_potential, int n_atoms,
+    const int *species, const int *species_center, const int *i_valid_point_2_i_full_points_map,
+    const double *coord_points, const double *multipole_radius_sq, const double *outer_potential_radius,
+    const double *current_delta_v_hart_part_spl_tile, const int *l_hartree, const int *n_grid,
+    const int *centers_hartree_potential, const int *center_to_atom, const double *coords_center,
+    const double *r_grid_min, const double *log_r_grid_inc, double *delta_v_hartree);
+
+/* Real spherical harmonics of the SHEval convention (sheval.hpp:6-343), ylm[(lmax+1)^2], from
+ * (sin theta, cos theta, sin phi, cos phi); lmax <= 31. */
+void oracle_ylm(int lmax, double sintheta, double costheta, double sinphi, double cosphi, double *ylm);
 
 /* fast_rap_src_dfpt_gpu_eval/evaluate_first_order_H.cpp:13-222 (all LDA and GGA terms). */
 void oracle_evaluate_first_order_h(

@@ -31,6 +31,9 @@ def lib():
         L.oracle_evaluate_first_order_rho.argtypes = [i] * 11 + [p] * 15
         L.oracle_sum_up_far.argtypes = [i] * 12 + [p] * 16
         L.oracle_evaluate_first_order_h.argtypes = [i] * 11 + [p] * 22
+        L.oracle_sum_up_near.argtypes = [i] * 9 + [p] * 15
+        d = ctypes.c_double
+        L.oracle_ylm.argtypes = [i, d, d, d, d, p]
         L.oracle_set_threads.argtypes = [i]
         L.oracle_set_threads.restype = i
         _LIB = L
@@ -101,6 +104,47 @@ def far(fp, delta_v=None, j_atom_begin=1, j_atom_end=None):
         _p(tb["index_cc"], np.int32), _p(tb["index_ijk_max_cc"], np.int32), _p(tb["cc"], np.float64),
         _p(fp.coords_center, np.float64), _p(delta_v, np.float64))
     return delta_v
+
+
+def near(fp, delta_v=None, j_atom_begin=1, j_atom_end=None):
+    """Near-field branch of sum_up_whole_potential on the CPU (fp carries the near inputs of
+    synth.add_near_field); delta_v (n_full, T) F accumulated into."""
+    T = fp.n_tile
+    if j_atom_end is None:
+        j_atom_end = j_atom_begin + T - 1
+    if delta_v is None:
+        delta_v = np.zeros((fp.n_full_points, T), order="F")
+    nf = fp.near
+    lib().oracle_sum_up_near(
+        T, j_atom_begin, j_atom_end, fp.n_full_points, fp.n_valid_points, fp.l_pot_max, nf["n_hartree_grid"],
+        len(fp.centers_hartree_potential), fp.n_atoms,
+        _p(fp.species, np.int32), _p(nf["species_center"], np.int32),
+        _p(fp.i_valid_point_2_i_full_points_map, np.int32), _p(fp.coord_points, np.float64),
+        _p(fp.multipole_radius_sq, np.float64), _p(fp.outer_potential_radius, np.float64),
+        _p(nf["spl"], np.float64), _p(fp.l_hartree, np.int32), _p(nf["n_grid"], np.int32),
+        _p(fp.centers_hartree_potential, np.int32), _p(fp.center_to_atom, np.int32),
+        _p(fp.coords_center, np.float64), _p(nf["r_grid_min"], np.float64), _p(nf["log_r_grid_inc"], np.float64),
+        _p(delta_v, np.float64))
+    return delta_v
+
+
+def ylm(lmax, xyz):
+    """SHEval-convention Y_lm at unit-vector rows xyz, through the reference's trigonometric route
+    (sin/cos of theta and phi as tab_single_trigonom); returns (n, (lmax+1)^2)."""
+    xyz = np.asarray(xyz, dtype=np.float64)
+    out = np.zeros((len(xyz), (lmax + 1) ** 2))
+    buf = np.zeros((lmax + 1) ** 2)
+    for n, (x, y, z) in enumerate(xyz):
+        ab = np.hypot(x, y)
+        if max(abs(x), abs(y)) > 1e-36:
+            cphi, sphi = x / ab, y / ab
+        else:
+            cphi, sphi, ab = 1.0, 0.0, 0.0
+        abc = np.hypot(ab, z)
+        sth, cth = (ab / abc, z / abc) if max(abs(x), abs(y), abs(z)) > 1e-36 else (1.0, 0.0)
+        lib().oracle_ylm(lmax, sth, cth, sphi, cphi, buf.ctypes.data_as(ctypes.c_void_p))
+        out[n] = buf
+    return out
 
 
 def h1(prob, T, j_atom_begin, j_atom_end, j_coord, H, arrays):

@@ -4,7 +4,8 @@ sum_up_run.cpp): init_dfpt_device_info, init_dfpt_device_data (57 arguments),
 evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_ (28 pointer arguments),
 sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_ (scalars by value), free_*.
 Outputs against the CPU oracle of the same reference functions, ATOM_TILE_SIZE = 2 with a short
-tile, GGA gradient and bias included; the far field accumulates into delta_v_hartree."""
+tile, GGA gradient and bias included; the whole Hartree sum (near-field splines of two species plus
+far-field multipoles) accumulates into delta_v_hartree."""
 import ctypes
 
 import numpy as np
@@ -61,6 +62,9 @@ def test_reference_entry_points_against_oracle(cuda, oracle, monkeypatch):
     prob = synth.make_rho1_problem(10, 11.0, 40, (20, 120), (10, 90), config_k=4)
     fp = synth.make_far_problem(10, 11.0, 4000, 6, config_k=4, xyz=prob.atom_xyz, n_tile=T,
                                 outer_radius="decreasing", all_far=False)
+    synth.add_near_field(fp, 60, "smooth", n_species=2, seed=9)
+    nf = fp.near
+    nhg = nf["n_hartree_grid"]
     tb = fp.tables
     L = _lib.lib()
     d, keep = _inputs(prob, fp, T, 21)
@@ -73,25 +77,26 @@ def test_reference_entry_points_against_oracle(cuda, oracle, monkeypatch):
         ncc=prob.n_compute_c_batches, species=fp.species, lh=fp.l_hartree, nccl=tb["n_cc_lm_ijk"],
         chp=fp.centers_hartree_potential, c2a=fp.center_to_atom, icc=tb["index_cc"],
         ijk=tb["index_ijk_max_cc"], lmfd=fp.l_hartree_max_far_distance,
-        fmap=fp.i_valid_point_2_i_full_points_map).items()}
+        fmap=fp.i_valid_point_2_i_full_points_map, spc=nf["species_center"], ngrid=nf["n_grid"]).items()}
     dbl = {k: _arr(v, np.float64) for k, v in dict(
         cp=prob.coord_points, fcp=fp.coord_points, cc=tb["cc"], cen=fp.coords_center,
-        mrsq=fp.multipole_radius_sq, opr=fp.outer_potential_radius, M=fp.multipole_moments).items()}
+        mrsq=fp.multipole_radius_sq, opr=fp.outer_potential_radius, M=fp.multipole_moments,
+        rmin=nf["r_grid_min"], linc=nf["log_r_grid_inc"], spl=nf["spl"]).items()}
     I = lambda k: ints[k][1]
     D = lambda k: dbl[k][1]
 
     L.init_dfpt_device_info()
     args = [_ip(prob.n_batches), _ip(prob.n_full_points), _ip(fp.n_valid_points), _ip(prob.n_basis),
             _ip(prob.n_atoms), _ip(prob.n_max_compute_ham), _ip(prob.n_centers_basis_I), _ip(prob.n_max_batch_size),
-            _ip(fp.l_pot_max), _ip(1), _ip(1), _ip(fp.n_centers), _ip(len(fp.centers_hartree_potential)),
+            _ip(fp.l_pot_max), _ip(1), _ip(nhg), _ip(fp.n_centers), _ip(len(fp.centers_hartree_potential)),
             _ip(fp.hartree_force_l_add), _ip(len(fp.l_hartree)), _ip(fp.l_max_analytic_multipole),
             _ip(tb["index_cc_dim_0"]),
             I("basis_atom"), I("batch_sizes"), I("npb"), I("pref"), I("map"), I("ncc"), d["ibasis"], d["avn"],
             d["woff"], d["wave"], d["gwave"], zd,
-            zi, I("species"), zi, D("cp"), I("lh"), zi, I("nccl"), I("chp"), I("c2a"), I("icc"), I("ijk"), D("cc"),
-            D("cen"), zd, zd, zd, zd,
+            zi, I("species"), I("spc"), D("cp"), I("lh"), I("ngrid"), I("nccl"), I("chp"), I("c2a"), I("icc"),
+            I("ijk"), D("cc"), D("cen"), D("rmin"), D("linc"), zd, zd,
             d["P1"], d["rho"], d["bias"], d["grad"], d["gbias"],
-            I("lmfd"), D("mrsq"), D("opr"), D("M"), zd, d["dv"]]
+            I("lmfd"), D("mrsq"), D("opr"), D("M"), D("spl"), d["dv"]]
     assert len(args) == 57
     L.init_dfpt_device_data(*args)
 
@@ -121,19 +126,23 @@ def test_reference_entry_points_against_oracle(cuda, oracle, monkeypatch):
     print(f"[reference rho1 entry, INT8 split, no gradient] rel {rel2:.3e}", flush=True)
     assert rel2 <= 1e-10 and np.all(keep["grad"] == 0.0)
 
-    # far field: accumulates into delta_v_hartree, both perturbations
+    # whole potential (near-field splines + far-field multipoles): accumulates into delta_v_hartree,
+    # both perturbations, two species with their own log grids
     L.sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_.argtypes = [ctypes.c_int] * 14 + [P] * 24
     L.sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_(
         1, T, fp.n_full_points, fp.l_max_analytic_multipole, tb["index_cc_dim_0"], fp.n_valid_points, fp.l_pot_max,
-        1, 1, fp.n_centers, len(fp.centers_hartree_potential), fp.n_atoms, fp.hartree_force_l_add,
+        1, nhg, fp.n_centers, len(fp.centers_hartree_potential), fp.n_atoms, fp.hartree_force_l_add,
         len(fp.l_hartree),
-        zi, I("species"), zi, I("lmfd"), I("fmap"), D("fcp"), D("mrsq"), D("opr"), D("M"), zd, I("lh"), zi,
-        I("nccl"), I("chp"), I("c2a"), I("icc"), I("ijk"), D("cc"), D("cen"), zd, zd, zd, zd, d["dv"])
-    vref = oracle.far(fp, delta_v=keep["dv0"].copy(order="F"), j_atom_begin=1, j_atom_end=T)
+        zi, I("species"), I("spc"), I("lmfd"), I("fmap"), D("fcp"), D("mrsq"), D("opr"), D("M"), D("spl"), I("lh"),
+        I("ngrid"), I("nccl"), I("chp"), I("c2a"), I("icc"), I("ijk"), D("cc"), D("cen"), D("rmin"), D("linc"), zd,
+        zd, d["dv"])
+    vfar = oracle.far(fp, delta_v=keep["dv0"].copy(order="F"), j_atom_begin=1, j_atom_end=T)
+    vref = oracle.near(fp, delta_v=vfar.copy(order="F"), j_atom_begin=1, j_atom_end=T)
+    assert np.abs(vref - vfar).max() > 0.1 * np.abs(vfar - keep["dv0"]).max()  # the near part matters
     vrel = np.abs(keep["dv"] - vref).max() / np.abs(vref - keep["dv0"]).max()
     for t in range(T):
         dt = np.abs(keep["dv"][:, t] - vref[:, t]).max() / np.abs(vref[:, t] - keep["dv0"][:, t]).max()
-        print(f"[reference far-field entry] perturbation {t}: rel {dt:.3e}", flush=True)
+        print(f"[reference sum_up_whole_potential entry] perturbation {t}: rel {dt:.3e}", flush=True)
     assert vrel <= 1e-12
     L.free_dfpt_device_data()
     L.free_dfpt_device_info()

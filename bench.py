@@ -132,6 +132,72 @@ def near_pairs(fpn):
     return n, nlm
 
 
+def bench_bias2(D, steps, warmup, no_cpu, threads, T=4, jc=1):
+    """generate_first_order_rho_bias_part2 for the tile of atoms 1..T (the reference's fixed tile of
+    4), j_coord = 1, over all batches: random gradient wave (3x the wave, on the device), P1 as the
+    density matrix.  HIP events on torch's current stream."""
+    import torch
+    prob, plan = D["prob"], D["plan"]
+    wl = int(prob.wave_offset[-1])
+    g = torch.Generator(device="cuda").manual_seed(17)
+    gwave = torch.empty(3 * wl, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0, generator=g)
+    avn_h = prob.atom_valid_n_compute_c_batches()
+    avn = torch.from_numpy(np.ascontiguousarray(avn_h.ravel(order="F"))).cuda()
+    out = torch.zeros(prob.n_max_batch_size * prob.n_batches * T, dtype=torch.float64, device="cuda")
+
+    def run():
+        plan.bias_part2(D["wave"], gwave, D["P1"], avn, prob.n_atoms, 1, n_tile=T, j_coord=jc, T=T, out=out)
+    for _ in range(max(warmup, 1)):
+        run()
+    torch.cuda.synchronize()
+    ev = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run()
+        e1.record()
+        ev.append((e0, e1))
+    torch.cuda.synchronize()
+    ms = float(np.mean([x.elapsed_time(y) for x, y in ev]))
+    # algorithmic work of the batches with rows of the tile's atoms
+    nc = prob.n_compute_c_batches.astype(np.float64)
+    npt = prob.n_point_batches.astype(np.float64)
+    nR = (avn_h[T, :] - avn_h[0, :]).astype(np.float64)
+    sel = (nc > 0) & (npt > 0) & (nR > 0)
+    nbytes = float(np.sum((8 * nc * npt + 16 * nR * nc + 8 * nR * npt)[sel]) + np.sum(8 * T * npt[(nc > 0)]))
+    flops = float(np.sum((2 * nR * nc * npt)[sel]))
+    ach = nbytes / (ms / 1e3) / 1e9
+    res = {"workload": f"tile of atoms 1-{T}, j_coord {jc}, all {prob.n_batches} batches "
+                       f"({int(sel.sum())} with rows of the tile's atoms)",
+           "ms": ms, "batches_with_rows": int(sel.sum()),
+           "roofline": {"kernel": "frpa::bias2_kernel", "bound": "hbm", "achieved": ach, "peak": PEAK_HBM,
+                        "unit": "GB/s", "frac": ach / PEAK_HBM, "algorithmic_bytes": nbytes,
+                        "algorithmic_flops": flops, "fp64_frac": flops / (ms / 1e3) / 1e12 / PEAK_FP64}}
+    if not no_cpu:
+        from oracle import oracle
+        rng = np.random.default_rng(5)
+        pick = np.sort(rng.choice(np.nonzero(sel)[0], size=min(300, int(sel.sum())), replace=False))
+        sub = subproblem(prob, pick, D["wave"])
+        # gradient wave of the sub-problem: the same batches' blocks
+        gw_h = np.concatenate([gwave[3 * int(prob.wave_offset[b]):3 * int(prob.wave_offset[b + 1])].cpu().numpy()
+                               for b in pick])
+        DM = D["P1"].view(prob.n_basis, prob.n_basis).T.cpu().numpy()  # column-major view of the same matrix
+        oracle.set_threads(threads)
+        t0 = time.time()
+        ref = oracle.bias2(sub, np.asfortranarray(DM), gw_h, T, 1, j_coord=jc)
+        cpu_s = time.time() - t0
+        got = out.view(T, prob.n_batches, prob.n_max_batch_size).cpu().numpy()
+        err = 0.0
+        for q, b in enumerate(pick):
+            n = prob.n_point_batches[b]
+            err = max(err, float(np.abs(got[:, b, :n].T - ref[:n, q, :]).max()))
+        res["vs_oracle_rel"] = err / float(np.abs(ref).max())
+        res["cpu_baseline"] = {"value": len(pick) / cpu_s, "unit": "batches/s", "cores": threads, "kind": "port",
+                               "sample": f"{len(pick)} random batches with rows of the tile's atoms"}
+    del gwave, avn, out
+    return res
+
+
 def time_near(N, steps, warmup):
     """Near-field launch timed with HIP events on torch's current stream (the stream it runs on)."""
     import torch
@@ -450,6 +516,9 @@ def main():
     ap.add_argument("--near", type=int, default=1,
                     help="1: also time the near-field (spline) branch of the Hartree sum over the rho1 grid "
                          "(reported as near_field, outside the step)")
+    ap.add_argument("--bias2", type=int, default=1,
+                    help="1: also time the bias generator (generate_first_order_rho_bias_part2) for the tile of "
+                         "atoms 1-4 over all batches (reported as bias_part2, outside the step)")
     ap.add_argument("--kernels", default="rho1,adjoint,far",
                     help="subset of the step to time (a partial step is not a bench line)")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -638,7 +707,9 @@ def main():
                 "roofline": {"kernel": "frpa::near_kernel", "bound": "mfma", "achieved": ach, "peak": PEAK_FP64,
                              "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "flop_per_pair_lm": 11,
                              "coef_bytes_per_s": coef_bytes / (ms / 1e3),
-                             "note": "FP64 VALU; coefficient gathers (32 B per lm per pair) are served by L2/MALL"}}
+                             "note": "FP64 VALU at the nominal count; the binding resource is the per-pair "
+                                     "gather of the interval's 4 (l+1)^2 coefficients (coef_bytes_per_s), served "
+                                     "from L2 (the tiles, 0.27 GB, are read from HBM about once)"}}
         if not a.no_cpu:
             from oracle import oracle
             sel = np.sort(np.random.default_rng(11).choice(fpn.n_valid_points, size=min(20000, fpn.n_valid_points),
@@ -658,6 +729,16 @@ def main():
         line["near_field"] = near
         log(f"near field: {ms:.3f} ms, {npairs} pairs, {near['pairs_per_s'] / 1e9:.2f} G pairs/s, "
             f"{ach:.1f} TF/s ({100 * ach / PEAK_FP64:.0f}% FP64)")
+
+    if a.bias2:
+        try:
+            line["bias_part2"] = bench_bias2(D, a.steps, a.warmup, a.no_cpu, a.cpu_threads)
+            b2 = line["bias_part2"]
+            log(f"bias part 2: {b2['ms']:.3f} ms, {b2['roofline']['achieved']:.0f} GB/s "
+                f"({100 * b2['roofline']['frac']:.0f}% HBM)")
+        except torch.cuda.OutOfMemoryError as e:
+            log(f"bias part 2 skipped: {e}")
+        torch.cuda.empty_cache()
 
     if not a.no_cpu:
         cb = cpu_baseline(D, a.cpu_threads)

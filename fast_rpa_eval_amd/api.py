@@ -125,6 +125,20 @@ class Rho1Plan:
         check(lib().frpa_rho1_set_launch(self.handle, precision, order, persistent, points_per_wave),
               "frpa_rho1_set_launch")
 
+    def bias_part2(self, wave, gradient_wave, density_matrix, atom_valid_n_compute_c_batches, n_atoms,
+                   j_atom_begin, n_tile=1, j_coord=1, T=None, out=None, stream=None):
+        """first_order_rho_bias_part2 (n_max_batch_size, n_batches, T) for the atoms j_atom_begin ..
+        j_atom_begin + n_tile - 1 (generate_first_order_rho_bias_part2.cpp:9-167); device tensors,
+        atom_valid_n_compute_c_batches int32 (n_atoms + 1, n_batches).  Entries of batches with
+        n_compute_c = 0 or n_points = 0 keep `out`'s values (zeros if out is None)."""
+        T = n_tile if T is None else T
+        if out is None:
+            out = torch.zeros(self.n_max_batch_size * self.n_batches * T, dtype=torch.float64, device="cuda")
+        check(lib().frpa_rho1_bias_part2(
+            self.handle, T, j_atom_begin, n_tile, j_coord, n_atoms, _dp(atom_valid_n_compute_c_batches), _dp(wave),
+            _dp(gradient_wave), _dp(density_matrix), _dp(out), _stream(stream)), "frpa_rho1_bias_part2")
+        return out
+
     def prepare_wave(self, wave, precision=OZAKI, stream=None):
         b = ctypes.c_longlong(0)
         check(lib().frpa_rho1_prepare_wave(self.handle, precision, _dp(wave), ctypes.byref(b), _stream(stream)),

@@ -582,6 +582,18 @@ int frpa_far_adjoint(frpa_far_plan *plan, int atom_tile_size, int n_tile, int n_
   });
 }
 
+int frpa_rho1_bias_part2(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord,
+                         int n_atoms, const int *atom_valid_n_compute_c_batches, const double *wave,
+                         const double *gradient_wave, const double *density_matrix,
+                         double *first_order_rho_bias_part2, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
+    rho1_bias_part2(*reinterpret_cast<Rho1Plan *>(plan), atom_tile_size, j_atom_begin, n_tile, j_coord, n_atoms,
+                    atom_valid_n_compute_c_batches, wave, gradient_wave, density_matrix, first_order_rho_bias_part2,
+                    as_stream(stream));
+  });
+}
+
 int frpa_far_plan_set_near(frpa_far_plan *plan, int n_species, const int *species_center, const int *n_grid,
                            const double *r_grid_min, const double *log_r_grid_inc) {
   return guarded([&] {

@@ -24,10 +24,11 @@ struct RefState {
   hipStream_t stream = nullptr;
   bool data = false;
   int T = 1;
-  int n_batches = 0, n_full = 0, n_basis = 0, nmb = 0, n_valid_points = 0;
+  int n_batches = 0, n_full = 0, n_basis = 0, nmb = 0, n_valid_points = 0, n_atoms = 0;
   std::unique_ptr<Rho1Plan> rho1;
   std::unique_ptr<FarPlan> far;
-  DevBuf<double> wave, gwave, P1, bias, gbias, rho, grad;
+  DevBuf<double> wave, gwave, P1, bias, gbias, rho, grad, dm;
+  DevBuf<int> avn;  // atom_valid_n_compute_c_batches (n_atoms + 1, n_batches)
   DevBuf<double> coord_points;
   // far per-call buffers
   DevBuf<double> f_cp, f_mrsq, f_opr, f_M, f_dv, f_spl;
@@ -122,6 +123,7 @@ void init_dfpt_device_data(
     S.n_basis = *n_basis_;
     S.nmb = *n_max_batch_size_;
     S.n_valid_points = *n_valid_points_;
+    S.n_atoms = *n_atoms_;
     // dense (n_centers_basis_I, n_batches) 1-based lists -> CSR; int32 wave offsets -> int64
     std::vector<long long> ibp(nb + 1, 0), woff(nb + 1);
     std::vector<int> ib;
@@ -139,6 +141,7 @@ void init_dfpt_device_data(
     const size_t wl = nb ? (size_t)woff[nb] : 0;
     S.wave.upload(wave_batches_compress_ptr, wl);
     S.gwave.upload(gradient_wave_batches_compress_ptr, 3 * wl);
+    S.avn.upload(atom_valid_n_compute_c_batches_ptr, (size_t)(S.n_atoms + 1) * nb);
     S.P1.alloc((size_t)S.n_basis * S.n_basis * S.T);
     S.bias.alloc((size_t)S.nmb * nb * S.T);
     S.gbias.alloc((size_t)3 * S.nmb * nb * S.T);
@@ -165,6 +168,8 @@ void free_dfpt_device_data(void) {
   S.P1.release();
   S.bias.release();
   S.gbias.release();
+  S.dm.release();
+  S.avn.release();
   S.rho.release();
   S.grad.release();
   S.coord_points.release();
@@ -213,6 +218,39 @@ void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
       fill_f64(S.grad.ptr, 3 * nbt, 0.0, s);
     FRPA_HIP(hipMemcpyAsync(first_order_rho_ptr, S.rho.ptr, S.rho.n * 8, hipMemcpyDeviceToHost, s));
     FRPA_HIP(hipMemcpyAsync(first_order_gradient_rho_ptr, S.grad.ptr, 3 * nbt * 8, hipMemcpyDeviceToHost, s));
+    FRPA_HIP(hipStreamSynchronize(s));
+  });
+}
+
+void generate_first_order_rho_bias_part2_(
+    int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
+    int *n_basis_, int *n_atoms_, int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_,
+    int *batch_sizes_ptr, int *n_point_batches_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *i_batch_2_wave_offset_ptr, double *wave_batches_compress_ptr,
+    double *gradient_wave_batches_compress_ptr, double *first_order_rho_bias_part2_ptr, double *partition_tab_ptr,
+    double *density_matrix_ptr) {
+  run_or_die("generate_first_order_rho_bias_part2_", [&] {
+    RefState &S = ref();
+    FRPA_REQUIRE(S.data && S.rho1, FRPA_ERR_STATE,
+                 "init_dfpt_device_data must be called first (the batch data are used from the device)");
+    FRPA_REQUIRE(*n_my_batches_work_ == S.n_batches && *n_basis_ == S.n_basis && *n_max_batch_size_ == S.nmb &&
+                     *n_atoms_ == S.n_atoms,
+                 FRPA_ERR_INVALID, "sizes differ from init_dfpt_device_data");
+    const int n_tile = *j_atom_end_ - *j_atom_begin_ + 1;
+    FRPA_REQUIRE(n_tile >= 1 && n_tile <= S.T, FRPA_ERR_INVALID,
+                 "j_atom_end - j_atom_begin + 1 = %d outside [1, ATOM_TILE_SIZE = %d]", n_tile, S.T);
+    hipStream_t s = S.stream;
+    const size_t nbt = (size_t)S.nmb * S.n_batches * S.T;
+    S.dm.alloc((size_t)S.n_basis * S.n_basis);
+    FRPA_HIP(hipMemcpyAsync(S.dm.ptr, density_matrix_ptr, S.dm.n * 8, hipMemcpyHostToDevice, s));
+    // entries the generator does not write (n_compute_c = 0 batches, points past n_points) keep the
+    // caller's values, as with the reference's in-place CPU loop
+    FRPA_HIP(hipMemcpyAsync(S.bias.ptr, first_order_rho_bias_part2_ptr, nbt * 8, hipMemcpyHostToDevice, s));
+    const int rc = frpa_rho1_bias_part2(reinterpret_cast<frpa_rho1_plan *>(S.rho1.get()), S.T, *j_atom_begin_,
+                                        n_tile, *j_coord_, S.n_atoms, S.avn.ptr, S.wave.ptr, S.gwave.ptr, S.dm.ptr,
+                                        S.bias.ptr, s);
+    FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
+    FRPA_HIP(hipMemcpyAsync(first_order_rho_bias_part2_ptr, S.bias.ptr, nbt * 8, hipMemcpyDeviceToHost, s));
     FRPA_HIP(hipStreamSynchronize(s));
   });
 }

@@ -115,4 +115,9 @@ void synth_wave(const Rho1Plan &P, int n_atoms, const double *atom_xyz, const in
                 const int *basis_m, const double *basis_alpha, int n_basis, const double *coord_points, double *wave,
                 hipStream_t s);
 
+// first_order_rho_bias_part2 (nmb, n_batches, T) for the atoms j_atom_begin .. + n_tile - 1 (rho1_bias2.hip);
+// avn = atom_valid_n_compute_c_batches (n_atoms + 1, n_batches) on the device.
+void rho1_bias_part2(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,
+                     const double *wave, const double *gwave, const double *DM, double *bias, hipStream_t s);
+
 } // namespace frpa

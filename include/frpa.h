@@ -126,6 +126,22 @@ int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int per
 int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wave, long long *bytes,
                            void *stream);
 
+/*
+ * Bias input of rho1 (generate_first_order_rho_bias_part2.cpp:9-167, a CPU function in the
+ * reference): for the atoms j = j_atom_begin .. j_atom_begin + n_tile - 1 (t = j - j_atom_begin) and
+ * every batch with n_compute_c > 0,
+ *   first_order_rho_bias_part2(p, b, t) = sum_{k in V} phi'(k, p) sum_i (DM(i, k) + DM(k, i)) phi(i, p),
+ * V = the batch's compute rows of atom j ([avn(j-1, b), avn(j, b))), phi' = -d phi / d x_{j_coord},
+ * DM(i, k) = density_matrix(i_basis(i), i_basis(k)); 0 for empty V; other entries untouched.
+ * Device arrays: atom_valid_n_compute_c_batches (n_atoms + 1, n_batches) int, wave and gradient_wave
+ * as for frpa_rho1_forward, density_matrix (n_basis, n_basis), first_order_rho_bias_part2
+ * (n_max_batch_size, n_batches, atom_tile_size).  n_tile <= min(atom_tile_size, 8), j_coord 1..3.
+ */
+int frpa_rho1_bias_part2(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord,
+                         int n_atoms, const int *atom_valid_n_compute_c_batches, const double *wave,
+                         const double *gradient_wave, const double *density_matrix,
+                         double *first_order_rho_bias_part2, void *stream);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Far-field Hartree multipole sum                                                            */
 /* ------------------------------------------------------------------------------------------ */
@@ -264,6 +280,20 @@ void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
     double *gradient_wave_batches_compress_ptr, double *first_order_density_matrix_ptr, double *first_order_rho_ptr,
     double *first_order_rho_bias_part2_ptr, double *first_order_gradient_rho_ptr,
     double *first_order_gradient_rho_bias_batches_atoms_ptr, int *basis_atom_ptr, double *partition_tab_ptr,
+    double *density_matrix_ptr);
+
+/* fast_rap_src_dfpt_gpu_eval/generate_first_order_rho_bias_part2.cpp:9-30 (a CPU function there;
+ * same name, C linkage, 21 pointer arguments).  Runs on the GPU with the batch data that
+ * init_dfpt_device_data put on the device (which must come first); uploads density_matrix and
+ * first_order_rho_bias_part2, writes the tile's bias and copies it back.  The reference file fixes
+ * the array's tile extent at 4 (its line 7); here it is ATOM_TILE_SIZE (FRPA_ATOM_TILE_SIZE, set it
+ * to 4 for the reference's layout). */
+void generate_first_order_rho_bias_part2_(
+    int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
+    int *n_basis_, int *n_atoms_, int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_,
+    int *batch_sizes_ptr, int *n_point_batches_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *i_batch_2_wave_offset_ptr, double *wave_batches_compress_ptr,
+    double *gradient_wave_batches_compress_ptr, double *first_order_rho_bias_part2_ptr, double *partition_tab_ptr,
     double *density_matrix_ptr);
 
 /* fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.h:50-95 (C++ linkage there; C linkage here).

@@ -21,6 +21,10 @@
  *   oracle_sum_up_near               closed form sum_lm f_lm(i_r_log) Y_lm for tiles cubic in the
  *                                    grid index (exact for a cubic Hermite spline, clamped ends
  *                                    included), node values, near/far split and l reduction.
+ *   oracle_generate_first_order_rho_bias_part2
+ *                                    central differences of rho = phi^T DM phi under a displacement
+ *                                    of atom j (its basis functions move with it), empty-V zeroing,
+ *                                    n_compute_c = 0 batches untouched.
  *   oracle_evaluate_first_order_h    adjoint identity <v, rho1(P)> = <H1(v), P>, exact
  *                                    d rho1 / d P_ij = phi_i phi_j, GGA terms against an independent
  *                                    numpy restatement on tiny inputs.
@@ -85,6 +89,16 @@ void oracle_evaluate_first_order_h(
     const double *first_order_gradient_rho_batches_tile, const double *v_hartree_gradient,
     const double *first_order_rho, const double *v2rho2_batches, const double *v2rhosigma_batches,
     const double *v2sigma2_batches, const double *partition_tab);
+
+/* fast_rap_src_dfpt_gpu_eval/generate_first_order_rho_bias_part2.cpp:9-167: the bias input of rho1
+ * (d rho / d R_j of the unperturbed density for the tile's atoms); tile extent atom_tile_size. */
+void oracle_generate_first_order_rho_bias_part2(
+    int atom_tile_size, int n_my_batches_work, int j_atom_begin, int j_atom_end, int j_coord, int n_basis,
+    int n_atoms, int n_centers_basis_I, int n_max_batch_size, const int *n_point_batches,
+    const int *n_compute_c_batches, const int *i_basis_batches, const int *atom_valid_n_compute_c_batches,
+    const int *i_batch_2_wave_offset, const double *wave_batches_compress,
+    const double *gradient_wave_batches_compress, double *first_order_rho_bias_part2,
+    const double *density_matrix);
 
 /* Number of OpenMP threads used by the two grid loops (rho1 over batches, far field over points);
  * 1 reproduces the reference's single-threaded build.  Returns the previous value. */

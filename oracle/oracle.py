@@ -32,6 +32,7 @@ def lib():
         L.oracle_sum_up_far.argtypes = [i] * 12 + [p] * 16
         L.oracle_evaluate_first_order_h.argtypes = [i] * 11 + [p] * 22
         L.oracle_sum_up_near.argtypes = [i] * 9 + [p] * 15
+        L.oracle_generate_first_order_rho_bias_part2.argtypes = [i] * 9 + [p] * 9
         d = ctypes.c_double
         L.oracle_ylm.argtypes = [i, d, d, d, d, p]
         L.oracle_set_threads.argtypes = [i]
@@ -144,6 +145,22 @@ def ylm(lmax, xyz):
         sth, cth = (ab / abc, z / abc) if max(abs(x), abs(y), abs(z)) > 1e-36 else (1.0, 0.0)
         lib().oracle_ylm(lmax, sth, cth, sphi, cphi, buf.ctypes.data_as(ctypes.c_void_p))
         out[n] = buf
+    return out
+
+
+def bias2(prob, density_matrix, gwave, T, j_atom_begin, j_atom_end=None, j_coord=1, bias=None):
+    """generate_first_order_rho_bias_part2 on the CPU; returns bias (n_max_batch_size, n_batches, T) F
+    (entries of batches with n_compute_c = 0 keep the given `bias`, zeros by default)."""
+    if j_atom_end is None:
+        j_atom_end = j_atom_begin + T - 1
+    nb, nmb = prob.n_batches, prob.n_max_batch_size
+    out = np.zeros((nmb, nb, T), order="F") if bias is None else np.asfortranarray(bias).copy(order="F")
+    lib().oracle_generate_first_order_rho_bias_part2(
+        T, nb, j_atom_begin, j_atom_end, j_coord, prob.n_basis, prob.n_atoms, prob.n_centers_basis_I, nmb,
+        _p(prob.n_point_batches, np.int32), _p(prob.n_compute_c_batches, np.int32),
+        _p(prob.dense_i_basis_batches(), np.int32), _p(prob.atom_valid_n_compute_c_batches(), np.int32),
+        _p(prob.wave_offset_i32(), np.int32), _p(prob.wave, np.float64), _p(gwave, np.float64), _p(out, np.float64),
+        _p(np.asfortranarray(density_matrix, dtype=np.float64), np.float64))
     return out
 
 

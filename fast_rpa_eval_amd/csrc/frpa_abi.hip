@@ -43,17 +43,50 @@ void fill_i32(int *p, size_t n, int v, hipStream_t s) {
   check_launch("fill_i32_kernel");
 }
 
-LaunchArgs launch_args(Rho1Plan &P, const LaunchCfg &cfg, int T, int blocks_per_cu, hipStream_t s) {
-  LaunchArgs a{nullptr, nullptr, dim3(P.n_valid, T)};
+LaunchArgs launch_args(Rho1Plan &P, const LaunchCfg &cfg, int T, int blocks_per_cu, hipStream_t s, int n_pos) {
+  if (n_pos < 0)
+    n_pos = P.n_valid;
+  LaunchArgs a{nullptr, nullptr, dim3(n_pos, T)};
   if (cfg.order == 1)
     a.order = P.d_order_spatial.ptr;
+  else if (cfg.order == 2)
+    a.order = P.d_order_items.ptr;
   const int grid = cfg.persistent >= N_XCD ? cfg.persistent : std::max(1, P.n_cu) * blocks_per_cu;
-  if (cfg.persistent && grid >= N_XCD && (long long)grid < (long long)P.n_valid * T) {
+  if (cfg.persistent && grid >= N_XCD && (long long)grid < (long long)n_pos * T) {
     fill_i32(P.d_counters.ptr, N_XCD, 0, s);
     a.counters = P.d_counters.ptr;
     a.grid = dim3(grid / N_XCD * N_XCD, 1);
   }
   return a;
+}
+
+void build_item_order(Rho1Plan &P, int pch) {
+  if (P.items_pch == pch)
+    return;
+  FRPA_REQUIRE(P.max_np <= 255 * pch, FRPA_ERR_UNSUPPORTED, "order 2: a batch of %d points has more than 255 passes",
+               P.max_np);
+  // items in the caller's batch order, the passes of a batch consecutive; cut into N_XCD contiguous
+  // ranges and interleaved like the spatial order, so a batch's passes are claimed back to back on
+  // one XCD and stream the same P' tiles through its L2
+  std::vector<int> caller(P.n_valid);
+  std::iota(caller.begin(), caller.end(), 0);
+  std::stable_sort(caller.begin(), caller.end(), [&](int a, int b) { return P.h_desc[a].b < P.h_desc[b].b; });
+  std::vector<int> seq;
+  for (int di : caller)
+    for (int q = 0; q * pch < P.h_desc[di].np; q++)
+      seq.push_back((di << 8) | q);
+  const int n = (int)seq.size();
+  std::vector<int> order(std::max(n, 1), 0);
+  int start = 0;
+  for (int x = 0; x < N_XCD; x++) {
+    const int len = n / N_XCD + (x < n % N_XCD ? 1 : 0);
+    for (int k = 0; k < len; k++)
+      order[N_XCD * k + x] = seq[start + k];
+    start += len;
+  }
+  P.d_order_items.upload(order);
+  P.n_items = n;
+  P.items_pch = pch;
 }
 
 void fill_f64(double *p, size_t n, double v, hipStream_t s) {
@@ -405,7 +438,8 @@ int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int per
     FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
     FRPA_REQUIRE(precision == FRPA_PREC_FP64 || precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4,
                  FRPA_ERR_INVALID, "unknown precision %d", precision);
-    FRPA_REQUIRE(order == 0 || order == 1, FRPA_ERR_INVALID, "order must be 0 or 1");
+    FRPA_REQUIRE(order == 0 || order == 1 || (order == 2 && precision != FRPA_PREC_FP64), FRPA_ERR_INVALID,
+                 "order must be 0 or 1 (or 2 for the INT8 split)");
     FRPA_REQUIRE(persistent == 0 || persistent == 1 || persistent >= N_XCD, FRPA_ERR_INVALID,
                  "persistent must be 0, 1 or a grid of at least %d workgroups", N_XCD);
     const bool ok = points_per_wave == 0 ||
@@ -439,7 +473,8 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
     else
       npgs = {0};
     std::vector<LaunchCfg> cands;
-    for (int order = 0; order < 2; order++)
+    const int n_orders = precision == FRPA_PREC_FP64 ? 2 : 3;  // order 2: point passes as items (INT8)
+    for (int order = 0; order < n_orders; order++)
       for (int pers = 0; pers < 2; pers++)
         for (int npg : npgs)
           cands.push_back(LaunchCfg{order, pers, npg});

@@ -59,6 +59,10 @@ struct Rho1Plan {
   AdjOzState *adjoz = nullptr;
   Relabel *relabel = nullptr;       // built on first use (get_relabel)
   DevBuf<int> d_order_spatial;      // launch position -> BatchDesc index, order 1
+  // order 2 (INT8 forward): the spatial order with every point pass of PCH points a work item of
+  // its own, (BatchDesc index << 8) | pass; the passes of a batch are adjacent in one XCD's range
+  DevBuf<int> d_order_items;
+  int n_items = 0, items_pch = 0;
   DevBuf<int> d_counters;           // N_XCD work counters of the persistent mode
   int n_cu = 0;                     // compute units of the device the plan was made on
   LaunchCfg cfg[3];                 // per precision (FP64, INT8 split S = 5, S = 4)
@@ -86,7 +90,10 @@ struct LaunchArgs {
   int *counters;
   dim3 grid;
 };
-LaunchArgs launch_args(Rho1Plan &P, const LaunchCfg &cfg, int T, int blocks_per_cu, hipStream_t s);
+// n_pos: launch positions per perturbation (-1: one per valid batch; order 2 passes n_items)
+LaunchArgs launch_args(Rho1Plan &P, const LaunchCfg &cfg, int T, int blocks_per_cu, hipStream_t s, int n_pos = -1);
+// order 2 position map for point passes of pch points (built on first use, rebuilt if pch changes)
+void build_item_order(Rho1Plan &P, int pch);
 void fill_i32(int *p, size_t n, int v, hipStream_t s);
 
 // Persistent-mode work claiming: item k of XCD group x is launch position 8 (k % len) + x of

@@ -407,7 +407,8 @@ __global__ __launch_bounds__(OZ_THREADS) void rho1_ozaki_kernel(
     const int *__restrict__ sib_all, const int *__restrict__ map, const unsigned long long *__restrict__ adig,
     const double *__restrict__ ainv, const signed char *__restrict__ bdig,
     const long long *__restrict__ boff, const double *__restrict__ binv, int n_basis,
-    const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all, int n_tile) {
+    const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all, int n_tile,
+    int items) {
   constexpr int PCH = 32 * NPG;
   __shared__ __attribute__((aligned(16))) signed char Bs[3 * S * PCH * BROW];
   __shared__ int jbs[192];
@@ -422,13 +423,15 @@ __global__ __launch_bounds__(OZ_THREADS) void rho1_ozaki_kernel(
     } else if (!first) {
       break;
     }
-    const int di = order ? order[pos] : pos;
+    const int it = order ? order[pos] : pos;
+    const int di = items ? it >> 8 : it;  // items: (BatchDesc index << 8) | point pass
     const BatchDesc d = descs[di];
     const int *sib = sib_all + d.ib_off;
     const size_t nn = (size_t)n_basis * n_basis * t;
     const double *ai = ainv + (size_t)n_basis * t;
     const signed char *Bg = bdig + boff[di];
-    for (int pc = 0; pc < d.np; pc += PCH) {
+    const int pbeg = items ? PCH * (it & 255) : 0, pend = items ? min(d.np, pbeg + PCH) : d.np;
+    for (int pc = pbeg; pc < pend; pc += PCH) {
       const int npc = min(PCH, d.np - pc);
       const int ngt = (npc + 15) >> 4;       // groups of 16 points in this chunk
       const int n0 = (ngt + 1) >> 1;         // the first half takes the extra group
@@ -471,11 +474,15 @@ void launch(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s)
     FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_ozaki_kernel<S, NPG>, OZ_THREADS, 0));
     per_cu = std::max(per_cu, 1);
   }
-  const LaunchArgs la = launch_args(P, cfg, c.T, per_cu, s);
+  const bool items = cfg.order == 2;
+  if (items)
+    build_item_order(P, 32 * NPG);
+  const int n_pos = items ? P.n_items : P.n_valid;
+  const LaunchArgs la = launch_args(P, cfg, c.T, per_cu, s, n_pos);
   hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG>), la.grid, dim3(OZ_THREADS), 0, s, P.d_desc.ptr, la.order,
-                     la.counters, P.n_valid, c.T, R.d_sib.ptr, P.d_map.ptr, st.d_adig.ptr, st.d_ainv.ptr,
+                     la.counters, n_pos, c.T, R.d_sib.ptr, P.d_map.ptr, st.d_adig.ptr, st.d_ainv.ptr,
                      st.d_bdig.ptr, st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full, P.nmb,
-                     P.n_batches, c.n_tile);
+                     P.n_batches, c.n_tile, (int)items);
   check_launch("rho1_ozaki_kernel");
 }
 

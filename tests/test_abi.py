@@ -24,7 +24,9 @@ def test_header_declares_reference_and_frpa_entries():
     for must in ("evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_",
                  "sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_",
                  "init_dfpt_device_info", "free_dfpt_device_info", "init_dfpt_device_data", "free_dfpt_device_data",
-                 "frpa_rho1_forward", "frpa_rho1_adjoint", "frpa_far_forward", "frpa_far_adjoint"):
+                 "generate_first_order_rho_bias_part2_", "frpa_rho1_forward", "frpa_rho1_adjoint",
+                 "frpa_rho1_bias_part2", "frpa_far_forward", "frpa_far_adjoint", "frpa_far_plan_set_near",
+                 "frpa_near_forward"):
         assert must in fns
 
 

@@ -154,8 +154,8 @@ def test_rho1_int8_split_nonsymmetric_multi_tile(cuda, oracle):
 
 @pytest.mark.parametrize("precision,npgs", [(api.FP64, (4, 8, 13)), (api.OZAKI, (4,)), (api.OZAKI4, (4, 5))])
 def test_rho1_launch_configs_bit_identical(cuda, oracle, precision, npgs):
-    """Every launch configuration the autotuner can choose (batch order, persistent grid with per-XCD
-    work counters, points per pass) gives the same bits; T = 2 exercises the persistent claim of
+    """Every launch configuration the autotuner can choose (batch order, point passes as separate
+    items, persistent grid with per-XCD work counters, points per pass) gives the same bits; T = 2 exercises the persistent claim of
     (position, perturbation) items; the autotuner keeps one of them."""
     prob = synth.make_rho1_problem(14, 12.0, 101, (1, 230), (1, 150), config_k=13, zero_nc=(4,), zero_np=(8,))
     plan = api.Rho1Plan.from_problem(prob)
@@ -168,15 +168,19 @@ def test_rho1_launch_configs_bit_identical(cuda, oracle, precision, npgs):
     oref, _ = oracle.rho1(prob, P1h, T=T)
     tol = 1e-12 if precision == api.FP64 else 1e-10 if precision == api.OZAKI else 1e-9
     _check(ref.cpu().numpy().reshape((prob.n_full_points, T), order="F"), oref, rel=tol, what="default launch")
-    for order in (0, 1):
+    orders = (0, 1) if precision == api.FP64 else (0, 1, 2)  # 2: point passes as separate items (INT8)
+    for order in orders:
         for persistent in (0, 1, 16):
             for npg in npgs:
                 plan.set_launch(precision, order, persistent, npg)
                 got = plan.forward(wave, P1, T=T, precision=precision)
                 assert torch.equal(got, ref), f"order {order} persistent {persistent} npg {npg}"
+    if precision == api.FP64:
+        with pytest.raises(RuntimeError, match="order"):
+            plan.set_launch(precision, 2, 0, 0)
     desc = plan.autotune(wave, P1, precision=precision, T=T, reps=1)
     print(f"[autotune precision {precision}] {desc}", flush=True)
-    assert desc.count("order=") == 2 * 2 * len(npgs)
+    assert desc.count("order=") == len(orders) * 2 * len(npgs)
     assert torch.equal(plan.forward(wave, P1, T=T, precision=precision), ref)
 
 

@@ -198,6 +198,100 @@ def bench_bias2(D, steps, warmup, no_cpu, threads, T=4, jc=1):
     return res
 
 
+H1_WORKLOAD = "rho1_512atoms_20kbatches_far200k_l6"
+
+
+def bench_h1_gga(steps, warmup, no_cpu, threads, jb=1, jc=1):
+    """Full first-order H (LDA + GGA terms of evaluate_first_order_H) for atom jb, coordinate jc, on the
+    512-atom, 20k-batch workload: wave synthesised on the device, random gradient and hessian waves and
+    XC arrays.  HIP events on torch's current stream around the whole call (Hs fill, kernel, unrelabel)."""
+    import torch
+
+    from fast_rpa_eval_amd import api, synth
+    w3 = WORKLOADS[H1_WORKLOAD]
+    t0 = time.time()
+    prob = synth.make_rho1_problem(w3["n_atoms"], w3["box_ang"], w3["n_batches"], w3["np_range"], w3["nc_range"],
+                                   config_k=w3["config_k"], fill_wave=False)
+    plan = api.Rho1Plan.from_problem(prob)
+    wl = int(prob.wave_offset[-1])
+    wave = plan.synth_wave(prob.atom_xyz, prob.basis, api.to_device(prob.coord_points), wl)
+    g = torch.Generator(device="cuda").manual_seed(29)
+    U = lambda n, lo=-1.0, hi=1.0: torch.empty(n, dtype=torch.float64, device="cuda").uniform_(lo, hi, generator=g)
+    nmb, nb, nf = prob.n_max_batch_size, prob.n_batches, prob.n_full_points
+    gwave, hwave = U(3 * wl), U(6 * wl)
+    xc = dict(vsigma=U(3 * nmb * nb), v2rho2=U(3 * nmb * nb), v2rhosigma=U(6 * nmb * nb), v2sigma2=U(6 * nmb * nb),
+              gradient_rho=U(3 * nf), H_prefactor1=U(nmb * nb), grad_rho1=U(3 * nmb * nb),
+              v_hartree_gradient=U(nf), first_order_rho=U(nf), partition_tab=U(nf, 0.1, 1.0))
+    avn_h = prob.atom_valid_n_compute_c_batches()
+    avn = torch.from_numpy(np.ascontiguousarray(avn_h.ravel(order="F"))).cuda()
+    H = torch.zeros(prob.n_basis * prob.n_basis, dtype=torch.float64, device="cuda")
+    log(f"H1 GGA inputs ready: {nb} batches, {prob.n_basis} basis, waves {(1 + 3 + 6) * wl * 8 / 1e9:.1f} GB "
+        f"({time.time() - t0:.1f}s)")
+
+    def run():
+        plan.h1(wave, gwave, hwave, xc, avn, prob.n_atoms, jb, n_tile=1, j_coord=jc, H=H)
+    for _ in range(max(warmup, 1)):
+        run()
+    torch.cuda.synchronize()
+    ev = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run()
+        e1.record()
+        ev.append((e0, e1))
+    torch.cuda.synchronize()
+    ms = float(np.mean([x.elapsed_time(y) for x, y in ev]))
+    # algorithmic work per batch: the two products C phi^T + phi C^T on the upper triangle (4 nc(nc+1)/2 np flop)
+    # plus C (8 flop per (row, point), 6 more on the atom's rows); bytes: phi, dphi (3), the atom's rows of the
+    # hessian (3 of 6 components), ~100 B of XC data per point, the atomics' read-modify-write of the triangle
+    nc, npt = _nc_np(prob)
+    sel = (prob.n_compute_c_batches > 0) & (prob.n_point_batches > 0)
+    nv = (avn_h[jb, :] - avn_h[jb - 1, :])[sel].astype(np.float64)
+    tri = nc * (nc + 1) / 2
+    flops = float(np.sum(4 * tri * npt + 8 * nc * npt + 6 * nv * npt))
+    nbytes = float(np.sum(8 * nc * npt + 24 * nc * npt + 24 * nv * npt + 100 * npt + 16 * tri))
+    ach = flops / (ms / 1e3) / 1e12
+    res = {"workload": f"{H1_WORKLOAD}: {nb} batches, {prob.n_basis} basis functions, atom {jb}, j_coord {jc}",
+           "ms": ms, "points_per_s": float(npt.sum()) / (ms / 1e3),
+           "roofline": {"kernel": "frpa::h1_gga_kernel", "bound": "mfma", "achieved": ach, "peak": PEAK_FP64,
+                        "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "algorithmic_flops": flops,
+                        "algorithmic_bytes": nbytes, "hbm_frac": nbytes / (ms / 1e3) / 1e9 / PEAK_HBM}}
+    if not no_cpu:
+        from oracle import oracle
+        rng = np.random.default_rng(31)
+        pick = np.sort(rng.choice(np.nonzero(sel)[0], size=min(200, int(sel.sum())), replace=False))
+        sub = subproblem(prob, pick, wave)
+        nmb2 = sub.n_max_batch_size
+        blocks = lambda a, k: np.concatenate([a[k * int(prob.wave_offset[b]):k * int(prob.wave_offset[b + 1])]
+                                              .cpu().numpy() for b in pick])
+        per_batch = lambda a, k: np.concatenate([a[k * nmb * b:k * nmb * (b + 1)].cpu().numpy()
+                                                 .reshape(k, nmb, order="F")[:, :nmb2].ravel(order="F") for b in pick])
+        arr = dict(vrho=np.zeros(nmb2 * len(pick)), vsigma=per_batch(xc["vsigma"], 3),
+                   v2rho2=per_batch(xc["v2rho2"], 3), v2rhosigma=per_batch(xc["v2rhosigma"], 6),
+                   v2sigma2=per_batch(xc["v2sigma2"], 6),
+                   gradient_rho=xc["gradient_rho"].cpu().numpy().reshape(3, nf, order="F"),
+                   H_prefactor1=per_batch(xc["H_prefactor1"], 1).reshape(nmb2, len(pick), 1, order="F"),
+                   grad_rho1=per_batch(xc["grad_rho1"], 3).reshape(3, nmb2, len(pick), 1, order="F"),
+                   v_hartree_gradient=xc["v_hartree_gradient"].cpu().numpy().reshape(nf, 1),
+                   first_order_rho=xc["first_order_rho"].cpu().numpy().reshape(nf, 1),
+                   partition_tab=xc["partition_tab"].cpu().numpy(), gwave=blocks(gwave, 3), hwave=blocks(hwave, 6))
+        oracle.set_threads(threads)
+        t1 = time.time()
+        ref = oracle.h1(sub, 1, jb, jb, jc, np.zeros((prob.n_basis, prob.n_basis, 1), order="F"), arr)
+        cpu_s = time.time() - t1
+        splan = api.Rho1Plan.from_problem(sub)
+        dxc = {k: api.to_device(v) for k, v in arr.items() if k not in ("vrho", "gwave", "hwave")}
+        Hs = splan.h1(api.to_device(sub.wave), api.to_device(arr["gwave"]), api.to_device(arr["hwave"]), dxc,
+                      api.to_device(sub.atom_valid_n_compute_c_batches()), prob.n_atoms, jb, j_coord=jc)
+        got = Hs.cpu().numpy().reshape(prob.n_basis, prob.n_basis, order="F")
+        res["vs_oracle_rel"] = float(np.abs(got - ref[:, :, 0]).max() / max(np.abs(ref).max(), 1e-300))
+        res["cpu_baseline"] = {"value": float(sub.n_point_batches.sum()) / cpu_s, "unit": "points/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{len(pick)} random batches ({int(sub.n_point_batches.sum())} points)"}
+    return res
+
+
 def time_near(N, steps, warmup):
     """Near-field launch timed with HIP events on torch's current stream (the stream it runs on)."""
     import torch
@@ -519,6 +613,9 @@ def main():
     ap.add_argument("--bias2", type=int, default=1,
                     help="1: also time the bias generator (generate_first_order_rho_bias_part2) for the tile of "
                          "atoms 1-4 over all batches (reported as bias_part2, outside the step)")
+    ap.add_argument("--h1-gga", type=int, default=1,
+                    help="1: also time the full first-order H (LDA + GGA terms) on the 512-atom, 20k-batch workload "
+                         "(reported as h1_gga, outside the step; run last, after the step's inputs are released)")
     ap.add_argument("--kernels", default="rho1,adjoint,far",
                     help="subset of the step to time (a partial step is not a bench line)")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -734,6 +831,8 @@ def main():
             near["vs_oracle_rel"] = float(np.abs(got - ref[mp]).max() / np.abs(ref[mp]).max())
             near["cpu_baseline"] = {"value": len(sel) / cpu_s, "unit": "points/s", "cores": a.cpu_threads,
                                     "kind": "port", "sample": f"{len(sel)} random points of the grid"}
+        tr = load_traffic(a.workload) or {}
+        near["roofline"]["traffic"] = next((v for k, v in tr.items() if k.startswith("frpa::near_kernel")), None)
         line["near_field"] = near
         log(f"near field: {ms:.3f} ms, {npairs} pairs, {near['pairs_per_s'] / 1e9:.2f} G pairs/s, "
             f"{ach:.1f} TF/s ({100 * ach / PEAK_FP64:.0f}% FP64)")
@@ -742,6 +841,7 @@ def main():
         try:
             line["bias_part2"] = bench_bias2(D, a.steps, a.warmup, a.no_cpu, a.cpu_threads)
             b2 = line["bias_part2"]
+            b2["roofline"]["traffic"] = (load_traffic(a.workload) or {}).get("frpa::bias2_kernel")
             log(f"bias part 2: {b2['ms']:.3f} ms, {b2['roofline']['achieved']:.0f} GB/s "
                 f"({100 * b2['roofline']['frac']:.0f}% HBM)")
         except torch.cuda.OutOfMemoryError as e:
@@ -772,6 +872,19 @@ def main():
             for p in outs:
                 dg = outs[p]["dv"].cpu().numpy()[mpf]
                 line[f"far_{p}_vs_oracle_rel"] = float(np.abs(dg - dref).max() / np.abs(dref).max())
+    if a.h1_gga:
+        # the 60k-batch inputs (and the step closures holding their plans) are released first: the
+        # gradient and hessian waves are 9x the wave
+        fns = None
+        D.clear()
+        import gc
+        gc.collect()
+        torch.cuda.empty_cache()
+        try:
+            line["h1_gga"] = bench_h1_gga(a.steps, a.warmup, a.no_cpu, a.cpu_threads)
+            log(f"H1 GGA: {line['h1_gga']['ms']:.3f} ms, {line['h1_gga']['roofline']['achieved']:.1f} TF/s")
+        except torch.cuda.OutOfMemoryError as e:
+            log(f"H1 GGA skipped: {e}")
     print(json.dumps(line), flush=True)
 
 

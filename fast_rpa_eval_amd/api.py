@@ -139,6 +139,23 @@ class Rho1Plan:
             _dp(gradient_wave), _dp(density_matrix), _dp(out), _stream(stream)), "frpa_rho1_bias_part2")
         return out
 
+    def h1(self, wave, gradient_wave, hessian_wave, xc, atom_valid_n_compute_c_batches, n_atoms, j_atom_begin,
+           n_tile=1, j_coord=1, T=None, H=None, stream=None):
+        """Full first-order H (LDA + GGA terms of evaluate_first_order_H), accumulated into H
+        (n_basis^2 * T, zeros if None).  xc: dict of device tensors vsigma, gradient_rho, H_prefactor1,
+        grad_rho1, v_hartree_gradient, first_order_rho, v2rho2, v2rhosigma, v2sigma2, partition_tab in the
+        reference's layouts (see frpa_rho1_h1)."""
+        T = n_tile if T is None else T
+        if H is None:
+            H = torch.zeros(self.n_basis * self.n_basis * T, dtype=torch.float64, device="cuda")
+        check(lib().frpa_rho1_h1(
+            self.handle, T, j_atom_begin, n_tile, j_coord, n_atoms, _dp(atom_valid_n_compute_c_batches), _dp(wave),
+            _dp(gradient_wave), _dp(hessian_wave), _dp(xc["vsigma"]), _dp(xc["gradient_rho"]), _dp(xc["H_prefactor1"]),
+            _dp(xc["grad_rho1"]), _dp(xc["v_hartree_gradient"]), _dp(xc["first_order_rho"]), _dp(xc["v2rho2"]),
+            _dp(xc["v2rhosigma"]), _dp(xc["v2sigma2"]), _dp(xc["partition_tab"]), _dp(H), _stream(stream)),
+            "frpa_rho1_h1")
+        return H
+
     def prepare_wave(self, wave, precision=OZAKI, stream=None):
         b = ctypes.c_longlong(0)
         check(lib().frpa_rho1_prepare_wave(self.handle, precision, _dp(wave), ctypes.byref(b), _stream(stream)),

@@ -629,6 +629,23 @@ int frpa_rho1_bias_part2(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_be
   });
 }
 
+int frpa_rho1_h1(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord, int n_atoms,
+                 const int *atom_valid_n_compute_c_batches, const double *wave, const double *gradient_wave,
+                 const double *hessian_wave, const double *vsigma_batches, const double *gradient_rho,
+                 const double *H_prefactor1_batches_tile, const double *first_order_gradient_rho_batches_tile,
+                 const double *v_hartree_gradient, const double *first_order_rho, const double *v2rho2_batches,
+                 const double *v2rhosigma_batches, const double *v2sigma2_batches, const double *partition_tab,
+                 double *first_order_H_tile, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
+    rho1_h1_gga(*reinterpret_cast<Rho1Plan *>(plan), atom_tile_size, j_atom_begin, n_tile, j_coord, n_atoms,
+                atom_valid_n_compute_c_batches, wave, gradient_wave, hessian_wave, vsigma_batches, gradient_rho,
+                H_prefactor1_batches_tile, first_order_gradient_rho_batches_tile, v_hartree_gradient, first_order_rho,
+                v2rho2_batches, v2rhosigma_batches, v2sigma2_batches, partition_tab, first_order_H_tile,
+                as_stream(stream));
+  });
+}
+
 int frpa_far_plan_set_near(frpa_far_plan *plan, int n_species, const int *species_center, const int *n_grid,
                            const double *r_grid_min, const double *log_r_grid_inc) {
   return guarded([&] {

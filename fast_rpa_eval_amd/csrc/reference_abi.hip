@@ -28,6 +28,8 @@ struct RefState {
   std::unique_ptr<Rho1Plan> rho1;
   std::unique_ptr<FarPlan> far;
   DevBuf<double> wave, gwave, P1, bias, gbias, rho, grad, dm;
+  // per-call arrays of the first-order H entry point
+  DevBuf<double> h_hw, h_vs, h_v2rr, h_v2rs, h_v2ss, h_grho, h_pref1, h_grho1, h_vhg, h_rho1, h_w, h_H;
   DevBuf<int> avn;  // atom_valid_n_compute_c_batches (n_atoms + 1, n_batches)
   DevBuf<double> coord_points;
   // far per-call buffers
@@ -170,6 +172,9 @@ void free_dfpt_device_data(void) {
   S.gbias.release();
   S.dm.release();
   S.avn.release();
+  for (DevBuf<double> *b : {&S.h_hw, &S.h_vs, &S.h_v2rr, &S.h_v2rs, &S.h_v2ss, &S.h_grho, &S.h_pref1, &S.h_grho1,
+                            &S.h_vhg, &S.h_rho1, &S.h_w, &S.h_H})
+    b->release();
   S.rho.release();
   S.grad.release();
   S.coord_points.release();
@@ -251,6 +256,58 @@ void generate_first_order_rho_bias_part2_(
                                         S.bias.ptr, s);
     FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
     FRPA_HIP(hipMemcpyAsync(first_order_rho_bias_part2_ptr, S.bias.ptr, nbt * 8, hipMemcpyDeviceToHost, s));
+    FRPA_HIP(hipStreamSynchronize(s));
+  });
+}
+
+void evaluate_first_order_h_reduce_memory_c_v2_(
+    int *j_atom_begin_, int *j_atom_end_, int *n_full_points_, int *j_coord_, int *n_basis_, int *n_atoms_,
+    int *n_max_compute_ham_, int *n_max_batch_size_, int *n_my_batches_work_, int *n_centers_basis_I_,
+    int *n_point_batches_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *n_point_batches_prefix_sum_ptr,
+    int *i_valid_point_2_i_full_points_map_ptr, double *vrho_batches_ptr, double *vsigma_batches_ptr,
+    double *gradient_rho_ptr, double *H_times_psi_batches_compress_ptr, int *i_batch_2_wave_offset_ptr,
+    double *wave_batches_compress_ptr, double *gradient_wave_batches_compress_ptr,
+    double *hessian_wave_batches_compress_ptr, double *first_order_H_tile_ptr, double *H_prefactor1_batches_tile_ptr,
+    double *first_order_gradient_rho_batches_tile_ptr, double *v_hartree_gradient_ptr, double *first_order_rho_ptr,
+    double *v2rho2_batches_ptr, double *v2rhosigma_batches_ptr, double *v2sigma2_batches_ptr,
+    double *partition_tab_ptr, int *not_use) {
+  run_or_die("evaluate_first_order_h_reduce_memory_c_v2_", [&] {
+    RefState &S = ref();
+    FRPA_REQUIRE(S.data && S.rho1, FRPA_ERR_STATE,
+                 "init_dfpt_device_data must be called first (the wave arrays are used from the device)");
+    FRPA_REQUIRE(*n_my_batches_work_ == S.n_batches && *n_full_points_ == S.n_full && *n_basis_ == S.n_basis &&
+                     *n_max_batch_size_ == S.nmb && *n_atoms_ == S.n_atoms,
+                 FRPA_ERR_INVALID, "sizes differ from init_dfpt_device_data");
+    const int n_tile = *j_atom_end_ - *j_atom_begin_ + 1;
+    FRPA_REQUIRE(n_tile >= 1 && n_tile <= S.T, FRPA_ERR_INVALID,
+                 "j_atom_end - j_atom_begin + 1 = %d outside [1, ATOM_TILE_SIZE = %d]", n_tile, S.T);
+    hipStream_t s = S.stream;
+    const size_t nb = S.n_batches, nmb = S.nmb, nf = S.n_full, T = S.T, nbas = S.n_basis;
+    const size_t wl = S.wave.n;
+    auto up = [&](DevBuf<double> &b, const double *h, size_t n) {
+      b.alloc(std::max<size_t>(n, 1));
+      if (n)
+        FRPA_HIP(hipMemcpyAsync(b.ptr, h, n * 8, hipMemcpyHostToDevice, s));
+    };
+    up(S.h_hw, hessian_wave_batches_compress_ptr, 6 * wl);
+    up(S.h_vs, vsigma_batches_ptr, 3 * nmb * nb);
+    up(S.h_v2rr, v2rho2_batches_ptr, 3 * nmb * nb);
+    up(S.h_v2rs, v2rhosigma_batches_ptr, 6 * nmb * nb);
+    up(S.h_v2ss, v2sigma2_batches_ptr, 6 * nmb * nb);
+    up(S.h_grho, gradient_rho_ptr, 3 * nf);
+    up(S.h_pref1, H_prefactor1_batches_tile_ptr, nmb * nb * T);
+    up(S.h_grho1, first_order_gradient_rho_batches_tile_ptr, 3 * nmb * nb * T);
+    up(S.h_vhg, v_hartree_gradient_ptr, nf * T);
+    up(S.h_rho1, first_order_rho_ptr, nf * T);
+    up(S.h_w, partition_tab_ptr, nf);
+    up(S.h_H, first_order_H_tile_ptr, nbas * nbas * T);
+    const int rc = frpa_rho1_h1(reinterpret_cast<frpa_rho1_plan *>(S.rho1.get()), S.T, *j_atom_begin_, n_tile,
+                                *j_coord_, S.n_atoms, S.avn.ptr, S.wave.ptr, S.gwave.ptr, S.h_hw.ptr, S.h_vs.ptr,
+                                S.h_grho.ptr, S.h_pref1.ptr, S.h_grho1.ptr, S.h_vhg.ptr, S.h_rho1.ptr, S.h_v2rr.ptr,
+                                S.h_v2rs.ptr, S.h_v2ss.ptr, S.h_w.ptr, S.h_H.ptr, s);
+    FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
+    FRPA_HIP(hipMemcpyAsync(first_order_H_tile_ptr, S.h_H.ptr, nbas * nbas * T * 8, hipMemcpyDeviceToHost, s));
     FRPA_HIP(hipStreamSynchronize(s));
   });
 }

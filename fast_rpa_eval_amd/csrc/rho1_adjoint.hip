@@ -333,6 +333,162 @@ void build_tiles(Rho1Plan &P, AdjState &st) {
   st.tiles_built = true;
 }
 
+// Full first-order H (LDA and GGA terms), evaluate_first_order_H.cpp:144-215: for batch b and
+// perturbation t (atom j = j_atom_begin + t, its compute rows V = [avn(j-1, b), avn(j, b))), per point
+//   sigma1 = 2 grad_rho1 . grad_rho,  a = (H_prefactor1 + pref3) / 2,
+//   pref3 = w (v_hartree_gradient + v2rho2 rho1 + v2rhosigma sigma1),
+//   pref2_c = 2 w grad_rho_c (v2rhosigma rho1 + v2sigma2 sigma1) + 2 w vsigma grad_rho1_c,
+//   pref5_c = 2 vsigma w grad_rho_c,
+//   C(i, p) = a phi(i, p) + sum_c pref2_c dphi_c(i, p) - [i in V] sum_c pref5_c d2phi_{c, j_coord}(i, p),
+//   Hloc = C phi^T + phi C^T,
+// with the push kernel's structure: tile pairs I <= J of the label-sorted rows, the points as the
+// contraction index in chunks of KCH (prefactors per chunk in LDS, C built while staging), both
+// products on FP64 MFMA into one accumulator, the upper triangle added with FP64 atomics into Hs.
+struct GgaIn {
+  const double *gwave, *hwave;                            // (ncp, 3, np) / (ncp, 6, np) per batch
+  const int *avn;                                         // (n_atoms + 1, n_batches)
+  const double *vsigma, *v2rho2, *v2rhosigma, *v2sigma2;  // (3 | 3 | 6 | 6, nmb) per batch
+  const double *gradient_rho;                             // (3, n_full)
+  const double *pref1;                                    // H_prefactor1 (nmb, n_batches, T)
+  const double *grho1;                                    // first_order_gradient_rho (3, nmb, n_batches, T)
+  const double *vhg, *rho1;                               // (n_full, T)
+  const double *wpart;                                    // partition_tab (n_full)
+  int n_atoms, j_atom_begin, j_coord, nmb, n_batches;
+};
+
+__global__ __launch_bounds__(256) void h1_gga_kernel(const BatchDesc *__restrict__ descs,
+                                                     const int *__restrict__ sib_all, const int *__restrict__ srow_all,
+                                                     const int *__restrict__ map, const double *__restrict__ wave,
+                                                     const GgaIn in, int n_full, int n_basis,
+                                                     double *__restrict__ Hs) {
+  constexpr int LDT = TILE + 1;
+  __shared__ double Ci[KCH * LDT], Pi[KCH * LDT], Cj[KCH * LDT], Pj[KCH * LDT];
+  __shared__ double pk[KCH][8];  // a, pref2[3], pref5[3] of the chunk's points
+  const BatchDesc d = descs[blockIdx.x];
+  const int t = blockIdx.y;
+  const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
+  const double *W = wave + d.wave_off;
+  const double *GW = in.gwave + 3 * d.wave_off;
+  const double *HW = in.hwave + 6 * d.wave_off;
+  const int *sib = sib_all + d.ib_off;
+  const int *srow = srow_all + d.ib_off;
+  double *Ht = Hs + (size_t)n_basis * n_basis * t;
+  const int *av = in.avn + (size_t)(in.n_atoms + 1) * d.b;
+  const int v0 = min(max(av[in.j_atom_begin - 1 + t], 0), nc), v1 = min(max(av[in.j_atom_begin + t], v0), nc);
+  // hessian component (c, j_coord) in the packed (xx, xy, xz, yy, yz, zz) order
+  const int jc = in.j_coord - 1;
+  const int h0 = jc, h1 = jc == 0 ? 1 : jc == 1 ? 3 : 4, h2 = jc == 0 ? 2 : jc == 1 ? 4 : 5;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wm = 32 * (w >> 1), wn = 32 * (w & 1);
+  const int nT = (nc + TILE - 1) / TILE;
+  const int m = tid % TILE;
+  const size_t bt = (size_t)d.b + (size_t)in.n_batches * t;
+  // C(row r, point p) from the chunk's prefactors (k = p - p0)
+  auto cval = [&](int r, int p, int k, double phi) {
+    const double *g = GW + r + (size_t)3 * ld * p;
+    double c = pk[k][0] * phi + pk[k][1] * g[0] + pk[k][2] * g[ld] + pk[k][3] * g[2 * ld];
+    if (r >= v0 && r < v1) {
+      const double *h = HW + r + (size_t)6 * ld * p;
+      c -= pk[k][4] * h[(size_t)ld * h0] + pk[k][5] * h[(size_t)ld * h1] + pk[k][6] * h[(size_t)ld * h2];
+    }
+    return c;
+  };
+  for (int I = 0; I < nT; I++) {
+    const int i_st = TILE * I + m;
+    const int ri = i_st < nc ? srow[i_st] : -1;
+    for (int J = I; J < nT; J++) {
+      const int j_st = TILE * J + m;
+      const int rj = j_st < nc ? srow[j_st] : -1;
+      v4d acc[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+          acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int p0 = 0; p0 < np; p0 += KCH) {
+        __syncthreads();
+        if (tid < KCH) {
+          const int p = p0 + tid;
+          double v[7] = {0, 0, 0, 0, 0, 0, 0};
+          if (p < np) {
+            const int q = map[d.pstart + p];
+            const double wq = in.wpart[q];
+            const double r1 = in.rho1[(size_t)q + (size_t)n_full * t];
+            const double *gr = in.gradient_rho + 3 * (size_t)q;
+            const double *g1 = in.grho1 + 3 * ((size_t)in.nmb * bt + p);
+            const double sig1 = 2.0 * (g1[0] * gr[0] + g1[1] * gr[1] + g1[2] * gr[2]);
+            const size_t pb = (size_t)in.nmb * d.b + p;
+            const double vs = in.vsigma[3 * pb], vrs = in.v2rhosigma[6 * pb], vss = in.v2sigma2[6 * pb];
+            const double pref3 = wq * (in.vhg[(size_t)q + (size_t)n_full * t] + in.v2rho2[3 * pb] * r1 + vrs * sig1);
+            v[0] = 0.5 * (in.pref1[(size_t)in.nmb * bt + p] + pref3);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              v[1 + c] = 2.0 * wq * gr[c] * (vrs * r1 + vss * sig1) + 2.0 * wq * vs * g1[c];
+              v[4 + c] = 2.0 * vs * wq * gr[c];
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < 7; c++)
+            pk[tid][c] = v[c];
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int q = tid; q < TILE * KCH; q += 256) {
+          const int k = q / TILE, p = p0 + k;
+          double ci = 0.0, fi = 0.0, cj = 0.0, fj = 0.0;
+          if (p < np) {
+            if (ri >= 0) {
+              fi = W[ri + (size_t)ld * p];
+              ci = cval(ri, p, k, fi);
+            }
+            if (rj >= 0) {
+              fj = W[rj + (size_t)ld * p];
+              cj = cval(rj, p, k, fj);
+            }
+          }
+          Ci[k * LDT + m] = ci;
+          Pi[k * LDT + m] = fi;
+          Cj[k * LDT + m] = cj;
+          Pj[k * LDT + m] = fj;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KCH; kk += 4) {
+          const int o = (kk + lk) * LDT;
+          const double ca0 = Ci[o + wm + lr], ca1 = Ci[o + wm + 16 + lr];
+          const double pa0 = Pi[o + wm + lr], pa1 = Pi[o + wm + 16 + lr];
+          const double cb0 = Cj[o + wn + lr], cb1 = Cj[o + wn + 16 + lr];
+          const double pb0 = Pj[o + wn + lr], pb1 = Pj[o + wn + 16 + lr];
+          // D = (J rows) x (I rows): C(i) phi(j) + phi(i) C(j)
+          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb0, ca0, acc[0][0], 0, 0, 0);
+          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb0, pa0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb1, ca0, acc[0][1], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb1, pa0, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb0, ca1, acc[1][0], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb0, pa1, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb1, ca1, acc[1][1], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb1, pa1, acc[1][1], 0, 0, 0);
+        }
+      }
+      // acc[a][b][r] = Hloc(i = 64 I + wm + 16 a + lr, j = 64 J + wn + 16 b + 4 r + lk)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int i = TILE * I + wm + 16 * a + lr;
+        const size_t x = i < nc ? (size_t)sib[i] : 0;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int j = TILE * J + wn + 16 * b + 4 * r + lk;
+            if (i < nc && j < nc && i <= j)
+              atomicAdd(&Ht[x + (size_t)n_basis * sib[j]], acc[a][b][r]);
+          }
+      }
+    }
+  }
+}
+
 } // namespace
 
 void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const double *g, double *H, hipStream_t s) {
@@ -371,6 +527,38 @@ void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const d
                      P.d_desc.ptr, P.d_ibasis0.ptr, R.d_srow.ptr, R.d_pi.ptr, R.d_pinv.ptr, P.d_map.ptr, wave, g,
                      P.n_full, n, H);
   check_launch("h1_tile_kernel");
+}
+
+void rho1_h1_gga(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,
+                 const double *wave, const double *gwave, const double *hwave, const double *vsigma,
+                 const double *gradient_rho, const double *pref1, const double *grho1, const double *vhg,
+                 const double *rho1, const double *v2rho2, const double *v2rhosigma, const double *v2sigma2,
+                 const double *partition_tab, double *H, hipStream_t s) {
+  FRPA_REQUIRE(n_tile >= 1 && n_tile <= T, FRPA_ERR_INVALID, "H1: n_tile %d outside [1, T = %d]", n_tile, T);
+  FRPA_REQUIRE(j_coord >= 1 && j_coord <= 3, FRPA_ERR_INVALID, "H1: j_coord %d outside [1, 3]", j_coord);
+  FRPA_REQUIRE(j_atom_begin >= 1 && j_atom_begin + n_tile - 1 <= n_atoms, FRPA_ERR_INVALID,
+               "H1: atoms %d..%d outside [1, n_atoms = %d]", j_atom_begin, j_atom_begin + n_tile - 1, n_atoms);
+  if (P.n_valid == 0)
+    return;
+  FRPA_REQUIRE(wave && gwave && hwave && avn && vsigma && gradient_rho && pref1 && grho1 && vhg && rho1 && v2rho2 &&
+                   v2rhosigma && v2sigma2 && partition_tab && H,
+               FRPA_ERR_INVALID, "H1: null array");
+  if (!P.adj)
+    P.adj = new AdjState();
+  AdjState &st = *P.adj;
+  const Relabel &R = get_relabel(P);
+  const int n = P.n_basis;
+  const size_t nn = (size_t)n * n * n_tile;
+  st.d_hs.alloc(std::max(st.d_hs.n, nn));
+  fill_f64(st.d_hs.ptr, nn, 0.0, s);
+  GgaIn in{gwave, hwave, avn, vsigma, v2rho2, v2rhosigma, v2sigma2, gradient_rho, pref1, grho1, vhg, rho1,
+           partition_tab, n_atoms, j_atom_begin, j_coord, P.nmb, P.n_batches};
+  hipLaunchKernelGGL(h1_gga_kernel, dim3(P.n_valid, n_tile), dim3(256), 0, s, P.d_desc.ptr, R.d_sib.ptr, R.d_srow.ptr,
+                     P.d_map.ptr, wave, in, P.n_full, n, st.d_hs.ptr);
+  check_launch("h1_gga_kernel");
+  hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, n_tile), dim3(256), 0, s, st.d_hs.ptr,
+                     R.d_pinv.ptr, n, H);
+  check_launch("h1_unrelabel_kernel");
 }
 
 } // namespace frpa

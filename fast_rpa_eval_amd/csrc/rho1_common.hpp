@@ -127,4 +127,12 @@ void synth_wave(const Rho1Plan &P, int n_atoms, const double *atom_xyz, const in
 void rho1_bias_part2(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,
                      const double *wave, const double *gwave, const double *DM, double *bias, hipStream_t s);
 
+// Full first-order H (LDA + GGA terms of evaluate_first_order_H.cpp:13-222) for the perturbations
+// t < n_tile (atoms j_atom_begin + t), accumulated into H (n_basis, n_basis, T) (rho1_adjoint.hip).
+void rho1_h1_gga(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,
+                 const double *wave, const double *gwave, const double *hwave, const double *vsigma,
+                 const double *gradient_rho, const double *pref1, const double *grho1, const double *vhg,
+                 const double *rho1, const double *v2rho2, const double *v2rhosigma, const double *v2sigma2,
+                 const double *partition_tab, double *H, hipStream_t s);
+
 } // namespace frpa

@@ -130,6 +130,29 @@ int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wa
                            void *stream);
 
 /*
+ * Full first-order H (evaluate_first_order_H.cpp:13-222, LDA and GGA terms; a CPU function in the
+ * reference): for the perturbations t < n_tile (atom j = j_atom_begin + t, coordinate j_coord),
+ *   first_order_H(:, :, t) += sum_batches scatter(C phi^T + phi C^T),
+ *   C = (H_prefactor1 + pref3) / 2 phi + sum_c pref2_c dphi_c - [rows of atom j] sum_c pref5_c d2phi_{c, j_coord}
+ * with the reference's per-point prefactors (sigma1, pref2, pref3, pref5 of evaluate_first_order_H.cpp:
+ * 144-168).  Device arrays in the reference's layouts: atom_valid_n_compute_c_batches (n_atoms + 1,
+ * n_batches) int; wave, gradient_wave (3x), hessian_wave (6x: xx, xy, xz, yy, yz, zz) as
+ * wave_batches_compress; vsigma_batches and v2rho2_batches (3, nmb) per batch, v2rhosigma_batches and
+ * v2sigma2_batches (6, nmb) per batch (component 0 used, as in the reference); gradient_rho (3,
+ * n_full_points); H_prefactor1_batches_tile (nmb, n_batches, T); first_order_gradient_rho_batches_tile
+ * (3, nmb, n_batches, T); v_hartree_gradient, first_order_rho (n_full_points, T); partition_tab
+ * (n_full_points); first_order_H_tile (n_basis, n_basis, T) ACCUMULATED.  FP64 (MFMA).  With every XC
+ * derivative zero it equals frpa_rho1_adjoint with g = partition_tab * v_hartree_gradient + H_prefactor1.
+ */
+int frpa_rho1_h1(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord, int n_atoms,
+                 const int *atom_valid_n_compute_c_batches, const double *wave, const double *gradient_wave,
+                 const double *hessian_wave, const double *vsigma_batches, const double *gradient_rho,
+                 const double *H_prefactor1_batches_tile, const double *first_order_gradient_rho_batches_tile,
+                 const double *v_hartree_gradient, const double *first_order_rho, const double *v2rho2_batches,
+                 const double *v2rhosigma_batches, const double *v2sigma2_batches, const double *partition_tab,
+                 double *first_order_H_tile, void *stream);
+
+/*
  * Bias input of rho1 (generate_first_order_rho_bias_part2.cpp:9-167, a CPU function in the
  * reference): for the atoms j = j_atom_begin .. j_atom_begin + n_tile - 1 (t = j - j_atom_begin) and
  * every batch with n_compute_c > 0,
@@ -298,6 +321,24 @@ void generate_first_order_rho_bias_part2_(
     int *atom_valid_n_compute_c_batches_ptr, int *i_batch_2_wave_offset_ptr, double *wave_batches_compress_ptr,
     double *gradient_wave_batches_compress_ptr, double *first_order_rho_bias_part2_ptr, double *partition_tab_ptr,
     double *density_matrix_ptr);
+
+/* fast_rap_src_dfpt_gpu_eval/evaluate_first_order_H.h:3-43 (a CPU function there; same name, C
+ * linkage, 34 pointer arguments).  Runs frpa_rho1_h1 on the GPU with the wave and gradient wave that
+ * init_dfpt_device_data put on the device (which must come first); uploads the hessian wave, the
+ * XC and per-point arrays and first_order_H_tile, accumulates and copies first_order_H_tile back.
+ * vrho_batches and H_times_psi_batches_compress are not read by the reference either. */
+void evaluate_first_order_h_reduce_memory_c_v2_(
+    int *j_atom_begin_, int *j_atom_end_, int *n_full_points_, int *j_coord_, int *n_basis_, int *n_atoms_,
+    int *n_max_compute_ham_, int *n_max_batch_size_, int *n_my_batches_work_, int *n_centers_basis_I_,
+    int *n_point_batches_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *n_point_batches_prefix_sum_ptr,
+    int *i_valid_point_2_i_full_points_map_ptr, double *vrho_batches_ptr, double *vsigma_batches_ptr,
+    double *gradient_rho_ptr, double *H_times_psi_batches_compress_ptr, int *i_batch_2_wave_offset_ptr,
+    double *wave_batches_compress_ptr, double *gradient_wave_batches_compress_ptr,
+    double *hessian_wave_batches_compress_ptr, double *first_order_H_tile_ptr, double *H_prefactor1_batches_tile_ptr,
+    double *first_order_gradient_rho_batches_tile_ptr, double *v_hartree_gradient_ptr, double *first_order_rho_ptr,
+    double *v2rho2_batches_ptr, double *v2rhosigma_batches_ptr, double *v2sigma2_batches_ptr,
+    double *partition_tab_ptr, int *not_use);
 
 /* fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.h:50-95 (C++ linkage there; C linkage here).
  * Both branches, as the reference (sum_up_whole_potential.cu:641-769): uploads the per-call arrays

@@ -408,11 +408,12 @@ def work_model(prob, kernel, prec, S=None):
     rho1 FP64 (full GEMM as in the reference): 2 nc^2 np + 2 nc np flop;
         bytes 8 nc^2 (P1 gather) + 8 nc np (phi) + 4 nc + 12 np (basis list, map, output).
     rho1 INT8 split (upper tile pairs of the symmetric part): S(S+1)/2 * 2 * nc(nc+1)/2 * np int8 ops;
-        bytes 8 nc(nc+1)/2 (packed P' digits) + S nc np (phi digits) + 8 nc np (phi, FP64 ddot)
-        + 4 nc + 12 np.
+        bytes 8 nc(nc+1)/2 (packed P' digits) + S nc np (phi digits; phi(i, p) for the ddot is rebuilt
+        from them, the FP64 wave is not read) + 4 nc + 12 np.
     adjoint FP64 (upper triangle of the symmetric local H): 2 nc(nc+1)/2 np + nc np flop;
         bytes 8 nc np (phi) + 12 np + 16 nc(nc+1)/2 (read-modify-write of the scatter into H).
-    adjoint INT8 split: S(S+1)/2 * 2 * nc(nc+1)/2 * np int8 ops; bytes as FP64 plus S nc np (phi digits).
+    adjoint INT8 split: S(S+1)/2 * 2 * nc(nc+1)/2 * np int8 ops; bytes S nc np (phi digits, made once per
+        wave) + 12 np + 16 nc(nc+1)/2.
     """
     nc, npt = _nc_np(prob)
     tri = nc * (nc + 1) / 2
@@ -420,12 +421,12 @@ def work_model(prob, kernel, prec, S=None):
         return float(np.sum(2 * nc * nc * npt + 2 * nc * npt)), float(np.sum(8 * nc * nc + 8 * nc * npt + 4 * nc + 12 * npt))
     if kernel == "rho1":
         return (float(np.sum(S * (S + 1) / 2 * 2 * tri * npt)),
-                float(np.sum(8 * tri + S * nc * npt + 8 * nc * npt + 4 * nc + 12 * npt)))
+                float(np.sum(8 * tri + S * nc * npt + 4 * nc + 12 * npt)))
     if kernel == "adjoint" and S is None:
         return float(np.sum(2 * tri * npt + nc * npt)), float(np.sum(8 * nc * npt + 12 * npt + 16 * tri))
     if kernel == "adjoint":
         return (float(np.sum(S * (S + 1) / 2 * 2 * tri * npt)),
-                float(np.sum(8 * nc * npt + S * nc * npt + 12 * npt + 16 * tri)))
+                float(np.sum(S * nc * npt + 12 * npt + 16 * tri)))
     raise ValueError(kernel)
 
 

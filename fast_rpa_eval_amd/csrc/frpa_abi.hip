@@ -214,6 +214,7 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
     P->max_nc = std::max(P->max_nc, d.nc);
   }
   P->n_valid = (int)desc.size();
+  P->wave_off_quads = std::all_of(desc.begin(), desc.end(), [](const BatchDesc &d) { return d.wave_off % 4 == 0; });
   P->h_desc = desc;
   P->d_desc.upload(desc);
   {

@@ -65,6 +65,12 @@ struct Rho1Plan {
   int n_items = 0, items_pch = 0;
   DevBuf<int> d_counters;           // N_XCD work counters of the persistent mode
   int n_cu = 0;                     // compute units of the device the plan was made on
+  bool wave_off_quads = false;      // every BatchDesc::wave_off a multiple of 4 doubles
+  // bias part 2 (rho1_bias2.hip): DM + DM^T on the rows of the tile's atoms, rebuilt per call
+  DevBuf<int> d_b2_rowmap;          // n_basis: local row of a tile basis function, -1 otherwise
+  DevBuf<int> d_b2_rows;            // 1 + n_basis: [0] = number of rows R, then their basis indices
+  DevBuf<double> d_b2_tab;          // (b2_cap, n_basis): column c holds the R rows at c
+  int b2_cap = 0;
   LaunchCfg cfg[3];                 // per precision (FP64, INT8 split S = 5, S = 4)
   ~Rho1Plan();
 };

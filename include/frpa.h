@@ -162,6 +162,10 @@ int frpa_rho1_h1(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int
  * Device arrays: atom_valid_n_compute_c_batches (n_atoms + 1, n_batches) int, wave and gradient_wave
  * as for frpa_rho1_forward, density_matrix (n_basis, n_basis), first_order_rho_bias_part2
  * (n_max_batch_size, n_batches, atom_tile_size).  n_tile <= min(atom_tile_size, 8), j_coord 1..3.
+ * Each call first writes DM(row, c) + DM(c, row) for the tile's row functions into a plan-owned
+ * table (allocated on the first call: 256 x n_basis doubles, capped at 1 GiB; environment variable
+ * FRPA_BIAS2_TABLE_ROWS sets the row capacity, 0 = none); a tile with more rows than that gathers
+ * from density_matrix directly, with the same result bits.
  */
 int frpa_rho1_bias_part2(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord,
                          int n_atoms, const int *atom_valid_n_compute_c_batches, const double *wave,

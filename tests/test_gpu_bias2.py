@@ -1,11 +1,13 @@
 """Bias input of rho1 (generate_first_order_rho_bias_part2) on the GPU against the CPU oracle.
 
-Bar: within 1e-12 of max|bias| (FP64 both sides).  Cases: a tile of 8 atoms (stacked rows over two
-64-row tiles), batches of up to 200 points (several point passes), a batch with n_compute_c = 0 and
-atoms without functions in a batch, a short tile with a nonsymmetric DM; entries the generator does
-not own keep their values; the reference-named drop-in symbol with the reference's tile extent 4.
+Bar: within 1e-12 of max|bias| (FP64 both sides).  Cases: a tile of 8 atoms (stacked rows over
+several 32-row sweeps), batches of up to 600 points (several 256-point passes), a batch with
+n_compute_c = 0 and atoms without functions in a batch, a short tile with a nonsymmetric DM; entries
+the generator does not own keep their values; the DM + DM^T table, direct DM gathers and unaligned
+wave blocks agree bit for bit; the reference-named drop-in symbol with the reference's tile extent 4.
 """
 import ctypes
+import dataclasses
 
 import numpy as np
 import pytest
@@ -16,17 +18,19 @@ from fast_rpa_eval_amd import _lib, api, synth
 pytestmark = pytest.mark.gpu
 
 
-def _setup(seed):
-    prob = synth.make_rho1_problem(16, 12.0, 60, (20, 200), (20, 150), config_k=8, zero_nc=(3,))
+def _setup(seed, np_range=(20, 200)):
+    prob = synth.make_rho1_problem(16, 12.0, 60, np_range, (20, 150), config_k=8, zero_nc=(3,))
     gw = synth.random_like_wave(prob, 3, seed)
     rng = np.random.default_rng(seed)
     A = rng.standard_normal((prob.n_basis, prob.n_basis))
     return prob, gw, A
 
 
-@pytest.mark.parametrize("T,n_tile,jb,jc,sym", [(8, 8, 1, 2, True), (3, 2, 5, 3, False)])
-def test_bias2_against_oracle(cuda, oracle, T, n_tile, jb, jc, sym):
-    prob, gw, A = _setup(T + jb)
+@pytest.mark.parametrize("T,n_tile,jb,jc,sym,np_range", [(8, 8, 1, 2, True, (20, 200)),
+                                                          (3, 2, 5, 3, False, (20, 200)),
+                                                          (4, 4, 9, 1, False, (150, 600))])
+def test_bias2_against_oracle(cuda, oracle, T, n_tile, jb, jc, sym, np_range):
+    prob, gw, A = _setup(T + jb, np_range)
     DM = (A + A.T) / 2 if sym else A
     init = np.asfortranarray(np.random.default_rng(1).uniform(-1, 1, size=(prob.n_max_batch_size, prob.n_batches, T)))
     ref = oracle.bias2(prob, DM, gw, T, jb, j_atom_end=jb + n_tile - 1, j_coord=jc, bias=init)
@@ -38,10 +42,59 @@ def test_bias2_against_oracle(cuda, oracle, T, n_tile, jb, jc, sym):
     torch.cuda.synchronize()
     got = out.cpu().numpy().reshape(init.shape, order="F")
     err = np.abs(got - ref).max() / np.abs(ref[:, :, :n_tile]).max()
-    print(f"[bias2 T={T} n_tile={n_tile} jb={jb} jc={jc} sym={sym}] rel {err:.2e}", flush=True)
+    print(f"[bias2 T={T} n_tile={n_tile} jb={jb} jc={jc} sym={sym} np_max={prob.n_point_batches.max()}] "
+          f"rel {err:.2e}", flush=True)
     assert err <= 1e-12
     assert np.array_equal(got[:, :, n_tile:], init[:, :, n_tile:])
     assert np.array_equal(got[:, 3, :], init[:, 3, :])  # n_compute_c = 0
+
+
+def _gapped(prob, gw):
+    """The same problem with one unused double after every wave block (block offsets no longer on
+    4-double boundaries); the gradient wave re-laid out to match."""
+    off = prob.wave_offset
+    noff = off + np.arange(len(off), dtype=off.dtype)
+    wave = np.full(int(noff[-1]), np.nan)
+    gwave = np.full(3 * int(noff[-1]), np.nan)
+    for b in range(prob.n_batches):
+        n = int(off[b + 1] - off[b])
+        wave[noff[b]:noff[b] + n] = prob.wave[off[b]:off[b + 1]]
+        gwave[3 * noff[b]:3 * noff[b] + 3 * n] = gw[3 * off[b]:3 * off[b + 1]]
+    return dataclasses.replace(prob, wave_offset=noff, wave=wave), gwave
+
+
+def _run(prob, gw, DM, T, n_tile, jb, jc, init):
+    plan = api.Rho1Plan.from_problem(prob)
+    out = api.to_device(init)
+    plan.bias_part2(api.to_device(prob.wave), api.to_device(gw), api.to_device(np.asfortranarray(DM)),
+                    api.to_device(prob.atom_valid_n_compute_c_batches()), prob.n_atoms, jb, n_tile=n_tile,
+                    j_coord=jc, T=T, out=out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().reshape(init.shape, order="F")
+
+
+@pytest.mark.parametrize("rows", ["0", "64"])
+def test_bias2_paths_agree_bit_for_bit(cuda, monkeypatch, rows):
+    """The DM + DM^T table, gathers straight from DM (table off, or a tile with more rows than the
+    table holds: 8 atoms, > 64 rows) and unaligned wave blocks all give the same bits."""
+    T = 8
+    prob, gw, A = _setup(5)
+    init = np.zeros((prob.n_max_batch_size, prob.n_batches, T), order="F")
+    ref = _run(prob, gw, A, T, T, 1, 2, init)
+    monkeypatch.setenv("FRPA_BIAS2_TABLE_ROWS", rows)
+    got = _run(prob, gw, A, T, T, 1, 2, init)
+    gp, ggw = _gapped(prob, gw)
+    assert gp.wave_offset[1] % 4 != 0
+    got_gapped = _run(gp, ggw, A, T, T, 1, 2, init)
+    monkeypatch.delenv("FRPA_BIAS2_TABLE_ROWS")
+    got_gapped_tab = _run(gp, ggw, A, T, T, 1, 2, init)
+    nR = prob.atom_valid_n_compute_c_batches()
+    print(f"[bias2 paths rows={rows}] max stacked rows {int((nR[T] - nR[0]).max())}, "
+          f"nonzero {int(np.count_nonzero(ref))}", flush=True)
+    assert np.count_nonzero(ref) > 0
+    assert np.array_equal(got, ref)
+    assert np.array_equal(got_gapped, ref)
+    assert np.array_equal(got_gapped_tab, ref)
 
 
 def test_bias2_reference_entry_point(cuda, oracle, monkeypatch):

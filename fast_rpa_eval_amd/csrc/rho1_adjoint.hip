@@ -53,6 +53,7 @@ struct AdjState {
   DevBuf<AdjEntry> d_entries;
   DevBuf<AdjTask> d_tasks;
   DevBuf<double> d_hs;       // push mode: upper triangle of the relabelled H, (n_basis, n_basis, T)
+  DevBuf<double> d_gga_cp;   // full first-order H: per point, C then phi of the batch's sorted rows
 };
 
 void adj_release(AdjState *st) { delete st; }
@@ -356,50 +357,89 @@ struct GgaIn {
   int n_atoms, j_atom_begin, j_coord, nmb, n_batches;
 };
 
-__global__ __launch_bounds__(256) void h1_gga_kernel(const BatchDesc *__restrict__ descs,
-                                                     const int *__restrict__ sib_all, const int *__restrict__ srow_all,
-                                                     const int *__restrict__ map, const double *__restrict__ wave,
-                                                     const GgaIn in, int n_full, int n_basis,
-                                                     double *__restrict__ Hs) {
-  constexpr int LDT = TILE + 1;
-  __shared__ double Ci[KCH * LDT], Pi[KCH * LDT], Cj[KCH * LDT], Pj[KCH * LDT];
-  __shared__ double pk[KCH][8];  // a, pref2[3], pref5[3] of the chunk's points
+// Full first-order H, pass 1: per batch, the staged operands of both products in the batch's sorted
+// row order, CP[2 wave_off + 2 ld p + k] = C(k, p) and CP[... + ld + k] = phi(k, p), k < nc, with
+//   C(r, p) = a(p) phi(r, p) + sum_c pref2_c(p) d_c phi(r, p) - [r in V] sum_c pref5_c(p) d_c d_jc phi(r, p)
+// and the per-point prefactors of evaluate_first_order_H.cpp:144-168 (computed once per point).
+__global__ __launch_bounds__(256) void h1_gga_prep_kernel(const BatchDesc *__restrict__ descs,
+                                                          const int *__restrict__ srow_all,
+                                                          const int *__restrict__ map, const double *__restrict__ wave,
+                                                          const GgaIn in, int t, int n_full, double *__restrict__ CP) {
+  __shared__ double pk[256][8];  // a, pref2[3], pref5[3] of the pass's points
   const BatchDesc d = descs[blockIdx.x];
-  const int t = blockIdx.y;
   const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
   const double *W = wave + d.wave_off;
   const double *GW = in.gwave + 3 * d.wave_off;
   const double *HW = in.hwave + 6 * d.wave_off;
-  const int *sib = sib_all + d.ib_off;
   const int *srow = srow_all + d.ib_off;
-  double *Ht = Hs + (size_t)n_basis * n_basis * t;
+  double *out = CP + 2 * d.wave_off;
   const int *av = in.avn + (size_t)(in.n_atoms + 1) * d.b;
   const int v0 = min(max(av[in.j_atom_begin - 1 + t], 0), nc), v1 = min(max(av[in.j_atom_begin + t], v0), nc);
   // hessian component (c, j_coord) in the packed (xx, xy, xz, yy, yz, zz) order
   const int jc = in.j_coord - 1;
   const int h0 = jc, h1 = jc == 0 ? 1 : jc == 1 ? 3 : 4, h2 = jc == 0 ? 2 : jc == 1 ? 4 : 5;
+  const int tid = threadIdx.x;
+  const size_t bt = (size_t)d.b + (size_t)in.n_batches * t;
+  for (int p0 = 0; p0 < np; p0 += 256) {
+    const int npp = min(256, np - p0);
+    __syncthreads();  // the previous pass is done with pk
+    if (tid < npp) {
+      const int p = p0 + tid;
+      const int q = map[d.pstart + p];
+      const double wq = in.wpart[q];
+      const double r1 = in.rho1[(size_t)q + (size_t)n_full * t];
+      const double *gr = in.gradient_rho + 3 * (size_t)q;
+      const double *g1 = in.grho1 + 3 * ((size_t)in.nmb * bt + p);
+      const double sig1 = 2.0 * (g1[0] * gr[0] + g1[1] * gr[1] + g1[2] * gr[2]);
+      const size_t pb = (size_t)in.nmb * d.b + p;
+      const double vs = in.vsigma[3 * pb], vrs = in.v2rhosigma[6 * pb], vss = in.v2sigma2[6 * pb];
+      const double pref3 = wq * (in.vhg[(size_t)q + (size_t)n_full * t] + in.v2rho2[3 * pb] * r1 + vrs * sig1);
+      pk[tid][0] = 0.5 * (in.pref1[(size_t)in.nmb * bt + p] + pref3);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        pk[tid][1 + c] = 2.0 * wq * gr[c] * (vrs * r1 + vss * sig1) + 2.0 * wq * vs * g1[c];
+        pk[tid][4 + c] = 2.0 * vs * wq * gr[c];
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < nc * npp; e += 256) {
+      const int k = e % nc, pl = e / nc, p = p0 + pl;
+      const int r = srow[k];
+      const double phi = W[r + (size_t)ld * p];
+      const double *g = GW + r + (size_t)3 * ld * p;
+      double c = pk[pl][0] * phi + pk[pl][1] * g[0] + pk[pl][2] * g[ld] + pk[pl][3] * g[2 * ld];
+      if (r >= v0 && r < v1) {
+        const double *h = HW + r + (size_t)6 * ld * p;
+        c -= pk[pl][4] * h[(size_t)ld * h0] + pk[pl][5] * h[(size_t)ld * h1] + pk[pl][6] * h[(size_t)ld * h2];
+      }
+      double *col = out + (size_t)2 * ld * p;
+      col[k] = c;
+      col[ld + k] = phi;
+    }
+  }
+}
+
+// Full first-order H, pass 2: Hloc = C phi^T + phi C^T on the tile pairs I <= J of the batch's sorted
+// rows, points as the contraction index in chunks of KCH (both products in one accumulator), the
+// upper triangle added with FP64 atomics into the relabelled scratch of perturbation t.
+__global__ __launch_bounds__(256) void h1_gga_kernel(const BatchDesc *__restrict__ descs,
+                                                     const int *__restrict__ sib_all, const double *__restrict__ CP,
+                                                     int n_basis, double *__restrict__ Ht) {
+  constexpr int LDT = TILE + 1;
+  __shared__ double Ci[KCH * LDT], Pi[KCH * LDT], Cj[KCH * LDT], Pj[KCH * LDT];
+  const BatchDesc d = descs[blockIdx.x];
+  const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
+  const double *CPb = CP + 2 * d.wave_off;
+  const int *sib = sib_all + d.ib_off;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int wm = 32 * (w >> 1), wn = 32 * (w & 1);
   const int nT = (nc + TILE - 1) / TILE;
   const int m = tid % TILE;
-  const size_t bt = (size_t)d.b + (size_t)in.n_batches * t;
-  // C(row r, point p) from the chunk's prefactors (k = p - p0)
-  auto cval = [&](int r, int p, int k, double phi) {
-    const double *g = GW + r + (size_t)3 * ld * p;
-    double c = pk[k][0] * phi + pk[k][1] * g[0] + pk[k][2] * g[ld] + pk[k][3] * g[2 * ld];
-    if (r >= v0 && r < v1) {
-      const double *h = HW + r + (size_t)6 * ld * p;
-      c -= pk[k][4] * h[(size_t)ld * h0] + pk[k][5] * h[(size_t)ld * h1] + pk[k][6] * h[(size_t)ld * h2];
-    }
-    return c;
-  };
   for (int I = 0; I < nT; I++) {
     const int i_st = TILE * I + m;
-    const int ri = i_st < nc ? srow[i_st] : -1;
     for (int J = I; J < nT; J++) {
       const int j_st = TILE * J + m;
-      const int rj = j_st < nc ? srow[j_st] : -1;
       v4d acc[2][2];
 #pragma unroll
       for (int a = 0; a < 2; a++)
@@ -407,44 +447,20 @@ __global__ __launch_bounds__(256) void h1_gga_kernel(const BatchDesc *__restrict
         for (int b = 0; b < 2; b++)
           acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
       for (int p0 = 0; p0 < np; p0 += KCH) {
-        __syncthreads();
-        if (tid < KCH) {
-          const int p = p0 + tid;
-          double v[7] = {0, 0, 0, 0, 0, 0, 0};
-          if (p < np) {
-            const int q = map[d.pstart + p];
-            const double wq = in.wpart[q];
-            const double r1 = in.rho1[(size_t)q + (size_t)n_full * t];
-            const double *gr = in.gradient_rho + 3 * (size_t)q;
-            const double *g1 = in.grho1 + 3 * ((size_t)in.nmb * bt + p);
-            const double sig1 = 2.0 * (g1[0] * gr[0] + g1[1] * gr[1] + g1[2] * gr[2]);
-            const size_t pb = (size_t)in.nmb * d.b + p;
-            const double vs = in.vsigma[3 * pb], vrs = in.v2rhosigma[6 * pb], vss = in.v2sigma2[6 * pb];
-            const double pref3 = wq * (in.vhg[(size_t)q + (size_t)n_full * t] + in.v2rho2[3 * pb] * r1 + vrs * sig1);
-            v[0] = 0.5 * (in.pref1[(size_t)in.nmb * bt + p] + pref3);
+        __syncthreads();  // the previous chunk's MFMAs are done with the staged operands
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-              v[1 + c] = 2.0 * wq * gr[c] * (vrs * r1 + vss * sig1) + 2.0 * wq * vs * g1[c];
-              v[4 + c] = 2.0 * vs * wq * gr[c];
-            }
-          }
-#pragma unroll
-          for (int c = 0; c < 7; c++)
-            pk[tid][c] = v[c];
-        }
-        __syncthreads();
-#pragma unroll 2
         for (int q = tid; q < TILE * KCH; q += 256) {
           const int k = q / TILE, p = p0 + k;
           double ci = 0.0, fi = 0.0, cj = 0.0, fj = 0.0;
           if (p < np) {
-            if (ri >= 0) {
-              fi = W[ri + (size_t)ld * p];
-              ci = cval(ri, p, k, fi);
+            const double *col = CPb + (size_t)2 * ld * p;
+            if (i_st < nc) {
+              ci = col[i_st];
+              fi = col[ld + i_st];
             }
-            if (rj >= 0) {
-              fj = W[rj + (size_t)ld * p];
-              cj = cval(rj, p, k, fj);
+            if (j_st < nc) {
+              cj = col[j_st];
+              fj = col[ld + j_st];
             }
           }
           Ci[k * LDT + m] = ci;
@@ -553,9 +569,15 @@ void rho1_h1_gga(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, 
   fill_f64(st.d_hs.ptr, nn, 0.0, s);
   GgaIn in{gwave, hwave, avn, vsigma, v2rho2, v2rhosigma, v2sigma2, gradient_rho, pref1, grho1, vhg, rho1,
            partition_tab, n_atoms, j_atom_begin, j_coord, P.nmb, P.n_batches};
-  hipLaunchKernelGGL(h1_gga_kernel, dim3(P.n_valid, n_tile), dim3(256), 0, s, P.d_desc.ptr, R.d_sib.ptr, R.d_srow.ptr,
-                     P.d_map.ptr, wave, in, P.n_full, n, st.d_hs.ptr);
-  check_launch("h1_gga_kernel");
+  st.d_gga_cp.alloc(std::max(st.d_gga_cp.n, (size_t)(2 * P.wave_len)));
+  for (int t = 0; t < n_tile; t++) {
+    hipLaunchKernelGGL(h1_gga_prep_kernel, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, R.d_srow.ptr, P.d_map.ptr,
+                       wave, in, t, P.n_full, st.d_gga_cp.ptr);
+    check_launch("h1_gga_prep_kernel");
+    hipLaunchKernelGGL(h1_gga_kernel, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, R.d_sib.ptr, st.d_gga_cp.ptr, n,
+                       st.d_hs.ptr + (size_t)n * n * t);
+    check_launch("h1_gga_kernel");
+  }
   hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, n_tile), dim3(256), 0, s, st.d_hs.ptr,
                      R.d_pinv.ptr, n, H);
   check_launch("h1_unrelabel_kernel");

@@ -143,6 +143,8 @@ int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wa
  * (3, nmb, n_batches, T); v_hartree_gradient, first_order_rho (n_full_points, T); partition_tab
  * (n_full_points); first_order_H_tile (n_basis, n_basis, T) ACCUMULATED.  FP64 (MFMA).  With every XC
  * derivative zero it equals frpa_rho1_adjoint with g = partition_tab * v_hartree_gradient + H_prefactor1.
+ * The plan keeps a device scratch of twice the wave's length (C and phi per batch, in its sorted row
+ * order; allocated on the first call) and the relabelled n_basis^2 x n_tile accumulator.
  */
 int frpa_rho1_h1(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord, int n_atoms,
                  const int *atom_valid_n_compute_c_batches, const double *wave, const double *gradient_wave,

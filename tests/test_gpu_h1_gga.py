@@ -2,7 +2,8 @@
 
 Bar: within 1e-12 of max|H1| (FP64 both sides).  Cases: random XC derivatives, gradient and hessian
 waves, T = 2 with both a full and a short tile, every j_coord, accumulation into a nonzero H, batches
-over three 64-row tiles and many 16-point chunks; with the XC derivatives zero it must equal the
+over three 64-row tiles and many 16-point chunks, batches of up to 600 points (the prefactor pass
+works in 256-point passes); with the XC derivatives zero it must equal the
 LDA adjoint frpa_rho1_adjoint (g = partition_tab * v_hartree_gradient + H_prefactor1); and the
 reference-named symbol evaluate_first_order_h_reduce_memory_c_v2_ through ctypes.
 """
@@ -17,8 +18,8 @@ from fast_rpa_eval_amd import _lib, api, synth
 pytestmark = pytest.mark.gpu
 
 
-def _problem():
-    return synth.make_rho1_problem(12, 11.0, 40, (20, 200), (30, 150), config_k=12, zero_nc=(5,))
+def _problem(np_range=(20, 200)):
+    return synth.make_rho1_problem(12, 11.0, 40, np_range, (30, 150), config_k=12, zero_nc=(5,))
 
 
 def _xc(prob, T, seed, gga=True):
@@ -40,9 +41,10 @@ def _dev_xc(a):
     return {k: api.to_device(v) for k, v in a.items() if k not in ("vrho", "gwave", "hwave")}
 
 
-@pytest.mark.parametrize("T,n_tile,jb,jc", [(2, 2, 2, 3), (2, 1, 7, 1), (1, 1, 4, 2)])
-def test_h1_gga_against_oracle(cuda, oracle, T, n_tile, jb, jc):
-    prob = _problem()
+@pytest.mark.parametrize("T,n_tile,jb,jc,np_range", [(2, 2, 2, 3, (20, 200)), (2, 1, 7, 1, (20, 200)),
+                                                     (1, 1, 4, 2, (20, 200)), (2, 2, 5, 1, (150, 600))])
+def test_h1_gga_against_oracle(cuda, oracle, T, n_tile, jb, jc, np_range):
+    prob = _problem(np_range)
     a = _xc(prob, T, 3 * jb + jc)
     H0 = np.asfortranarray(np.random.default_rng(1).uniform(-1, 1, size=(prob.n_basis, prob.n_basis, T)))
     ref = oracle.h1(prob, T, jb, jb + n_tile - 1, jc, H0.copy(order="F"), a)
@@ -54,7 +56,8 @@ def test_h1_gga_against_oracle(cuda, oracle, T, n_tile, jb, jc):
     torch.cuda.synchronize()
     got = H.cpu().numpy().reshape(H0.shape, order="F")
     err = np.abs(got - ref).max() / np.abs(ref[:, :, :n_tile] - H0[:, :, :n_tile]).max()
-    print(f"[h1 gga T={T} n_tile={n_tile} jb={jb} jc={jc}] rel {err:.2e}", flush=True)
+    print(f"[h1 gga T={T} n_tile={n_tile} jb={jb} jc={jc} np_max={prob.n_point_batches.max()}] rel {err:.2e}",
+          flush=True)
     assert err <= 1e-12
     assert np.array_equal(got[:, :, n_tile:], H0[:, :, n_tile:])
 

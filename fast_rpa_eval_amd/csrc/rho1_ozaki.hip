@@ -203,14 +203,15 @@ __device__ __forceinline__ int gather_bytes(unsigned long long e0, unsigned long
 }
 
 // ---- fused forward ------------------------------------------------------------------------
-// Block: 8 waves; wave w owns rows 16 (w & 3) .. +15 of each 64-row tile I and one half of the
-// block's point chunk (w >> 2): up to 2 * 16 * NPG points per pass, so the P' digits of a batch
-// are gathered once for batches of up to 32 NPG points.  For row tile I the K loop runs over the
+// Block: 4 H waves; wave w owns rows 16 (w & 3) .. +15 of each 64-row tile I and, for H = 2, one half
+// of the block's point chunk (w >> 2): up to H * 16 * NPG points per pass, so the P' digits of a
+// batch are gathered once for batches of up to 16 H NPG points.  H = 2 (8 waves, 128-point passes
+// at NPG = 4) fills the LDS with one block per CU; H = 1 (4 waves, 64-point passes) fits two blocks
+// per CU, whose barriers and gathers overlap each other's MFMAs.  For row tile I the K loop runs over the
 // column tiles J = I .. nT - 1 (chunks of 64 sorted rows); the phi digits of chunk J + 1 are staged
 // through registers into the other LDS buffer and the P' digits of chunk J + 1 are gathered into
 // registers while chunk J's MFMAs run (one barrier per chunk).
 constexpr int BROW = 80;  // bytes per (digit, point) row of a staged B chunk: 64 k + 16 pad
-constexpr int OZ_THREADS = 512;
 
 template <int S>
 __device__ __forceinline__ void repack_a(const unsigned long long (&e)[16], v4i (&a)[S]) {
@@ -225,7 +226,7 @@ __device__ __forceinline__ void repack_a(const unsigned long long (&e)[16], v4i 
 // LDS: three staged phi-digit chunks -- buffer 2 holds the diagonal chunk J = I for the whole row
 // tile (its digits also give phi(i, p) for the ddot, so the FP64 wave is not read), buffers 0 and 1
 // alternate for J > I.
-template <int S, int NPG, int NG>
+template <int S, int NPG, int NG, int H>
 __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *red, const BatchDesc &d, int t,
                                             int pc, int npc, int gp0, const int *__restrict__ sib,
                                             const int *__restrict__ map, const unsigned long long *__restrict__ At,
@@ -233,10 +234,11 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
                                             const signed char *__restrict__ Bg, const double *__restrict__ binv,
                                             int n_basis, const double *__restrict__ bias, double *__restrict__ rho,
                                             int n_full, int nmb, int n_batches_all, int n_tile) {
-  constexpr int PCH = 32 * NPG;                 // point slots per chunk
+  constexpr int THREADS = 256 * H;
+  constexpr int PCH = 16 * H * NPG;             // point slots per chunk
   constexpr int BUF = S * PCH * BROW;           // bytes per LDS buffer
   constexpr int NQ = S * PCH * 4;               // 16-byte pieces of one staged chunk
-  constexpr int QPT = (NQ + OZ_THREADS - 1) / OZ_THREADS;
+  constexpr int QPT = (NQ + THREADS - 1) / THREADS;
   const int nc = d.nc, ncK = ceil64(nc);
   const size_t plane = (size_t)ncK * d.np;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rg = w & 3;
@@ -252,7 +254,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
     const int j0 = 64 * c;
 #pragma unroll
     for (int u = 0; u < QPT; u++) {
-      const int e = tid + OZ_THREADS * u;
+      const int e = tid + THREADS * u;
       const int q = e & 3, p = (e >> 2) % PCH, sl = (e >> 2) / PCH;
       bq[u] = (e < NQ && p < npc)
                   ? *reinterpret_cast<const v4i *>(Bg + sl * plane + (size_t)ncK * (pc + p) + j0 + 16 * q)
@@ -263,7 +265,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
     signed char *B = Bs + buf * BUF;
 #pragma unroll
     for (int u = 0; u < QPT; u++) {
-      const int e = tid + OZ_THREADS * u;
+      const int e = tid + THREADS * u;
       const int q = e & 3, p = (e >> 2) % PCH, sl = (e >> 2) / PCH;
       if (e < NQ)
         *reinterpret_cast<v4i *>(&B[(sl * PCH + p) * BROW + 16 * q]) = bq[u];
@@ -378,7 +380,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
       rd[rg][16 * (gp0 + g) + lr] = racc[g];
   }
   __syncthreads();
-  for (int p = tid; p < npc; p += OZ_THREADS) {
+  for (int p = tid; p < npc; p += THREADS) {
     const size_t bi = (size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
     double sum = 2.0 * (rd[0][p] + rd[1][p] + rd[2][p] + rd[3][p]);
     if (bias)
@@ -388,33 +390,33 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
   }
 }
 
-template <int S, int NPG, int... Gs>
+template <int S, int NPG, int H, int... Gs>
 __device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>, int ng, signed char *Bs, int *jbs,
                                                double *red, const BatchDesc &d, int t, int pc, int npc, int gp0,
                                                const int *sib, const int *map, const unsigned long long *At,
                                                const double *ai, const signed char *Bg,
                                                const double *binv, int n_basis, const double *bias, double *rho,
                                                int n_full, int nmb, int n_batches_all, int n_tile) {
-  ((ng == Gs ? ozaki_chunk<S, NPG, Gs>(Bs, jbs, red, d, t, pc, npc, gp0, sib, map, At, ai, Bg, binv, n_basis,
+  ((ng == Gs ? ozaki_chunk<S, NPG, Gs, H>(Bs, jbs, red, d, t, pc, npc, gp0, sib, map, At, ai, Bg, binv, n_basis,
                                        bias, rho, n_full, nmb, n_batches_all, n_tile)
              : (void)0),
    ...);
 }
 
-template <int S, int NPG>
-__global__ __launch_bounds__(OZ_THREADS) void rho1_ozaki_kernel(
+template <int S, int NPG, int H>
+__global__ __launch_bounds__(256 * H, 2) void rho1_ozaki_kernel(
     const BatchDesc *__restrict__ descs, const int *__restrict__ order, int *__restrict__ counters, int n_pos, int T,
     const int *__restrict__ sib_all, const int *__restrict__ map, const unsigned long long *__restrict__ adig,
     const double *__restrict__ ainv, const signed char *__restrict__ bdig,
     const long long *__restrict__ boff, const double *__restrict__ binv, int n_basis,
     const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all, int n_tile,
     int items) {
-  constexpr int PCH = 32 * NPG;
+  constexpr int PCH = 16 * H * NPG;
   __shared__ __attribute__((aligned(16))) signed char Bs[3 * S * PCH * BROW];
   __shared__ int jbs[192];
   __shared__ double red[4 * PCH];
   __shared__ int s_item;
-  const int half = threadIdx.x >> 8;
+  const int half = H == 2 ? (int)(threadIdx.x >> 8) : 0;
   int pos = blockIdx.x, t = blockIdx.y;
   for (bool first = true;; first = false) {
     if (counters) {
@@ -434,9 +436,9 @@ __global__ __launch_bounds__(OZ_THREADS) void rho1_ozaki_kernel(
     for (int pc = pbeg; pc < pend; pc += PCH) {
       const int npc = min(PCH, d.np - pc);
       const int ngt = (npc + 15) >> 4;       // groups of 16 points in this chunk
-      const int n0 = (ngt + 1) >> 1;         // the first half takes the extra group
+      const int n0 = H == 2 ? (ngt + 1) >> 1 : ngt;  // the first half takes the extra group
       const int ng = half ? ngt - n0 : n0, gp0 = half ? n0 : 0;
-      ozaki_dispatch<S, NPG>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc, gp0, sib,
+      ozaki_dispatch<S, NPG, H>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc, gp0, sib,
                              map, adig + nn, ai, Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all, n_tile);
     }
   }
@@ -465,21 +467,21 @@ void prepare(Rho1Plan &P, OzakiState &st, const double *wave, hipStream_t s) {
   st.bytes = off[P.n_valid] + (long long)nvp * 8;
 }
 
-template <int S, int NPG>
+template <int S, int NPG, int H>
 void launch(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s) {
   OzakiState &st = *P.ozaki;
   const Relabel &R = get_relabel(P);
   static int per_cu = -1;
   if (per_cu < 0) {
-    FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_ozaki_kernel<S, NPG>, OZ_THREADS, 0));
+    FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_ozaki_kernel<S, NPG, H>, 256 * H, 0));
     per_cu = std::max(per_cu, 1);
   }
   const bool items = cfg.order == 2;
   if (items)
-    build_item_order(P, 32 * NPG);
+    build_item_order(P, 16 * H * NPG);
   const int n_pos = items ? P.n_items : P.n_valid;
   const LaunchArgs la = launch_args(P, cfg, c.T, per_cu, s, n_pos);
-  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG>), la.grid, dim3(OZ_THREADS), 0, s, P.d_desc.ptr, la.order,
+  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG, H>), la.grid, dim3(256 * H), 0, s, P.d_desc.ptr, la.order,
                      la.counters, n_pos, c.T, R.d_sib.ptr, P.d_map.ptr, st.d_adig.ptr, st.d_ainv.ptr,
                      st.d_bdig.ptr, st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full, P.nmb,
                      P.n_batches, c.n_tile, (int)items);
@@ -509,18 +511,21 @@ void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   hipLaunchKernelGGL(p1_digits_kernel<S>, dim3((n + 63) / 64, (n + 63) / 64, c.T), dim3(256), 0, s, c.P1,
                      R.d_pinv.ptr, st.d_ainv.ptr, n, st.d_adig.ptr);
   check_launch("p1_digits_kernel");
-  // point-chunk width 32 NPG: at most the widest whose accumulators, prefetched digits and staged B
-  // fit 256 registers per lane (two waves per SIMD) without spilling: 128 points for 5 digits, 160
-  // for 4 (the default when some batch has more than 128 points)
+  // point-chunk width 16 H NPG: NPG at most the widest whose accumulators, prefetched digits and
+  // staged B fit 256 registers per lane (two waves per SIMD) without spilling: 4 groups for 5 digits,
+  // 5 for 4 (the default when some batch has more than 128 points).  cfg.npg = 2 selects the
+  // four-wave block (H = 1, 64-point passes, two blocks per CU).
   const LaunchCfg &cfg = P.cfg[S == 5 ? FRPA_PREC_OZAKI : FRPA_PREC_OZAKI4];
-  if constexpr (S == 5) {
-    launch<S, 4>(P, c, cfg, s);
+  if (cfg.npg == 2) {
+    launch<S, 4, 1>(P, c, cfg, s);
+  } else if constexpr (S == 5) {
+    launch<S, 4, 2>(P, c, cfg, s);
   } else {
     const int npg = cfg.npg ? cfg.npg : (P.max_np <= 128 ? 4 : 5);
     if (npg == 4)
-      launch<S, 4>(P, c, cfg, s);
+      launch<S, 4, 2>(P, c, cfg, s);
     else
-      launch<S, 5>(P, c, cfg, s);
+      launch<S, 5, 2>(P, c, cfg, s);
   }
 }
 

@@ -118,8 +118,10 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
  * adjacent on one XCD so they share its P' gathers in L2; persistent 0 = one
  * workgroup per batch, 1 = a persistent grid sized from the occupancy, k >= 8 = a persistent grid of
  * k workgroups (rounded down to a multiple of 8); points_per_wave = groups of 16 points per wave and
- * pass (FP64: 4, 8, 13; INT8 split S = 5: 4; S = 4: 4, 5; 0 = default).  Results do not depend on
- * it.  No counterpart in the reference (compile-time tiling). */
+ * pass (FP64: 4, 8, 13; INT8 split S = 5: 4; S = 4: 4, 5; 0 = default), and for the INT8 split 2 =
+ * four-wave workgroups over 64-point passes (two per CU, where the 8-wave workgroups over 128-point
+ * passes fill a CU's LDS alone).  Results do not depend on it.  No counterpart in the reference
+ * (compile-time tiling). */
 int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave);
 
 /* One-off conversion of wave_batches_compress into the INT8 digits used by the INT8-split

@@ -18,6 +18,7 @@ sample of the same inputs).
 import argparse
 import copy
 import json
+import re
 import os
 import sys
 import time
@@ -762,6 +763,10 @@ def main():
         t = tms / 1e3
         path = "fast" if "fast" in res else "fp64"
         prof = PROFILED_KERNEL[(name, path)]
+        if name == "rho1" and path == "fast":
+            # the INT8 kernel's template arguments follow the chosen launch: npg 2 = four-wave workgroups
+            m = re.search(r"npg=(\d+)", tuned.get(path) or "")
+            prof += "<5,4,1>" if (m and int(m.group(1)) == 2) else "<5,4,2>"
         tr = next((v for k, v in traffic.items() if k.startswith(prof)), None)
         if name == "far":
             from fast_rpa_eval_amd.multipole_tables import horner_fma_count

@@ -245,13 +245,20 @@ __global__ __launch_bounds__(256) void h1_push_kernel(const BatchDesc *__restric
 }
 
 // H(pinv x, pinv y) += Hs(min(x, y), max(x, y)) for every element; 64 x 64 tiles of the relabelled
-// indices, the transposed tile read through LDS so that both reads run down columns.
+// indices, the transposed tile read through LDS so that both reads run down columns.  With bound
+// non-null Hs holds int64 fixed point with the scale fixed_scale(bound[t]) (the INT8-split adjoint).
+__device__ __forceinline__ double hs_value(const double *Hst, size_t e, double inv) {
+  return inv > 0.0 ? (double)reinterpret_cast<const long long *>(Hst)[e] * inv : Hst[e];
+}
+
 __global__ __launch_bounds__(256) void h1_unrelabel_kernel(const double *__restrict__ Hs, const int *__restrict__ pinv,
-                                                           int n_basis, double *__restrict__ H) {
+                                                           int n_basis, const double *__restrict__ bound,
+                                                           double *__restrict__ H) {
   __shared__ double tr[64][65];
   const int x0 = 64 * blockIdx.x, y0 = 64 * blockIdx.y, t = blockIdx.z;
   const size_t nn = (size_t)n_basis * n_basis;
   const double *Hst = Hs + nn * t;
+  const double inv = bound ? 1.0 / fixed_scale(bound[t]) : 0.0;  // 0: Hs holds FP64
   double *Ht = H + nn * t;
   const int a = threadIdx.x & 63, b0 = threadIdx.x >> 6;
   if (x0 > y0 || x0 == y0) {
@@ -260,7 +267,7 @@ __global__ __launch_bounds__(256) void h1_unrelabel_kernel(const double *__restr
 #pragma unroll 4
     for (int u = 0; u < 16; u++) {
       const int i = b0 + 4 * u, x = x0 + i;
-      tr[i][a] = (x < n_basis && y < n_basis) ? Hst[y + (size_t)n_basis * x] : 0.0;
+      tr[i][a] = (x < n_basis && y < n_basis) ? hs_value(Hst, y + (size_t)n_basis * x, inv) : 0.0;
     }
   }
   __syncthreads();
@@ -273,7 +280,7 @@ __global__ __launch_bounds__(256) void h1_unrelabel_kernel(const double *__restr
     const int j = b0 + 4 * u, y = y0 + j;
     if (y >= n_basis)
       break;
-    const double v = x <= y ? Hst[x + (size_t)n_basis * y] : tr[a][j];
+    const double v = x <= y ? hs_value(Hst, x + (size_t)n_basis * y, inv) : tr[a][j];
     Ht[px + (size_t)n_basis * pinv[y]] += v;
   }
 }
@@ -522,16 +529,18 @@ void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const d
   if (precision != FRPA_PREC_FP64 || !(mode && strcmp(mode, "tiles") == 0)) {
     const size_t nn = (size_t)n * n * T;
     st.d_hs.alloc(nn);
-    fill_f64(st.d_hs.ptr, nn, 0.0, s);
+    fill_f64(st.d_hs.ptr, nn, 0.0, s);  // +0.0 and int64 0 have the same bits
+    const double *bound = nullptr;      // non-null: Hs holds int64 fixed point (INT8 split)
     if (precision == FRPA_PREC_FP64) {
       hipLaunchKernelGGL(h1_push_kernel, dim3(P.n_valid, T), dim3(256), 0, s, P.d_desc.ptr, R.d_sib.ptr,
                          R.d_srow.ptr, P.d_map.ptr, wave, g, P.n_full, n, st.d_hs.ptr);
       check_launch("h1_push_kernel");
     } else {
-      h1_ozaki_push(P, T, precision == FRPA_PREC_OZAKI ? 5 : 4, wave, g, st.d_hs.ptr, s);
+      bound = h1_ozaki_push(P, T, precision == FRPA_PREC_OZAKI ? 5 : 4, wave, g,
+                            reinterpret_cast<unsigned long long *>(st.d_hs.ptr), s);
     }
     hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, T), dim3(256), 0, s, st.d_hs.ptr,
-                       R.d_pinv.ptr, n, H);
+                       R.d_pinv.ptr, n, bound, H);
     check_launch("h1_unrelabel_kernel");
     return;
   }
@@ -579,7 +588,7 @@ void rho1_h1_gga(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, 
     check_launch("h1_gga_kernel");
   }
   hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, n_tile), dim3(256), 0, s, st.d_hs.ptr,
-                     R.d_pinv.ptr, n, H);
+                     R.d_pinv.ptr, n, nullptr, H);
   check_launch("h1_unrelabel_kernel");
 }
 

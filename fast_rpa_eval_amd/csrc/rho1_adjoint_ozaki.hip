@@ -15,8 +15,14 @@
 // fragment (16 consecutive points of one row) is one 16-byte load.  The kernel computes the Y digits
 // of one 64-row tile I for all points of the pass from the phi digits and g into LDS, and
 // streams the phi digits of the tiles J >= I from global memory (two steps of register prefetch, no
-// barrier inside the J loop); the upper triangle of each tile pair is added with FP64 atomics into
-// the relabelled scratch H (see rho1_adjoint.hip).
+// barrier inside the J loop); the upper triangle of each tile pair is added into the relabelled
+// scratch H (see rho1_adjoint.hip) as 64-bit fixed point with integer atomics: value * 2^E rounded to
+// an int64, 2^E per call and perturbation from B = sum over points |g(p)| max_i phi_i(p)^2 (summed
+// in a fixed order), which bounds |H(i, k)| and every partial sum of it, so that 2 B 2^E <= 2^61 (no
+// overflow; resolution 2^-E <= 4 B / 2^62).  Integer adds are exact and commute, so H does not
+// depend on the order in which the atomics land: the INT8-split adjoint is bit-reproducible (the
+// FP64 atomics of the other adjoints are not), at the speed of the FP64 atomics.  Non-finite g or
+// phi leave H undefined.
 #include <algorithm>
 #include <vector>
 
@@ -33,6 +39,9 @@ struct AdjOzState {
   DevBuf<long long> d_off;       // per BatchDesc: byte offset of its digit planes
   DevBuf<signed char> d_fdig;    // phi digits (per wave)
   DevBuf<double> d_finv;         // 1 / sf per sorted row (compact, at ib_off)
+  DevBuf<double> d_pmax;         // max_i |phi_i(p)| per valid point (pstart + p)
+  DevBuf<double> d_bound;        // per call: B per perturbation
+  DevBuf<double> d_bpart;        // per call: B's partial sums, one per workgroup of h1_bound_kernel
 };
 
 void adjoz_release(AdjOzState *st) { delete st; }
@@ -103,6 +112,65 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const BatchDesc *__restr
   }
 }
 
+// max_i |phi_i(p)| over the batch's rows for every point (once per wave): one wave per point, lanes
+// down the point's column.
+__global__ __launch_bounds__(256) void h1_pointmax_kernel(const BatchDesc *__restrict__ descs,
+                                                          const double *__restrict__ wave, double *__restrict__ pmax) {
+  const BatchDesc d = descs[blockIdx.x];
+  const int ld = (d.nc + 3) & ~3;
+  const double *W = wave + d.wave_off;
+  const int lane = threadIdx.x & 63;
+  for (int p = threadIdx.x >> 6; p < d.np; p += 4) {
+    double mx = 0.0;
+    for (int r = lane; r < d.nc; r += 64)
+      mx = fmax(mx, fabs(W[r + (size_t)ld * p]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+      mx = fmax(mx, __shfl_xor(mx, o));
+    if (lane == 0)
+      pmax[d.pstart + p] = mx;
+  }
+}
+
+// part[t][block] = sum over the block's valid points q of |g(map(q), t)| pmax(q)^2
+__global__ __launch_bounds__(256) void h1_bound_kernel(const int *__restrict__ map, const double *__restrict__ g,
+                                                       int n_full, const double *__restrict__ pmax, int n_points,
+                                                       double *__restrict__ part) {
+  __shared__ double red[4];
+  const int t = blockIdx.y;
+  double v = 0.0;
+  for (int q = blockIdx.x * 256 + threadIdx.x; q < n_points; q += gridDim.x * 256) {
+    const double m = pmax[q];
+    v += fabs(g[(size_t)map[q] + (size_t)n_full * t]) * m * m;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    part[(size_t)gridDim.x * t + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// bound[t] = sum of the n_part partial sums of perturbation t, in a fixed order (one workgroup per t)
+__global__ __launch_bounds__(256) void h1_bound_sum_kernel(const double *__restrict__ part, int n_part,
+                                                           double *__restrict__ bound) {
+  __shared__ double red[4];
+  const int t = blockIdx.x;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n_part; i += 256)
+    v += part[(size_t)n_part * t + i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    bound[t] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // Block: 8 waves; wave w multiplies phi rows 16 (w & 3) .. +15 of a tile J (MFMA rows) against the
 // 64 Y columns of tile I, the two halves w >> 2 taking alternate J, so D(k, i) lands with i along
 // lane % 16 (consecutive relabelled rows of H for the atomics).
@@ -114,7 +182,8 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
                                                               const double *__restrict__ g, int n_full,
                                                               const signed char *__restrict__ fdig,
                                                               const double *__restrict__ finv, int n_basis,
-                                                              double *__restrict__ Hs) {
+                                                              const double *__restrict__ bound,
+                                                              unsigned long long *__restrict__ Hs) {
   constexpr int YROW = PMAX + 16;  // LDS bytes per (plane, row) of the staged Y tile
   __shared__ __attribute__((aligned(16))) signed char Ys[S * 64 * YROW];
   __shared__ double gs[PMAX];      // g(map(p), t) of the pass's points
@@ -129,7 +198,8 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
   const double *fi = finv + d.ib_off;
   const int *sib = sib_all + d.ib_off;
   const double *gt = g + (size_t)n_full * t;
-  double *Ht = Hs + (size_t)n_basis * n_basis * t;
+  unsigned long long *Ht = Hs + (size_t)n_basis * n_basis * t;
+  const double scale = fixed_scale(bound[t]);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int kw = w & 3, iw = w >> 2;
@@ -273,7 +343,8 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
 #pragma unroll
                 for (int l = S - 2; l >= 0; l--)
                   v = v * (1.0 / RADIX) + (double)acc[l][gi][r];
-                atomicAdd(&Ht[x + (size_t)n_basis * sib[k]], v * ysc[gi] * fi[k]);
+                const long long q = (long long)rint(v * ysc[gi] * fi[k] * scale);
+                atomicAdd(&Ht[x + (size_t)n_basis * sib[k]], (unsigned long long)q);
               }
             }
           }
@@ -308,22 +379,40 @@ void prepare_f(Rho1Plan &P, AdjOzState &st, const double *wave, hipStream_t s) {
   hipLaunchKernelGGL((h1_digits_kernel<S>), dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, st.d_off.ptr,
                      R.d_spos.ptr, wave, st.d_fdig.ptr, st.d_finv.ptr);
   check_launch("h1_digits_kernel");
+  int nvp = 0;
+  for (const BatchDesc &d : P.h_desc)
+    nvp = std::max(nvp, d.pstart + d.np);
+  st.d_pmax.alloc(std::max(nvp, 1));
+  fill_f64(st.d_pmax.ptr, st.d_pmax.n, 0.0, s);  // points of empty batches stay 0
+  hipLaunchKernelGGL(h1_pointmax_kernel, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, wave, st.d_pmax.ptr);
+  check_launch("h1_pointmax_kernel");
   st.S = S;
   st.wave = wave;
 }
 
 template <int S>
-void run(Rho1Plan &P, AdjOzState &st, int T, const double *wave, const double *g, double *Hs, hipStream_t s) {
+void run(Rho1Plan &P, AdjOzState &st, int T, const double *wave, const double *g, unsigned long long *Hs,
+         hipStream_t s) {
   const Relabel &R = get_relabel(P);
+  const int nvp = (int)st.d_pmax.n;
+  const int bblocks = std::max(1, std::min((nvp + 255) / 256, 4096));
+  st.d_bound.alloc(std::max<size_t>(st.d_bound.n, (size_t)T));
+  st.d_bpart.alloc(std::max<size_t>(st.d_bpart.n, (size_t)bblocks * T));
+  hipLaunchKernelGGL(h1_bound_kernel, dim3(bblocks, T), dim3(256), 0, s, P.d_map.ptr, g, P.n_full, st.d_pmax.ptr,
+                     nvp, st.d_bpart.ptr);
+  check_launch("h1_bound_kernel");
+  hipLaunchKernelGGL(h1_bound_sum_kernel, dim3(T), dim3(256), 0, s, st.d_bpart.ptr, bblocks, st.d_bound.ptr);
+  check_launch("h1_bound_sum_kernel");
   hipLaunchKernelGGL((h1_ozaki_kernel<S>), dim3(P.n_valid, T), dim3(AZ_THREADS), 0, s, P.d_desc.ptr, st.d_off.ptr,
                      R.d_sib.ptr, P.d_map.ptr, g, P.n_full, st.d_fdig.ptr, st.d_finv.ptr,
-                     P.n_basis, Hs);
+                     P.n_basis, st.d_bound.ptr, Hs);
   check_launch("h1_ozaki_kernel");
 }
 
 } // namespace
 
-void h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *g, double *Hs, hipStream_t s) {
+const double *h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *g, unsigned long long *Hs,
+                            hipStream_t s) {
   FRPA_REQUIRE(S == 4 || S == 5, FRPA_ERR_INVALID, "INT8 split supports 4 or 5 slices, not %d", S);
   if (!P.adjoz)
     P.adjoz = new AdjOzState();
@@ -338,6 +427,7 @@ void h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *
     run<5>(P, st, T, wave, g, Hs, s);
   else
     run<4>(P, st, T, wave, g, Hs, s);
+  return st.d_bound.ptr;
 }
 
 } // namespace frpa

@@ -121,8 +121,20 @@ __device__ __forceinline__ bool claim_item(int *counters, int n_pos, int T, int 
 void ozaki_release(OzakiState *st);
 void adj_release(AdjState *st);
 void adjoz_release(AdjOzState *st);
-// INT8-split adjoint of one call: atomics into the zeroed relabelled upper-triangle scratch Hs
-void h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *g, double *Hs, hipStream_t s);
+// INT8-split adjoint of one call: int64 fixed-point atomics into the zeroed relabelled upper-triangle
+// scratch Hs; returns the device array of the per-perturbation bounds B that fix the scales
+const double *h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *g, unsigned long long *Hs,
+                            hipStream_t s);
+
+// Scale 2^E of the adjoint's fixed-point accumulation for the bound B >= |every partial sum|:
+// 2 B 2^E <= 2^61 (1 when B = 0)
+__host__ __device__ inline double fixed_scale(double B) {
+  if (!(B > 0.0))
+    return 1.0;
+  int e = 0;
+  (void)frexp(2.0 * B, &e);  // 2 B < 2^e
+  return ldexp(1.0, 61 - e);
+}
 long long ozaki_prepare_wave(Rho1Plan &P, const double *wave, int slices, hipStream_t s);
 void synth_wave(const Rho1Plan &P, int n_atoms, const double *atom_xyz, const int *basis_atom, const int *basis_l,
                 const int *basis_m, const double *basis_alpha, int n_basis, const double *coord_points, double *wave,

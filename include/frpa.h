@@ -93,7 +93,9 @@ int frpa_rho1_forward(frpa_rho1_plan *plan, int atom_tile_size, int n_tile, int 
  *   first_order_H(ib_i, ib_j, t) += sum_{batches, p} g(map(p), t) phi_i(p) phi_j(p)
  * g (n_full_points, T) device; first_order_H (n_basis, n_basis, T) device, ACCUMULATED into.
  * With g = partition_tab * (v_hartree_gradient + v2rho2 * rho1) + H_prefactor1 this is the
- * reference's LDA H1; with g = d L / d rho1 it is d L / d P1.
+ * reference's LDA H1; with g = d L / d rho1 it is d L / d P1.  The INT8-split precisions accumulate in
+ * 64-bit fixed point (integer atomics, scale from a bound on |H|) and are bit-reproducible; FP64 uses
+ * FP64 atomics (run-to-run differences in the last bits).  Non-finite g or wave leave H undefined.
  */
 int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, const double *wave, const double *g,
                       double *first_order_H, void *stream);

@@ -87,3 +87,20 @@ def test_h1_int8_split_against_oracle(cuda, oracle, k, args, precision, tol):
         (prob.n_basis, prob.n_basis, T), order="F")
     ref2 = _oracle_h1(oracle, prob, g2, T)
     assert np.abs(H2 - ref2).max() <= tol * np.abs(ref2).max()
+
+
+@pytest.mark.parametrize("precision", [api.OZAKI, api.OZAKI4])
+def test_h1_int8_split_is_bit_reproducible(cuda, precision):
+    """The INT8-split adjoint accumulates H in 64-bit fixed point with integer atomics, so repeated
+    calls give the same bits whatever order the atomics land in (T = 2, many batches sharing each
+    H element); a g scaled by 2 gives exactly 2 H (the scale follows the bound by a power of two)."""
+    prob = synth.make_rho1_problem(40, 16.0, 160, (1, 260), (1, 260), config_k=5)
+    T = 2
+    g = api.to_device(np.asfortranarray(np.random.default_rng(3).standard_normal((prob.n_full_points, T))))
+    plan = api.Rho1Plan.from_problem(prob)
+    wave = api.to_device(prob.wave)
+    H = [plan.adjoint(wave, g, T=T, precision=precision).clone() for _ in range(3)]
+    assert torch.count_nonzero(H[0]) > 0
+    assert torch.equal(H[0], H[1]) and torch.equal(H[0], H[2])
+    H2 = plan.adjoint(wave, 2.0 * g, T=T, precision=precision)
+    assert torch.equal(H2, 2.0 * H[0])

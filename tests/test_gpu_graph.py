@@ -1,6 +1,7 @@
-"""The whole step (INT8-split rho1, H1 adjoint, far field) captured once in a HIP graph and replayed:
-every replay, with eager work and synchronisations between replays, must give the eager results
-(rho1 and the far field bit for bit; H1, summed with atomics in a run-dependent order, to 1e-13)."""
+"""The whole step (INT8-split rho1, INT8-split H1 adjoint, far field) captured once in a HIP graph and
+replayed: every replay, with eager work and synchronisations between replays, must give the eager
+results bit for bit (the INT8 adjoint sums in 64-bit fixed point, so its atomics commute); the FP64
+adjoint, whose FP64 atomics land in a run-dependent order, to 1e-13."""
 import numpy as np
 import pytest
 import torch
@@ -21,12 +22,15 @@ def test_step_graph_replays_match_eager(cuda):
     g = api.to_device(np.random.default_rng(1).standard_normal(prob.n_full_points))
     rho = torch.empty(prob.n_full_points, dtype=torch.float64, device="cuda")
     H = torch.empty(prob.n_basis * prob.n_basis, dtype=torch.float64, device="cuda")
+    H64 = torch.empty(prob.n_basis * prob.n_basis, dtype=torch.float64, device="cuda")
     dv = torch.empty(fp.n_full_points, dtype=torch.float64, device="cuda")
 
     def step():
         plan.forward(wave, P1, precision=api.OZAKI, out=rho)
         H.zero_()
-        plan.adjoint(wave, g, H=H)
+        plan.adjoint(wave, g, H=H, precision=api.OZAKI)
+        H64.zero_()
+        plan.adjoint(wave, g, H=H64, precision=api.FP64)
         dv.zero_()
         fplan.forward(dv, *finp.args(), variant=api.FAR_HORNER)
 
@@ -37,22 +41,24 @@ def test_step_graph_replays_match_eager(cuda):
         step()
     torch.cuda.current_stream().wait_stream(s)
     torch.cuda.synchronize()
-    ref = [rho.clone(), H.clone(), dv.clone()]
+    ref = [rho.clone(), H.clone(), dv.clone(), H64.clone()]
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         step()
     for rep in range(4):
         rho.fill_(np.nan)
         H.fill_(np.nan)
+        H64.fill_(np.nan)
         dv.fill_(np.nan)
         torch.cuda.synchronize()
         graph.replay()
         torch.cuda.synchronize()
         assert torch.equal(rho, ref[0]), f"replay {rep}: rho1 differs"
         assert torch.equal(dv, ref[2]), f"replay {rep}: far field differs"
-        hmax = float(ref[1].abs().max())
-        herr = float((H - ref[1]).abs().max())
-        print(f"[graph replay {rep}] H1 max|diff| / max {herr / hmax:.2e}", flush=True)
+        assert torch.equal(H, ref[1]), f"replay {rep}: INT8 adjoint differs"
+        hmax = float(ref[3].abs().max())
+        herr = float((H64 - ref[3]).abs().max())
+        print(f"[graph replay {rep}] FP64 H1 max|diff| / max {herr / hmax:.2e}", flush=True)
         assert herr <= 1e-13 * hmax
         step()  # eager work between replays
         torch.cuda.synchronize()

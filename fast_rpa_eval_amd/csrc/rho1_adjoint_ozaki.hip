@@ -24,6 +24,7 @@
 // FP64 atomics of the other adjoints are not), at the speed of the FP64 atomics.  Non-finite g or
 // phi leave H undefined.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "rho1_common.hpp"
@@ -42,6 +43,7 @@ struct AdjOzState {
   DevBuf<double> d_pmax;         // max_i |phi_i(p)| per valid point (pstart + p)
   DevBuf<double> d_bound;        // per call: B per perturbation
   DevBuf<double> d_bpart;        // per call: B's partial sums, one per workgroup of h1_bound_kernel
+  DevBuf<int> d_spin_fail;       // waits of the producer / consumer mode that gave up (0 when all is well)
 };
 
 void adjoz_release(AdjOzState *st) { delete st; }
@@ -174,7 +176,18 @@ __global__ __launch_bounds__(256) void h1_bound_sum_kernel(const double *__restr
 // Block: 8 waves; wave w multiplies phi rows 16 (w & 3) .. +15 of a tile J (MFMA rows) against the
 // 64 Y columns of tile I, the two halves w >> 2 taking alternate J, so D(k, i) lands with i along
 // lane % 16 (consecutive relabelled rows of H for the atomics).
-template <int S>
+// PC (producer / consumer): waves 0 .. PC_CW - 1 multiply and leave each tile-pair block in an LDS
+// slot of their own; the other PC_NA waves issue every atomic from the slots.  A wave's vector-memory counter is in
+// order, so a multiplying wave that issued atomics would wait for their memory-side completion before
+// using its next phi-digit loads; with the atomics in other waves the two overlap.  Slot handover:
+// release / acquire flags in LDS, every wait bounded (a wave that gives up leaves H incomplete
+// rather than hanging; the count is added to *spin_fail).
+constexpr int PC_CW = 6;         // multiplying waves of the producer / consumer mode
+constexpr int PC_NA = 8 - PC_CW; // atomic waves; atomic wave aw serves the multiplying waves aw + PC_NA j
+constexpr int PC_PER = (PC_CW + PC_NA - 1) / PC_NA;
+constexpr int PC_SPIN = 1 << 16; // bound on the polls of one wait (each ~64 clocks: ~2 ms)
+
+template <int S, bool PC>
 __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *__restrict__ descs,
                                                               const long long *__restrict__ off,
                                                               const int *__restrict__ sib_all,
@@ -183,12 +196,15 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
                                                               const signed char *__restrict__ fdig,
                                                               const double *__restrict__ finv, int n_basis,
                                                               const double *__restrict__ bound,
-                                                              unsigned long long *__restrict__ Hs) {
+                                                              unsigned long long *__restrict__ Hs,
+                                                              int *__restrict__ spin_fail) {
   constexpr int YROW = PMAX + 16;  // LDS bytes per (plane, row) of the staged Y tile
   __shared__ __attribute__((aligned(16))) signed char Ys[S * 64 * YROW];
   __shared__ double gs[PMAX];      // g(map(p), t) of the pass's points
   __shared__ double red[8][64];
   __shared__ double ysc_s[64];     // 1 / sy of the tile's rows
+  __shared__ long long slot[PC ? PC_CW : 1][PC ? 16 * 64 : 1];  // q(k row kk, i row ii) at kk * 64 + ii
+  __shared__ int slot_full[PC_CW], slot_cols[PC_CW][16];      // cols: H column labels of the block's 16 rows
   const int di = blockIdx.x, t = blockIdx.y;
   const BatchDesc d = descs[di];
   const int nc = d.nc, np = d.np;
@@ -204,6 +220,8 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
   const int lr = lane & 15, lk = lane >> 4;
   const int kw = w & 3, iw = w >> 2;
   const int nT = nc64 / 64;
+  if (PC && tid < PC_CW)
+    slot_full[tid] = 0;
 
   for (int pp = 0; pp < npk; pp += PMAX) {
     const int npp = min(PMAX, npk - pp);  // points of this pass (multiple of 64)
@@ -288,75 +306,214 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
 #pragma unroll
       for (int gi = 0; gi < 4; gi++)
         ysc[gi] = ysc_s[16 * gi + lr];
-      // The two halves of the block (iw) take alternate column tiles J = I + iw, I + iw + 2, ...; a
-      // wave multiplies phi rows 16 kw .. + 15 of its J against all 64 Y rows of tile I.
-      // A fragments of step s = (J, c) = (I + iw + 2 (s / nck), s % nck): phi digits of row
-      // 64 J + 16 kw + lr, points pp + 64 c + 16 lk .. + 15; step s + 2 is in flight while s runs.
-      auto load_a = [&](int st, v4i (&a)[S]) {
-        const int J = I + iw + 2 * (st / nck), c = st % nck;
-        const signed char *src = F + (size_t)npk * (64 * J + 16 * kw + lr) + pp + 64 * c + 16 * lk;
+      if constexpr (PC) {
+        // units (J, kw) = (I + u / 4, u % 4), u < 4 (nT - I): multiplying wave w takes u = w, w + 6, ...
+        const int NU = 4 * (nT - I);
+        auto units_of = [&](int cw) { return NU > cw ? (NU - cw + PC_CW - 1) / PC_CW : 0; };
+        if (w < PC_CW) {
+          auto unit_jk = [&](int st, int &J, int &kwu) {
+            const int u = w + PC_CW * (st / nck);
+            J = I + u / 4;
+            kwu = u % 4;
+          };
+          auto load_a = [&](int st, v4i (&a)[S]) {
+            int J, kwu;
+            unit_jk(st, J, kwu);
+            const int c = st % nck;
+            const signed char *src = F + (size_t)npk * (64 * J + 16 * kwu + lr) + pp + 64 * c + 16 * lk;
 #pragma unroll
-        for (int sl = 0; sl < S; sl++)
-          a[sl] = *reinterpret_cast<const v4i *>(src + sl * plane);
-      };
-      const int npairs = nT - I > iw ? (nT - I - iw + 1) / 2 : 0;
-      const int nsteps = npairs * nck;
-      v4i a0[S], a1[S], a2[S];
-      if (nsteps > 0)
-        load_a(0, a0);
-      if (nsteps > 1)
-        load_a(1, a1);
-      v4i acc[S][4];
-      for (int st = 0, J = I + iw, c = 0; st < nsteps; st++) {
-        if (c == 0) {
+            for (int sl = 0; sl < S; sl++)
+              a[sl] = *reinterpret_cast<const v4i *>(src + sl * plane);
+          };
+          const int nsteps = units_of(w) * nck;
+          v4i a0[S], a1[S], a2[S];
+          if (nsteps > 0)
+            load_a(0, a0);
+          if (nsteps > 1)
+            load_a(1, a1);
+          v4i acc[S][4];
+          for (int st = 0, c = 0; st < nsteps; st++) {
+            if (c == 0) {
 #pragma unroll
-          for (int l = 0; l < S; l++)
+              for (int l = 0; l < S; l++)
 #pragma unroll
-            for (int gi = 0; gi < 4; gi++)
-              acc[l][gi] = v4i{0, 0, 0, 0};
+                for (int gi = 0; gi < 4; gi++)
+                  acc[l][gi] = v4i{0, 0, 0, 0};
+            }
+            if (st + 2 < nsteps)
+              load_a(st + 2, a2);
+#pragma unroll
+            for (int gi = 0; gi < 4; gi++) {
+              v4i b[S];
+#pragma unroll
+              for (int sl = 0; sl < S; sl++)
+                b[sl] =
+                    *reinterpret_cast<const v4i *>(&Ys[(sl * 64 + 16 * gi + lr) * YROW + 64 * c + 16 * lk]);
+#pragma unroll
+              for (int sa = 0; sa < S; sa++)
+#pragma unroll
+                for (int sb = 0; sb < S - sa; sb++)
+                  acc[sa + sb][gi] =
+                      __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[sa], b[sb], acc[sa + sb][gi], 0, 0, 0);
+            }
+            if (c == nck - 1) {
+              int J, kwu;
+              unit_jk(st, J, kwu);
+              // wait until this wave's slot is empty, fill it, hand it over
+              int spins = 0;
+              while (__hip_atomic_load(&slot_full[w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0 &&
+                     ++spins < PC_SPIN)
+                __builtin_amdgcn_s_sleep(1);
+              if (spins >= PC_SPIN && lane == 0)
+                atomicAdd(spin_fail, 1);
+              // acc[l][gi][r] = C_l(k = 64 J + 16 kwu + 4 lk + r, i = 64 I + 16 gi + lr)
+#pragma unroll
+              for (int gi = 0; gi < 4; gi++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                  const int k = 64 * J + 16 * kwu + 4 * lk + r;
+                  double v = (double)acc[S - 1][gi][r];
+#pragma unroll
+                  for (int l = S - 2; l >= 0; l--)
+                    v = v * (1.0 / RADIX) + (double)acc[l][gi][r];
+                  slot[w][(4 * lk + r) * 64 + 16 * gi + lr] =
+                      k < nc ? (long long)rint(v * ysc[gi] * fi[k] * scale) : 0ll;
+                }
+              if (lane < 16) {
+                const int k = 64 * J + 16 * kwu + lane;
+                slot_cols[w][lane] = k < nc ? sib[k] : -1;
+              }
+              if (lane == 0)
+                __hip_atomic_store(&slot_full[w], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+#pragma unroll
+            for (int sl = 0; sl < S; sl++) {
+              a0[sl] = a1[sl];
+              a1[sl] = a2[sl];
+            }
+            if (++c == nck)
+              c = 0;
+          }
+        } else {
+          // atomic wave aw serves multiplying waves aw + PC_NA e: exactly units_of(cw) blocks each
+          const int aw = w - PC_CW;
+          int left[PC_PER], total = 0;
+#pragma unroll
+          for (int e = 0; e < PC_PER; e++) {
+            const int cw = aw + PC_NA * e;
+            left[e] = cw < PC_CW ? units_of(cw) : 0;
+            total += left[e];
+          }
+          const int i = 64 * I + lane;
+          const int xi = i < nc ? sib[i] : 0;
+          const size_t x = (size_t)xi;
+          int spins = 0;
+          while (total > 0 && spins < PC_SPIN) {
+            bool got = false;
+#pragma unroll
+            for (int e = 0; e < PC_PER; e++) {
+              const int cw = aw + PC_NA * e;
+              if (left[e] > 0 &&
+                  __hip_atomic_load(&slot_full[cw], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0) {
+                // the block's column labels come with it (no global load here, which would wait for
+                // this wave's earlier atomics); sorted labels ascend, so i <= k is xi <= col
+                long long qv[16];
+                int col[16];
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) {
+                  qv[kk] = slot[cw][kk * 64 + lane];
+                  col[kk] = slot_cols[cw][kk];
+                }
+                if (lane == 0)
+                  __hip_atomic_store(&slot_full[cw], 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++)
+                  if (i < nc && col[kk] >= 0 && xi <= col[kk])
+                    atomicAdd(&Ht[x + (size_t)n_basis * col[kk]], (unsigned long long)qv[kk]);
+                left[e]--;
+                total--;
+                got = true;
+              }
+            }
+            if (!got) {
+              spins++;
+              __builtin_amdgcn_s_sleep(1);
+            }
+          }
+          if (total > 0 && lane == 0)
+            atomicAdd(spin_fail, 1);
         }
-        if (st + 2 < nsteps)
-          load_a(st + 2, a2);
-#pragma unroll
-        for (int gi = 0; gi < 4; gi++) {
-          v4i b[S];
-#pragma unroll
+      } else {
+        // The two halves of the block (iw) take alternate column tiles J = I + iw, I + iw + 2, ...; a
+        // wave multiplies phi rows 16 kw .. + 15 of its J against all 64 Y rows of tile I.
+        // A fragments of step s = (J, c) = (I + iw + 2 (s / nck), s % nck): phi digits of row
+        // 64 J + 16 kw + lr, points pp + 64 c + 16 lk .. + 15; step s + 2 is in flight while s runs.
+        auto load_a = [&](int st, v4i (&a)[S]) {
+          const int J = I + iw + 2 * (st / nck), c = st % nck;
+          const signed char *src = F + (size_t)npk * (64 * J + 16 * kw + lr) + pp + 64 * c + 16 * lk;
+  #pragma unroll
           for (int sl = 0; sl < S; sl++)
-            b[sl] = *reinterpret_cast<const v4i *>(&Ys[(sl * 64 + 16 * gi + lr) * YROW + 64 * c + 16 * lk]);
-#pragma unroll
-          for (int sa = 0; sa < S; sa++)
-#pragma unroll
-            for (int sb = 0; sb < S - sa; sb++)
-              acc[sa + sb][gi] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[sa], b[sb], acc[sa + sb][gi], 0, 0, 0);
-        }
-        if (c == nck - 1) {
-          // acc[l][gi][r] = C_l(k = 64 J + 16 kw + 4 lk + r, i = 64 I + 16 gi + lr)
-#pragma unroll
+            a[sl] = *reinterpret_cast<const v4i *>(src + sl * plane);
+        };
+        const int npairs = nT - I > iw ? (nT - I - iw + 1) / 2 : 0;
+        const int nsteps = npairs * nck;
+        v4i a0[S], a1[S], a2[S];
+        if (nsteps > 0)
+          load_a(0, a0);
+        if (nsteps > 1)
+          load_a(1, a1);
+        v4i acc[S][4];
+        for (int st = 0, J = I + iw, c = 0; st < nsteps; st++) {
+          if (c == 0) {
+  #pragma unroll
+            for (int l = 0; l < S; l++)
+  #pragma unroll
+              for (int gi = 0; gi < 4; gi++)
+                acc[l][gi] = v4i{0, 0, 0, 0};
+          }
+          if (st + 2 < nsteps)
+            load_a(st + 2, a2);
+  #pragma unroll
           for (int gi = 0; gi < 4; gi++) {
-            const int i = 64 * I + 16 * gi + lr;
-            const size_t x = i < nc ? (size_t)sib[i] : 0;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-              const int k = 64 * J + 16 * kw + 4 * lk + r;
-              if (i < nc && k < nc && i <= k) {
-                double v = (double)acc[S - 1][gi][r];
-#pragma unroll
-                for (int l = S - 2; l >= 0; l--)
-                  v = v * (1.0 / RADIX) + (double)acc[l][gi][r];
-                const long long q = (long long)rint(v * ysc[gi] * fi[k] * scale);
-                atomicAdd(&Ht[x + (size_t)n_basis * sib[k]], (unsigned long long)q);
+            v4i b[S];
+  #pragma unroll
+            for (int sl = 0; sl < S; sl++)
+              b[sl] = *reinterpret_cast<const v4i *>(&Ys[(sl * 64 + 16 * gi + lr) * YROW + 64 * c + 16 * lk]);
+  #pragma unroll
+            for (int sa = 0; sa < S; sa++)
+  #pragma unroll
+              for (int sb = 0; sb < S - sa; sb++)
+                acc[sa + sb][gi] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[sa], b[sb], acc[sa + sb][gi], 0, 0, 0);
+          }
+          if (c == nck - 1) {
+            // acc[l][gi][r] = C_l(k = 64 J + 16 kw + 4 lk + r, i = 64 I + 16 gi + lr)
+  #pragma unroll
+            for (int gi = 0; gi < 4; gi++) {
+              const int i = 64 * I + 16 * gi + lr;
+              const size_t x = i < nc ? (size_t)sib[i] : 0;
+  #pragma unroll
+              for (int r = 0; r < 4; r++) {
+                const int k = 64 * J + 16 * kw + 4 * lk + r;
+                if (i < nc && k < nc && i <= k) {
+                  double v = (double)acc[S - 1][gi][r];
+  #pragma unroll
+                  for (int l = S - 2; l >= 0; l--)
+                    v = v * (1.0 / RADIX) + (double)acc[l][gi][r];
+                  const long long q = (long long)rint(v * ysc[gi] * fi[k] * scale);
+                  atomicAdd(&Ht[x + (size_t)n_basis * sib[k]], (unsigned long long)q);
+                }
               }
             }
           }
-        }
-#pragma unroll
-        for (int sl = 0; sl < S; sl++) {
-          a0[sl] = a1[sl];
-          a1[sl] = a2[sl];
-        }
-        if (++c == nck) {
-          c = 0;
-          J += 2;
+  #pragma unroll
+          for (int sl = 0; sl < S; sl++) {
+            a0[sl] = a1[sl];
+            a1[sl] = a2[sl];
+          }
+          if (++c == nck) {
+            c = 0;
+            J += 2;
+          }
         }
       }
     }
@@ -403,10 +560,26 @@ void run(Rho1Plan &P, AdjOzState &st, int T, const double *wave, const double *g
   check_launch("h1_bound_kernel");
   hipLaunchKernelGGL(h1_bound_sum_kernel, dim3(T), dim3(256), 0, s, st.d_bpart.ptr, bblocks, st.d_bound.ptr);
   check_launch("h1_bound_sum_kernel");
-  hipLaunchKernelGGL((h1_ozaki_kernel<S>), dim3(P.n_valid, T), dim3(AZ_THREADS), 0, s, P.d_desc.ptr, st.d_off.ptr,
-                     R.d_sib.ptr, P.d_map.ptr, g, P.n_full, st.d_fdig.ptr, st.d_finv.ptr,
-                     P.n_basis, st.d_bound.ptr, Hs);
+  st.d_spin_fail.alloc(1);
+  fill_i32(st.d_spin_fail.ptr, 1, 0, s);
+  const char *pc = std::getenv("FRPA_ADJOINT_PC");
+  auto kern = (pc && pc[0] == '0') ? h1_ozaki_kernel<S, false> : h1_ozaki_kernel<S, true>;
+  hipLaunchKernelGGL(kern, dim3(P.n_valid, T), dim3(AZ_THREADS), 0, s, P.d_desc.ptr, st.d_off.ptr, R.d_sib.ptr,
+                     P.d_map.ptr, g, P.n_full, st.d_fdig.ptr, st.d_finv.ptr, P.n_basis, st.d_bound.ptr, Hs,
+                     st.d_spin_fail.ptr);
   check_launch("h1_ozaki_kernel");
+  // FRPA_CHECK_ADJOINT_WAITS=1: wait for the kernel and fail if a slot handover gave up (tests; not
+  // while the stream is being captured)
+  const char *chk = std::getenv("FRPA_CHECK_ADJOINT_WAITS");
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (chk && chk[0] == '1')
+    FRPA_HIP(hipStreamIsCapturing(s, &cap));
+  if (chk && chk[0] == '1' && cap == hipStreamCaptureStatusNone) {
+    int fails = 0;
+    FRPA_HIP(hipMemcpyAsync(&fails, st.d_spin_fail.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    FRPA_HIP(hipStreamSynchronize(s));
+    FRPA_REQUIRE(fails == 0, FRPA_ERR_HIP, "INT8 adjoint: %d slot waits gave up", fails);
+  }
 }
 
 } // namespace

@@ -104,3 +104,20 @@ def test_h1_int8_split_is_bit_reproducible(cuda, precision):
     assert torch.equal(H[0], H[1]) and torch.equal(H[0], H[2])
     H2 = plan.adjoint(wave, 2.0 * g, T=T, precision=precision)
     assert torch.equal(H2, 2.0 * H[0])
+
+
+def test_h1_int8_split_producer_consumer_matches_direct(cuda, monkeypatch):
+    """The default INT8 adjoint hands each tile-pair block from the multiplying waves to atomic waves
+    through LDS slots; with FRPA_ADJOINT_PC=0 every wave issues its own atomics.  Both add the same
+    int64 values, so H is the same bits; every slot wait completes (FRPA_CHECK_ADJOINT_WAITS)."""
+    monkeypatch.setenv("FRPA_CHECK_ADJOINT_WAITS", "1")
+    prob = synth.make_rho1_problem(40, 16.0, 160, (1, 600), (1, 300), config_k=6, zero_nc=(3,))
+    T = 2
+    g = api.to_device(np.asfortranarray(np.random.default_rng(4).standard_normal((prob.n_full_points, T))))
+    plan = api.Rho1Plan.from_problem(prob)
+    wave = api.to_device(prob.wave)
+    H_pc = plan.adjoint(wave, g, T=T, precision=api.OZAKI).clone()
+    monkeypatch.setenv("FRPA_ADJOINT_PC", "0")
+    H_direct = plan.adjoint(wave, g, T=T, precision=api.OZAKI)
+    assert torch.count_nonzero(H_pc) > 0
+    assert torch.equal(H_pc, H_direct)

@@ -176,12 +176,15 @@ __global__ __launch_bounds__(256) void h1_bound_sum_kernel(const double *__restr
 // Block: 8 waves; wave w multiplies phi rows 16 (w & 3) .. +15 of a tile J (MFMA rows) against the
 // 64 Y columns of tile I, the two halves w >> 2 taking alternate J, so D(k, i) lands with i along
 // lane % 16 (consecutive relabelled rows of H for the atomics).
-// PC (producer / consumer): waves 0 .. PC_CW - 1 multiply and leave each tile-pair block in an LDS
-// slot of their own; the other PC_NA waves issue every atomic from the slots.  A wave's vector-memory counter is in
-// order, so a multiplying wave that issued atomics would wait for their memory-side completion before
-// using its next phi-digit loads; with the atomics in other waves the two overlap.  Slot handover:
-// release / acquire flags in LDS, every wait bounded (a wave that gives up leaves H incomplete
-// rather than hanging; the count is added to *spin_fail).
+//
+// PC (producer / consumer, the default): waves 0 .. PC_CW - 1 multiply -- the units (J, 16-row group)
+// of a tile I dealt round-robin -- and leave each 16 x 64 tile-pair block, in fixed point, in an LDS
+// slot of their own; the other PC_NA waves issue every atomic from the slots.  A wave's
+// vector-memory counter is in order, so a multiplying wave that issued atomics would wait for their
+// memory-side completion before using its next phi-digit loads; with the atomics in other waves the
+// two overlap.  Slot handover: release / acquire flags in LDS, every wait bounded (a wave that gives
+// up leaves H incomplete rather than hanging; the count is added to *spin_fail).  Every wave stages
+// the Y digits; an atomic wave drains all blocks of tile I before the barrier that starts tile I + 1.
 constexpr int PC_CW = 6;         // multiplying waves of the producer / consumer mode
 constexpr int PC_NA = 8 - PC_CW; // atomic waves; atomic wave aw serves the multiplying waves aw + PC_NA j
 constexpr int PC_PER = (PC_CW + PC_NA - 1) / PC_NA;

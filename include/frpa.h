@@ -96,6 +96,10 @@ int frpa_rho1_forward(frpa_rho1_plan *plan, int atom_tile_size, int n_tile, int 
  * reference's LDA H1; with g = d L / d rho1 it is d L / d P1.  The INT8-split precisions accumulate in
  * 64-bit fixed point (integer atomics, scale from a bound on |H|) and are bit-reproducible; FP64 uses
  * FP64 atomics (run-to-run differences in the last bits).  Non-finite g or wave leave H undefined.
+ * The INT8-split precisions merge consecutive batches with contiguous points into one work item over
+ * the union of their compute sets (one atomic per H element per group); env FRPA_ADJOINT_MERGE sets
+ * the threshold (|union|^2 <= merge * sum n_compute^2; default 0.8, 0 = every batch alone), read per
+ * call; the groups and digits are rebuilt when it changes.
  */
 int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, const double *wave, const double *g,
                       double *first_order_H, void *stream);

@@ -121,3 +121,29 @@ def test_h1_int8_split_producer_consumer_matches_direct(cuda, monkeypatch):
     H_direct = plan.adjoint(wave, g, T=T, precision=api.OZAKI)
     assert torch.count_nonzero(H_pc) > 0
     assert torch.equal(H_pc, H_direct)
+
+
+def test_h1_int8_split_merged_groups_against_oracle(cuda, oracle, monkeypatch):
+    """The INT8 adjoint merges consecutive batches with contiguous points into one work item over the
+    union of their compute sets (FRPA_ADJOINT_MERGE: 0 = every batch alone, 0.8 = the default,
+    4 = whenever the points fit one 256-point pass).  Every threshold agrees with the FP64 oracle; a
+    batch with n_compute_c = 0 (its points break the contiguity) and one with no points are in the
+    list.  Merged groups have other digit scales, so H differs in the last bits from the unmerged one,
+    which shows the groups were formed."""
+    monkeypatch.setenv("FRPA_CHECK_ADJOINT_WAITS", "1")
+    prob = synth.make_rho1_problem(40, 16.0, 200, (20, 120), (40, 300), config_k=7, zero_nc=(5,), zero_np=(9,))
+    T = 2
+    g_host = np.asfortranarray(np.random.default_rng(8).standard_normal((prob.n_full_points, T)))
+    ref = _oracle_h1(oracle, prob, g_host, T)
+    plan = api.Rho1Plan.from_problem(prob)
+    wave, g = api.to_device(prob.wave), api.to_device(g_host)
+    out = {}
+    for m in ("0", "0.8", "4"):
+        monkeypatch.setenv("FRPA_ADJOINT_MERGE", m)
+        H = plan.adjoint(wave, g, T=T, precision=api.OZAKI).clone()
+        got = H.cpu().numpy().reshape((prob.n_basis, prob.n_basis, T), order="F")
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        print(f"[h1 int8 split merge {m}] rel {err:.3e}", flush=True)
+        assert err <= 1e-10
+        out[m] = H
+    assert not torch.equal(out["0"], out["4"])

@@ -81,6 +81,7 @@ namespace {
 constexpr double RADIX = 254.0;
 constexpr int AZ_THREADS = 512;
 constexpr int PMAX = 256;  // points held in LDS per pass (larger batches take several passes)
+constexpr int GMAX = 16;   // batches per group
 
 __device__ __forceinline__ int ceil64(int x) { return (x + 63) & ~63; }
 
@@ -100,7 +101,11 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const BatchDesc *__restr
   const int nc = d.nc, np = d.np;
   const int nc64 = ceil64(nc), npk = ceil64(np);
   const size_t plane = (size_t)nc64 * npk;
-  const int m0 = gmem[blockIdx.x], m1 = gmem[blockIdx.x + 1];
+  __shared__ GMember sm[GMAX];
+  const int m0 = gmem[blockIdx.x], nm = gmem[blockIdx.x + 1] - m0;
+  if ((int)threadIdx.x < nm)
+    sm[threadIdx.x] = members[m0 + threadIdx.x];
+  __syncthreads();
   signed char *out = dig + off[blockIdx.x];
   double *iv = inv + d.ib_off;
   for (int r0 = 0; r0 < nc64; r0 += 256) {
@@ -110,17 +115,22 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const BatchDesc *__restr
     const bool real = r < nc;
     double mx = 0.0;
     if (real)
-      for (int m = m0; m < m1; m++) {
-        const GMember M = members[m];
-        const int lr = rowmap[M.rowmap_off + r];
+      for (int m = 0; m < nm; m++) {
+        const int lr = rowmap[sm[m].rowmap_off + r];
         if (lr >= 0)
-          for (int p = 0; p < M.np; p++)
-            mx = fmax(mx, fabs(wave[M.wave_off + lr + (size_t)M.ld * p]));
+          for (int p = 0; p < sm[m].np; p++)
+            mx = fmax(mx, fabs(wave[sm[m].wave_off + lr + (size_t)sm[m].ld * p]));
       }
     const double sc = mx > 0.0 ? 127.0 / mx : 1.0;
     if (real)
       iv[r] = 1.0 / sc;
-    int m = m0;  // member holding point p (points ascend through the members)
+    // member m holds points [pend - its np, pend); Wr = its column of this row (nullptr: row absent)
+    int m = 0, pend = nm ? sm[0].np : 0;
+    const double *Wr = nullptr;
+    if (real && nm) {
+      const int lr = rowmap[sm[0].rowmap_off + r];
+      Wr = lr >= 0 ? wave + sm[0].wave_off + lr : nullptr;
+    }
     for (int p0 = 0; p0 < npk; p0 += 16) {
       unsigned int w4[S][4];
 #pragma unroll
@@ -133,12 +143,14 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const BatchDesc *__restr
         const int p = p0 + j;
         double x = 0.0;
         if (real && p < np) {
-          while (p >= members[m].p0 + members[m].np)
+          while (p >= pend) {  // p < np = the members' points, so m stays < nm
             m++;
-          const GMember M = members[m];
-          const int lr = rowmap[M.rowmap_off + r];
-          if (lr >= 0)
-            x = wave[M.wave_off + lr + (size_t)M.ld * (p - M.p0)] * sc;
+            pend += sm[m].np;
+            const int lr = rowmap[sm[m].rowmap_off + r];
+            Wr = lr >= 0 ? wave + sm[m].wave_off + lr : nullptr;
+          }
+          if (Wr)
+            x = Wr[(size_t)sm[m].ld * (p - (pend - sm[m].np))] * sc;
         }
 #pragma unroll
         for (int sl = 0; sl < S; sl++) {
@@ -577,7 +589,6 @@ double merge_threshold() {
 // Builds the groups into st; returns their descriptors (launch order).
 std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge) {
   const Relabel &R = get_relabel(P);
-  constexpr int GMAX = 16;  // members per group
   std::vector<int> order(P.n_valid);
   for (int i = 0; i < P.n_valid; i++)
     order[i] = i;

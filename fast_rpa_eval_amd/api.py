@@ -6,20 +6,22 @@ Two levels, both calling libfrpa.so through its C-ABI:
   torch tensors already in HBM (the measured path).  Argument names and layouts are those of the
   reference's Fortran-ABI arguments (fast_rap_src_dfpt_gpu_eval/evaluate_first_order_rho.h:3-35,
   sum_up_whole_potential.h:3-48); Fortran-ordered arrays are passed as flat tensors in memory order.
-* ``reference_abi``: the reference-named entry points (init_dfpt_device_info,
-  init_dfpt_device_data, evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_,
-  sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_) called with host numpy
-  arrays, exactly as a Fortran/C++ host would call them.
+* The reference-named entry points (init_dfpt_device_info, init_dfpt_device_data,
+  evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_,
+  sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_, ...) are exported by the same
+  library and called with host arrays, as a Fortran/C++ host calls them; tests/test_gpu_reference_abi.py
+  drives them through ctypes.
 
 torch provides device memory and streams only; all arithmetic runs in libfrpa's HIP kernels.
 """
 import ctypes
+import weakref
 
 import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, lib
+from ._lib import FrpaError, check, lib  # noqa: F401  (FrpaError re-exported)
 
 FP64, OZAKI, OZAKI4 = _lib.FRPA_PREC_FP64, _lib.FRPA_PREC_OZAKI, _lib.FRPA_PREC_OZAKI4
 FP32, BF16X3 = _lib.FRPA_PREC_FP32, _lib.FRPA_PREC_BF16X3
@@ -67,6 +69,7 @@ class Rho1Plan:
         k = self._keep
         self.n_batches = len(k[0])
         self.n_basis, self.n_full_points, self.n_max_batch_size = n_basis, n_full_points, n_max_batch_size
+        self._wave_seen = None  # (weakref, version, address) of the wave the INT8 digits were made from
         self.handle = ctypes.c_void_p()
         check(lib().frpa_rho1_plan_create(
             ctypes.byref(self.handle), self.n_batches, n_basis, n_full_points, n_max_batch_size,
@@ -87,9 +90,21 @@ class Rho1Plan:
                 pass
             self.handle = None
 
+    def _follow_wave(self, wave, precision, stream=None):
+        """The library keeps INT8 digits of a wave by its address; a different tensor at the same
+        address (a freed one's memory reused) or an in-place change (a new version) needs them
+        rebuilt, which frpa_rho1_prepare_wave does for the forward and the adjoint."""
+        if precision not in (OZAKI, OZAKI4):
+            return
+        seen = self._wave_seen
+        if seen is not None and (seen[0]() is not wave or seen[1] != wave._version or seen[2] != wave.data_ptr()):
+            self.prepare_wave(wave, precision, stream)
+        self._wave_seen = (weakref.ref(wave), wave._version, wave.data_ptr())
+
     def forward(self, wave, density_matrix, T=1, n_tile=None, precision=FP64, rho_bias=None,
                 gradient_wave=None, grad_bias=None, out=None, grad_out=None, stream=None):
         """first_order_rho (n_full_points * T,) [and first_order_gradient_rho] on the device."""
+        self._follow_wave(wave, precision, stream)
         n_tile = T if n_tile is None else n_tile
         dev = wave.device
         if out is None:
@@ -104,11 +119,21 @@ class Rho1Plan:
 
     def adjoint(self, wave, g, T=1, precision=FP64, H=None, stream=None):
         """first_order_H (n_basis^2 * T,) += sum_p g(map(p), t) phi_i phi_j."""
+        self._follow_wave(wave, precision, stream)
         if H is None:
             H = torch.zeros(self.n_basis * self.n_basis * T, dtype=torch.float64, device=wave.device)
         check(lib().frpa_rho1_adjoint(self.handle, T, precision, _dp(wave), _dp(g), _dp(H), _stream(stream)),
               "frpa_rho1_adjoint")
         return H
+
+    def set_adjoint_wait_limit(self, polls=0):
+        """Bound (in polls) of the INT8 adjoint's slot waits; 0 = the default (see frpa.h)."""
+        check(lib().frpa_rho1_set_adjoint_wait_limit(self.handle, polls), "frpa_rho1_set_adjoint_wait_limit")
+
+    def adjoint_status(self, stream=None):
+        """Raises FrpaError if an INT8 adjoint slot wait gave up since the last check (graph replays
+        included); synchronises the stream."""
+        check(lib().frpa_rho1_adjoint_status(self.handle, _stream(stream)), "frpa_rho1_adjoint_status")
 
     def autotune(self, wave, density_matrix, precision=FP64, T=1, reps=3, out=None, stream=None):
         """Time the launch configurations of forward() for this precision on these inputs and keep the

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -83,6 +84,12 @@ struct DevBuf {
       return;
     FRPA_HIP(hipMalloc((void **)&ptr, count * sizeof(T)));
     n = count;
+  }
+  // Per-call scratch: reallocated only when a call needs more than it holds (n is then the
+  // capacity), so that calls with smaller sizes never free memory a captured graph still uses.
+  void grow(size_t count) {
+    if (count > n || !ptr)
+      alloc(std::max<size_t>(count, 1));
   }
   void upload(const T *host, size_t count) {
     alloc(count);

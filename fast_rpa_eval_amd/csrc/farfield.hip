@@ -562,13 +562,13 @@ void far_forward(FarPlan &P, const FarCall &c, int variant, double *dv, hipStrea
   if (variant == FRPA_FAR_TERMS) {
     FRPA_REQUIRE(P.terms_ok, FRPA_ERR_UNSUPPORTED,
                  "index_cc differs from the compiled term table; use FRPA_FAR_HORNER");
-    P.d_mc.alloc((size_t)P.mc_dim0 * P.n_atoms * c.T);
+    P.d_mc.grow((size_t)P.mc_dim0 * P.n_atoms * c.T);
     hipLaunchKernelGGL(far_init_mc_kernel, dim3(P.n_atoms, c.n_tile), dim3(256), 0, s, P.n_atoms, P.mc_dim0, P.dim0,
                        P.l_pot_max, lama, P.d_index_cc.ptr, P.d_cc.ptr, P.d_ncc.ptr, c.lmfd, c.moments, P.d_mc.ptr);
     check_launch("far_init_mc_kernel");
     launch_forward<FRPA_FAR_TERMS>(P, c, dv, s);
   } else if (variant == FRPA_FAR_HORNER) {
-    P.d_coef.alloc((size_t)P.coef_stride * P.n_atoms * c.T);
+    P.d_coef.grow((size_t)P.coef_stride * P.n_atoms * c.T);
     hipLaunchKernelGGL(far_init_horner_kernel, dim3(P.n_atoms, c.n_tile), dim3(256), 0, s, P.n_atoms,
                        P.coef_stride, P.dim0, P.l_pot_max, lama, P.d_index_cc.ptr, P.d_ncc.ptr, c.lmfd, c.moments,
                        P.d_hc_ptr.ptr, P.d_hc_term.ptr, P.d_hc_w.ptr, P.d_coef.ptr);
@@ -596,7 +596,7 @@ void far_adjoint(FarPlan &P, const FarCall &c, const double *gV, double *gM, hip
   int LM = P.l_pot_max <= 4 ? 4 : (P.l_pot_max <= 6 ? 6 : (P.l_pot_max <= 8 ? 8 : 10));
   FRPA_REQUIRE(P.l_pot_max <= 10, FRPA_ERR_UNSUPPORTED, "l_pot_max %d > 10", P.l_pot_max);
   const int nlm = (LM + 1) * (LM + 1);
-  P.d_partial.alloc((size_t)c.n_tile * P.n_hc * nbx * nlm);
+  P.d_partial.grow((size_t)c.n_tile * P.n_hc * nbx * nlm);
   dim3 grid(nbx, P.n_hc, c.n_tile);
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, P.d_centers.ptr, c.coord_points, c.map, c.n_valid, c.opr,

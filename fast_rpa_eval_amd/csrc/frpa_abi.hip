@@ -532,10 +532,32 @@ int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wa
     FRPA_REQUIRE(plan && wave, FRPA_ERR_INVALID, "bad arguments");
     FRPA_REQUIRE(precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4, FRPA_ERR_INVALID,
                  "prepare_wave applies to the INT8-split precisions");
-    long long b = ozaki_prepare_wave(*reinterpret_cast<Rho1Plan *>(plan), wave,
-                                     precision == FRPA_PREC_OZAKI ? 5 : 4, as_stream(stream));
+    Rho1Plan &P = *reinterpret_cast<Rho1Plan *>(plan);
+    const int S = precision == FRPA_PREC_OZAKI ? 5 : 4;
+    long long b = ozaki_prepare_wave(P, wave, S, as_stream(stream));
+    // the adjoint keeps digits of its own: rebuild them too once the adjoint has made them
+    if (adjoz_built(P))
+      adjoz_prepare(P, S, wave, as_stream(stream));
     if (bytes)
       *bytes = b;
+  });
+}
+
+int frpa_rho1_set_adjoint_wait_limit(frpa_rho1_plan *plan, int polls) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && polls >= 0, FRPA_ERR_INVALID, "bad arguments");
+    reinterpret_cast<Rho1Plan *>(plan)->adj_spin_limit = polls;
+  });
+}
+
+int frpa_rho1_adjoint_status(frpa_rho1_plan *plan, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
+    const int fails = adjoz_take_wait_failures(*reinterpret_cast<Rho1Plan *>(plan), as_stream(stream));
+    FRPA_REQUIRE(fails == 0, FRPA_ERR_HIP,
+                 "INT8 adjoint: %d slot waits gave up since the last check; the first_order_H of those calls is "
+                 "incomplete",
+                 fails);
   });
 }
 

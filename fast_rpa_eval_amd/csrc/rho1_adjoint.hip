@@ -528,7 +528,7 @@ void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const d
   const char *mode = getenv("FRPA_ADJOINT");
   if (precision != FRPA_PREC_FP64 || !(mode && strcmp(mode, "tiles") == 0)) {
     const size_t nn = (size_t)n * n * T;
-    st.d_hs.alloc(nn);
+    st.d_hs.grow(nn);
     fill_f64(st.d_hs.ptr, nn, 0.0, s);  // +0.0 and int64 0 have the same bits
     const double *bound = nullptr;      // non-null: Hs holds int64 fixed point (INT8 split)
     if (precision == FRPA_PREC_FP64) {
@@ -538,6 +538,18 @@ void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const d
     } else {
       bound = h1_ozaki_push(P, T, precision == FRPA_PREC_OZAKI ? 5 : 4, wave, g,
                             reinterpret_cast<unsigned long long *>(st.d_hs.ptr), s);
+      // a slot wait that gave up (here or in a graph replay since the last check) means an
+      // incomplete H: report it instead of adding it (not while the stream is being captured;
+      // frpa_rho1_adjoint_status covers graph replays)
+      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+      FRPA_HIP(hipStreamIsCapturing(s, &cap));
+      if (cap == hipStreamCaptureStatusNone) {
+        const int fails = adjoz_take_wait_failures(P, s);
+        FRPA_REQUIRE(fails == 0, FRPA_ERR_HIP,
+                     "INT8 adjoint: %d slot waits gave up (in this call or a graph replay since the last check); "
+                     "first_order_H was not updated by this call",
+                     fails);
+      }
     }
     hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, T), dim3(256), 0, s, st.d_hs.ptr,
                        R.d_pinv.ptr, n, bound, H);
@@ -574,11 +586,11 @@ void rho1_h1_gga(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, 
   const Relabel &R = get_relabel(P);
   const int n = P.n_basis;
   const size_t nn = (size_t)n * n * n_tile;
-  st.d_hs.alloc(std::max(st.d_hs.n, nn));
+  st.d_hs.grow(nn);
   fill_f64(st.d_hs.ptr, nn, 0.0, s);
   GgaIn in{gwave, hwave, avn, vsigma, v2rho2, v2rhosigma, v2sigma2, gradient_rho, pref1, grho1, vhg, rho1,
            partition_tab, n_atoms, j_atom_begin, j_coord, P.nmb, P.n_batches};
-  st.d_gga_cp.alloc(std::max(st.d_gga_cp.n, (size_t)(2 * P.wave_len)));
+  st.d_gga_cp.grow((size_t)(2 * P.wave_len));
   for (int t = 0; t < n_tile; t++) {
     hipLaunchKernelGGL(h1_gga_prep_kernel, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, R.d_srow.ptr, P.d_map.ptr,
                        wave, in, t, P.n_full, st.d_gga_cp.ptr);

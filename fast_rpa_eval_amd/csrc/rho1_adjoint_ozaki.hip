@@ -235,13 +235,16 @@ __global__ __launch_bounds__(256) void h1_bound_sum_kernel(const double *__restr
 // slot of their own; the other PC_NA waves issue every atomic from the slots.  A wave's
 // vector-memory counter is in order, so a multiplying wave that issued atomics would wait for their
 // memory-side completion before using its next phi-digit loads; with the atomics in other waves the
-// two overlap.  Slot handover: release / acquire flags in LDS, every wait bounded (a wave that gives
-// up leaves H incomplete rather than hanging; the count is added to *spin_fail).  Every wave stages
-// the Y digits; an atomic wave drains all blocks of tile I before the barrier that starts tile I + 1.
+// two overlap.  Slot handover: release / acquire flags in LDS, every wait bounded by spin_limit polls
+// (a producer waits for its slot to empty, an atomic wave for the next block; the count restarts at
+// every block taken).  A wave that gives up adds 1 to *spin_fail and goes on -- a producer drops its
+// block rather than overwrite a full slot -- so the kernel always ends and H is known to be incomplete
+// (the host reports it).  Every wave stages the Y digits; an atomic wave drains all blocks of tile I
+// before the barrier that starts tile I + 1, and the slot flags are cleared after that barrier.
 constexpr int PC_CW = 6;         // multiplying waves of the producer / consumer mode
 constexpr int PC_NA = 8 - PC_CW; // atomic waves; atomic wave aw serves the multiplying waves aw + PC_NA j
 constexpr int PC_PER = (PC_CW + PC_NA - 1) / PC_NA;
-constexpr int PC_SPIN = 1 << 16; // bound on the polls of one wait (each ~64 clocks: ~2 ms)
+constexpr int PC_SPIN = 1 << 16; // default bound on the polls of one wait (each ~64 clocks: ~2 ms)
 
 template <int S, bool PC>
 __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *__restrict__ descs,
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
                                                               const double *__restrict__ finv, int n_basis,
                                                               const double *__restrict__ bound,
                                                               unsigned long long *__restrict__ Hs,
-                                                              int *__restrict__ spin_fail) {
+                                                              int *__restrict__ spin_fail, int spin_limit) {
   constexpr int YROW = PMAX + 16;  // LDS bytes per (plane, row) of the staged Y tile
   __shared__ __attribute__((aligned(16))) signed char Ys[S * 64 * YROW];
   __shared__ double gs[PMAX];      // g(map(p), t) of the pass's points
@@ -276,8 +279,6 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
   const int lr = lane & 15, lk = lane >> 4;
   const int kw = w & 3, iw = w >> 2;
   const int nT = nc64 / 64;
-  if (PC && tid < PC_CW)
-    slot_full[tid] = 0;
 
   for (int pp = 0; pp < npk; pp += PMAX) {
     const int npp = min(PMAX, npk - pp);  // points of this pass (multiple of 64)
@@ -293,7 +294,9 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
       // 16-point groups w and w + 8 of the pass.
       const int a = tid & 63;
       const int ia = 64 * I + a;
-      __syncthreads();  // gs written; the previous tile's Y and scales are no longer read
+      __syncthreads();  // gs written; the previous tile's Y, scales and slots are no longer read
+      if (PC && tid < PC_CW)
+        slot_full[tid] = 0;  // read again only after the barrier that ends the staging
       double z[2][16];
       double mx = 0.0;
 #pragma unroll
@@ -415,32 +418,41 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
             if (c == nck - 1) {
               int J, kwu;
               unit_jk(st, J, kwu);
-              // wait until this wave's slot is empty, fill it, hand it over
+              // wait until this wave's slot is empty, fill it, hand it over; a wait that gives up
+              // drops the block (counted) instead of overwriting the one still in the slot
               int spins = 0;
-              while (__hip_atomic_load(&slot_full[w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0 &&
-                     ++spins < PC_SPIN)
-                __builtin_amdgcn_s_sleep(1);
-              if (spins >= PC_SPIN && lane == 0)
-                atomicAdd(spin_fail, 1);
-              // acc[l][gi][r] = C_l(k = 64 J + 16 kwu + 4 lk + r, i = 64 I + 16 gi + lr)
-#pragma unroll
-              for (int gi = 0; gi < 4; gi++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                  const int k = 64 * J + 16 * kwu + 4 * lk + r;
-                  double v = (double)acc[S - 1][gi][r];
-#pragma unroll
-                  for (int l = S - 2; l >= 0; l--)
-                    v = v * (1.0 / RADIX) + (double)acc[l][gi][r];
-                  slot[w][(4 * lk + r) * 64 + 16 * gi + lr] =
-                      k < nc ? (long long)rint(v * ysc[gi] * fi[k] * scale) : 0ll;
+              bool free_slot = true;
+              while (__hip_atomic_load(&slot_full[w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0) {
+                if (++spins >= spin_limit) {
+                  free_slot = false;
+                  break;
                 }
-              if (lane < 16) {
-                const int k = 64 * J + 16 * kwu + lane;
-                slot_cols[w][lane] = k < nc ? sib[k] : -1;
+                __builtin_amdgcn_s_sleep(1);
               }
-              if (lane == 0)
-                __hip_atomic_store(&slot_full[w], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+              if (!free_slot) {
+                if (lane == 0)
+                  atomicAdd(spin_fail, 1);
+              } else {
+                // acc[l][gi][r] = C_l(k = 64 J + 16 kwu + 4 lk + r, i = 64 I + 16 gi + lr)
+#pragma unroll
+                for (int gi = 0; gi < 4; gi++)
+#pragma unroll
+                  for (int r = 0; r < 4; r++) {
+                    const int k = 64 * J + 16 * kwu + 4 * lk + r;
+                    double v = (double)acc[S - 1][gi][r];
+#pragma unroll
+                    for (int l = S - 2; l >= 0; l--)
+                      v = v * (1.0 / RADIX) + (double)acc[l][gi][r];
+                    slot[w][(4 * lk + r) * 64 + 16 * gi + lr] =
+                        k < nc ? (long long)rint(v * ysc[gi] * fi[k] * scale) : 0ll;
+                  }
+                if (lane < 16) {
+                  const int k = 64 * J + 16 * kwu + lane;
+                  slot_cols[w][lane] = k < nc ? sib[k] : -1;
+                }
+                if (lane == 0)
+                  __hip_atomic_store(&slot_full[w], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+              }
             }
 #pragma unroll
             for (int sl = 0; sl < S; sl++) {
@@ -463,8 +475,8 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
           const int i = 64 * I + lane;
           const int xi = i < nc ? sib[i] : 0;
           const size_t x = (size_t)xi;
-          int spins = 0;
-          while (total > 0 && spins < PC_SPIN) {
+          int spins = 0;  // polls since the last block taken
+          while (total > 0) {
             bool got = false;
 #pragma unroll
             for (int e = 0; e < PC_PER; e++) {
@@ -491,8 +503,11 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
                 got = true;
               }
             }
-            if (!got) {
-              spins++;
+            if (got) {
+              spins = 0;
+            } else {
+              if (++spins >= spin_limit)
+                break;
               __builtin_amdgcn_s_sleep(1);
             }
           }
@@ -722,38 +737,59 @@ void run(Rho1Plan &P, AdjOzState &st, int T, const double *wave, const double *g
          hipStream_t s) {
   const int nvp = (int)st.d_pmax.n;
   const int bblocks = std::max(1, std::min((nvp + 255) / 256, 4096));
-  st.d_bound.alloc(std::max<size_t>(st.d_bound.n, (size_t)T));
-  st.d_bpart.alloc(std::max<size_t>(st.d_bpart.n, (size_t)bblocks * T));
+  st.d_bound.grow((size_t)T);
+  st.d_bpart.grow((size_t)bblocks * T);
   hipLaunchKernelGGL(h1_bound_kernel, dim3(bblocks, T), dim3(256), 0, s, P.d_map.ptr, g, P.n_full, st.d_pmax.ptr,
                      nvp, st.d_bpart.ptr);
   check_launch("h1_bound_kernel");
   hipLaunchKernelGGL(h1_bound_sum_kernel, dim3(T), dim3(256), 0, s, st.d_bpart.ptr, bblocks, st.d_bound.ptr);
   check_launch("h1_bound_sum_kernel");
-  st.d_spin_fail.alloc(1);
-  fill_i32(st.d_spin_fail.ptr, 1, 0, s);
   const char *pc = std::getenv("FRPA_ADJOINT_PC");
   auto kern = (pc && pc[0] == '0') ? h1_ozaki_kernel<S, false> : h1_ozaki_kernel<S, true>;
+  const int spin_limit = P.adj_spin_limit > 0 ? P.adj_spin_limit : PC_SPIN;
   if (st.n_groups) {
     hipLaunchKernelGGL(kern, dim3(st.n_groups, T), dim3(AZ_THREADS), 0, s, st.d_gdesc.ptr, st.d_off.ptr,
                        st.d_gsib.ptr, P.d_map.ptr, g, P.n_full, st.d_fdig.ptr, st.d_finv.ptr, P.n_basis,
-                       st.d_bound.ptr, Hs, st.d_spin_fail.ptr);
+                       st.d_bound.ptr, Hs, st.d_spin_fail.ptr, spin_limit);
     check_launch("h1_ozaki_kernel");
-  }
-  // FRPA_CHECK_ADJOINT_WAITS=1: wait for the kernel and fail if a slot handover gave up (tests; not
-  // while the stream is being captured)
-  const char *chk = std::getenv("FRPA_CHECK_ADJOINT_WAITS");
-  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  if (chk && chk[0] == '1')
-    FRPA_HIP(hipStreamIsCapturing(s, &cap));
-  if (chk && chk[0] == '1' && cap == hipStreamCaptureStatusNone) {
-    int fails = 0;
-    FRPA_HIP(hipMemcpyAsync(&fails, st.d_spin_fail.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-    FRPA_HIP(hipStreamSynchronize(s));
-    FRPA_REQUIRE(fails == 0, FRPA_ERR_HIP, "INT8 adjoint: %d slot waits gave up", fails);
   }
 }
 
 } // namespace
+
+// (Re)builds the adjoint's groups and phi digits for this wave (once per wave; frpa_rho1_prepare_wave
+// calls it again after the wave changed in place).
+void adjoz_prepare(Rho1Plan &P, int S, const double *wave, hipStream_t s) {
+  FRPA_REQUIRE(S == 4 || S == 5, FRPA_ERR_INVALID, "INT8 split supports 4 or 5 slices, not %d", S);
+  if (!P.adjoz)
+    P.adjoz = new AdjOzState();
+  const double merge = merge_threshold();
+  if (S == 5)
+    prepare_f<5>(P, *P.adjoz, wave, merge, s);
+  else
+    prepare_f<4>(P, *P.adjoz, wave, merge, s);
+}
+
+// Waits of the INT8 adjoint that gave up since the last read, read and cleared (synchronises s).
+int adjoz_take_wait_failures(Rho1Plan &P, hipStream_t s) {
+  if (!P.adjoz || !P.adjoz->d_spin_fail.ptr)
+    return 0;
+  int fails = 0;
+  FRPA_HIP(hipMemcpyAsync(&fails, P.adjoz->d_spin_fail.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+  FRPA_HIP(hipStreamSynchronize(s));
+  if (fails) {
+    fill_i32(P.adjoz->d_spin_fail.ptr, 1, 0, s);
+    FRPA_HIP(hipStreamSynchronize(s));
+  }
+  return fails;
+}
+
+void adjoz_invalidate(Rho1Plan &P) {
+  if (P.adjoz)
+    P.adjoz->wave = nullptr;
+}
+
+bool adjoz_built(const Rho1Plan &P) { return P.adjoz && P.adjoz->wave; }
 
 const double *h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *g, unsigned long long *Hs,
                             hipStream_t s) {
@@ -761,13 +797,13 @@ const double *h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const
   if (!P.adjoz)
     P.adjoz = new AdjOzState();
   AdjOzState &st = *P.adjoz;
-  const double merge = merge_threshold();
-  if (st.wave != wave || st.S != S || st.merge != merge) {
-    if (S == 5)
-      prepare_f<5>(P, st, wave, merge, s);
-    else
-      prepare_f<4>(P, st, wave, merge, s);
+  if (!st.d_spin_fail.ptr) {  // zeroed once; cleared again only when read (adjoz_take_wait_failures)
+    st.d_spin_fail.alloc(1);
+    fill_i32(st.d_spin_fail.ptr, 1, 0, s);
   }
+  const double merge = merge_threshold();
+  if (st.wave != wave || st.S != S || st.merge != merge)
+    adjoz_prepare(P, S, wave, s);
   if (S == 5)
     run<5>(P, st, T, wave, g, Hs, s);
   else

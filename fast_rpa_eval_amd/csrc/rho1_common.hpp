@@ -72,6 +72,7 @@ struct Rho1Plan {
   DevBuf<double> d_b2_tab;          // (b2_cap, n_basis): column c holds the R rows at c
   int b2_cap = 0;
   LaunchCfg cfg[3];                 // per precision (FP64, INT8 split S = 5, S = 4)
+  int adj_spin_limit = 0;           // polls per slot wait of the INT8 adjoint (0 = its default)
   ~Rho1Plan();
 };
 
@@ -125,6 +126,12 @@ void adjoz_release(AdjOzState *st);
 // scratch Hs; returns the device array of the per-perturbation bounds B that fix the scales
 const double *h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *g, unsigned long long *Hs,
                             hipStream_t s);
+// INT8-split adjoint digits: (re)build for a wave, forget them (next call rebuilds), whether built
+void adjoz_prepare(Rho1Plan &P, int S, const double *wave, hipStream_t s);
+void adjoz_invalidate(Rho1Plan &P);
+bool adjoz_built(const Rho1Plan &P);
+// Slot waits of the INT8 adjoint that gave up since the last read (read and cleared; synchronises s)
+int adjoz_take_wait_failures(Rho1Plan &P, hipStream_t s);
 
 // Scale 2^E of the adjoint's fixed-point accumulation for the bound B >= |every partial sum|:
 // 2 B 2^E <= 2^61 (1 when B = 0)

@@ -495,9 +495,9 @@ void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   const int n = P.n_basis;
   const size_t nbb = (size_t)n * n * c.T;
   const size_t nr = (size_t)n * c.T;
-  st.d_adig.alloc(nbb);
-  st.d_ainv.alloc(nr);
-  st.d_rowbits.alloc(nr);
+  st.d_adig.grow(nbb);
+  st.d_ainv.grow(nr);
+  st.d_rowbits.grow(nr);
   fill_f64(reinterpret_cast<double *>(st.d_rowbits.ptr), nr, 0.0, s);  // +0.0 has all-zero bits
   const int chunk = 256;
   hipLaunchKernelGGL(p1_rowmax_kernel, dim3((n + 255) / 256, (n + chunk - 1) / chunk, c.T), dim3(256), 0, s, c.P1,

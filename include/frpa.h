@@ -131,11 +131,25 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
 int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave);
 
 /* One-off conversion of wave_batches_compress into the INT8 digits used by the INT8-split
- * precisions (kept in the plan and reused while the same wave pointer is passed; call again after
- * changing wave in place).  Optional: the first INT8-split forward converts on demand.
- * Returns the device bytes held in *bytes (nullable). */
+ * precisions.  The forward and the adjoint each keep digits of their own in the plan, made on their
+ * first INT8-split call and reused while the same wave pointer is passed: after changing the wave
+ * in place (or passing new contents at the same address), call this function, which rebuilds the
+ * forward's digits and, once the adjoint has made its own, the adjoint's.  Optional otherwise.
+ * Returns the device bytes held by the forward's digits in *bytes (nullable). */
 int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wave, long long *bytes,
                            void *stream);
+
+/* The INT8-split adjoint hands blocks between its waves through LDS slots; every wait for a slot is
+ * bounded (default 65536 polls, ~2 ms).  A wait that gives up leaves first_order_H incomplete and
+ * is counted on the device: frpa_rho1_adjoint reads the count on every call not under stream
+ * capture and returns FRPA_ERR_HIP (first_order_H then not updated by that call);
+ * frpa_rho1_adjoint_status reads and clears it (synchronising the stream) and returns FRPA_ERR_HIP
+ * if any wait gave up since the last check -- the check for calls replayed from a HIP graph.
+ * frpa_rho1_set_adjoint_wait_limit sets the bound in polls (0 = default; small values make the waits
+ * give up, for tests of this path).  No counterpart in the reference (its H integration is a CPU
+ * function, evaluate_first_order_H.cpp:13-222). */
+int frpa_rho1_set_adjoint_wait_limit(frpa_rho1_plan *plan, int polls);
+int frpa_rho1_adjoint_status(frpa_rho1_plan *plan, void *stream);
 
 /*
  * Full first-order H (evaluate_first_order_H.cpp:13-222, LDA and GGA terms; a CPU function in the

@@ -109,8 +109,7 @@ def test_h1_int8_split_is_bit_reproducible(cuda, precision):
 def test_h1_int8_split_producer_consumer_matches_direct(cuda, monkeypatch):
     """The default INT8 adjoint hands each tile-pair block from the multiplying waves to atomic waves
     through LDS slots; with FRPA_ADJOINT_PC=0 every wave issues its own atomics.  Both add the same
-    int64 values, so H is the same bits; every slot wait completes (FRPA_CHECK_ADJOINT_WAITS)."""
-    monkeypatch.setenv("FRPA_CHECK_ADJOINT_WAITS", "1")
+    int64 values, so H is the same bits; every slot wait completes (each call checks the count)."""
     prob = synth.make_rho1_problem(40, 16.0, 160, (1, 600), (1, 300), config_k=6, zero_nc=(3,))
     T = 2
     g = api.to_device(np.asfortranarray(np.random.default_rng(4).standard_normal((prob.n_full_points, T))))
@@ -130,7 +129,6 @@ def test_h1_int8_split_merged_groups_against_oracle(cuda, oracle, monkeypatch):
     batch with n_compute_c = 0 (its points break the contiguity) and one with no points are in the
     list.  Merged groups have other digit scales, so H differs in the last bits from the unmerged one,
     which shows the groups were formed."""
-    monkeypatch.setenv("FRPA_CHECK_ADJOINT_WAITS", "1")
     prob = synth.make_rho1_problem(40, 16.0, 200, (20, 120), (40, 300), config_k=7, zero_nc=(5,), zero_np=(9,))
     T = 2
     g_host = np.asfortranarray(np.random.default_rng(8).standard_normal((prob.n_full_points, T)))
@@ -147,3 +145,77 @@ def test_h1_int8_split_merged_groups_against_oracle(cuda, oracle, monkeypatch):
         assert err <= 1e-10
         out[m] = H
     assert not torch.equal(out["0"], out["4"])
+
+
+def test_h1_int8_split_wait_failures_are_reported(cuda, oracle):
+    """Slot waits of the INT8 adjoint that give up (forced from the host with a bound of one poll,
+    so that the atomic waves stop waiting for blocks still being multiplied) are counted on the
+    device: the call returns an error and leaves H untouched, every kernel ends normally, and with the
+    default bound restored the same plan gives the oracle's H again.  The same count under graph
+    capture is read by frpa_rho1_adjoint_status."""
+    prob = synth.make_rho1_problem(40, 16.0, 160, (1, 300), (100, 400), config_k=6)
+    T = 1
+    g_host = np.asfortranarray(np.random.default_rng(5).standard_normal((prob.n_full_points, T)))
+    ref = _oracle_h1(oracle, prob, g_host, T)
+    plan = api.Rho1Plan.from_problem(prob)
+    wave, g = api.to_device(prob.wave), api.to_device(g_host)
+    plan.set_adjoint_wait_limit(1)
+    H = torch.full((prob.n_basis * prob.n_basis * T,), 7.0, dtype=torch.float64, device="cuda")
+    with pytest.raises(api.FrpaError, match="slot waits gave up"):
+        plan.adjoint(wave, g, T=T, precision=api.OZAKI, H=H)
+    torch.cuda.synchronize()
+    assert bool(torch.all(H == 7.0)), "H changed by a call that reported failed waits"
+    plan.adjoint_status()  # the failing call has read and cleared the count
+    # under capture the call cannot check (its digits and groups were built by the call above): the
+    # graph's replay leaves the count for adjoint_status
+    stream = torch.cuda.Stream()
+    Hg = torch.zeros(prob.n_basis * prob.n_basis * T, dtype=torch.float64, device="cuda")
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        plan.adjoint(wave, g, T=T, precision=api.OZAKI, H=Hg, stream=torch.cuda.current_stream())
+    graph.replay()
+    torch.cuda.synchronize()
+    with pytest.raises(api.FrpaError, match="slot waits gave up"):
+        plan.adjoint_status()
+    plan.adjoint_status()  # cleared by the read
+    plan.set_adjoint_wait_limit(0)
+    H = plan.adjoint(wave, g, T=T, precision=api.OZAKI)
+    got = H.cpu().numpy().reshape((prob.n_basis, prob.n_basis, T), order="F")
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"[h1 int8 split after failed waits] rel {err:.3e}", flush=True)
+    assert err <= 1e-10
+    plan.adjoint_status()
+
+
+def test_h1_int8_split_wave_changed_in_place(cuda, oracle):
+    """The INT8 adjoint keeps digits of the wave it first saw.  After the wave tensor is changed in
+    place, frpa_rho1_prepare_wave rebuilds them (the C-ABI contract), and the host mirror does so by
+    itself (it follows the tensor's version counter): both give the oracle's H for the new wave."""
+    prob = synth.make_rho1_problem(30, 15.0, 80, (20, 150), (30, 200), config_k=9)
+    T = 1
+    rng = np.random.default_rng(6)
+    g_host = np.asfortranarray(rng.standard_normal((prob.n_full_points, T)))
+    plan = api.Rho1Plan.from_problem(prob)
+    wave, g = api.to_device(prob.wave), api.to_device(g_host)
+    plan.adjoint(wave, g, T=T, precision=api.OZAKI)  # digits of the first wave
+    prob2 = synth.make_rho1_problem(30, 15.0, 80, (20, 150), (30, 200), config_k=9)
+    prob2.wave = synth.random_like_wave(prob2, 1, 77)
+    ref2 = _oracle_h1(oracle, prob2, g_host, T)
+    # C-ABI: change in place, then frpa_rho1_prepare_wave (bypassing the mirror's own tracking)
+    wave.copy_(api.to_device(prob2.wave))
+    plan.prepare_wave(wave, api.OZAKI)
+    H = torch.zeros(prob.n_basis * prob.n_basis * T, dtype=torch.float64, device="cuda")
+    api._lib.check(api.lib().frpa_rho1_adjoint(plan.handle, T, api.OZAKI, api._dp(wave), api._dp(g), api._dp(H),
+                                               api._stream()), "frpa_rho1_adjoint")
+    got = H.cpu().numpy().reshape((prob.n_basis, prob.n_basis, T), order="F")
+    err = np.abs(got - ref2).max() / np.abs(ref2).max()
+    print(f"[h1 int8 split, wave changed + prepare_wave] rel {err:.3e}", flush=True)
+    assert err <= 1e-10
+    # host mirror: change in place again (back to the first wave), no explicit call
+    ref1 = _oracle_h1(oracle, prob, g_host, T)
+    wave.copy_(api.to_device(prob.wave))
+    got = plan.adjoint(wave, g, T=T, precision=api.OZAKI).cpu().numpy().reshape(
+        (prob.n_basis, prob.n_basis, T), order="F")
+    err = np.abs(got - ref1).max() / np.abs(ref1).max()
+    print(f"[h1 int8 split, wave changed, mirror refresh] rel {err:.3e}", flush=True)
+    assert err <= 1e-10

@@ -51,6 +51,7 @@ def lib():
         L.frpa_rho1_bias_part2.argtypes = [p, i, i, i, i, i, p, p, p, p, p, p]
         L.frpa_rho1_h1.argtypes = [p, i, i, i, i, i] + [p] * 16
         L.frpa_near_forward.argtypes = [p, i, i, i, i, i, p, p, p, p, p, p, p]
+        L.frpa_far_forward_noalloc.argtypes = [p, i, i, i, i, i, p, p, p, p, p, p, p, p]
         L.frpa_rho1_set_adjoint_wait_limit.argtypes = [p, i]
         L.frpa_rho1_adjoint_status.argtypes = [p, p]
         _lib = L

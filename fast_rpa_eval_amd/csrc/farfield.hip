@@ -19,9 +19,16 @@
 //        (1/r) sum_q u^q Q_q(X, Y, Z),   Q_q of degree l - 2q in X, Y, Z (X degree <= 1),
 //    evaluated by nested Horner: one FMA per merged coefficient (165 at l = 8 vs 425 terms).
 #include <cstdlib>
+#include <mutex>
 #include <utility>
 
 #include "farfield_common.hpp"
+
+// Coefficient table of frpa_far_forward_noalloc: a fixed array in every device's memory (held until
+// the process ends), used in place of the plan's per-call buffer so that those calls allocate
+// nothing.  Host code never passes this name to a kernel (on the host it is a stand-in, not a
+// device address); far_scratch() returns the device's own address of it.
+static __device__ __align__(256) double scratch[FRPA_FAR_NOALLOC_SCRATCH_DOUBLES];
 
 namespace frpa {
 namespace {
@@ -509,11 +516,11 @@ __global__ void far_adjoint_reduce_kernel(int n_atoms, int n_hc, int nbx, int nl
 }
 
 template <int N>
-void launch_horner(FarPlan &P, const FarCall &c, double *dv, hipStream_t s) {
+void launch_horner(FarPlan &P, const FarCall &c, const double *coef, double *dv, hipStream_t s) {
   dim3 grid((c.n_valid + 256 * N - 1) / (256 * N), c.n_tile);
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, P.d_centers.ptr, P.n_hc, c.coord_points, c.map, c.n_valid, c.opr,
-                       P.l_pot_max + 1, c.mrsq, P.n_atoms, P.d_coef.ptr, P.coef_stride, dv, c.n_full);
+                       P.l_pot_max + 1, c.mrsq, P.n_atoms, coef, P.coef_stride, dv, c.n_full);
   };
   switch (P.l_pot_max) {
 #define FRPA_CASE(LL)                                                                                          \
@@ -530,9 +537,8 @@ void launch_horner(FarPlan &P, const FarCall &c, double *dv, hipStream_t s) {
 }
 
 template <int VARIANT>
-void launch_forward(FarPlan &P, const FarCall &c, double *dv, hipStream_t s) {
+void launch_forward(FarPlan &P, const FarCall &c, const double *coefs, double *dv, hipStream_t s) {
   dim3 grid((c.n_valid + 255) / 256, c.n_tile);
-  const double *coefs = VARIANT == FRPA_FAR_HORNER ? P.d_coef.ptr : P.d_mc.ptr;
   const int cstride = VARIANT == FRPA_FAR_HORNER ? P.coef_stride : P.mc_dim0;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, P.d_centers.ptr, P.n_hc, c.coord_points, c.map, c.n_valid, c.opr,
@@ -555,37 +561,88 @@ void launch_forward(FarPlan &P, const FarCall &c, double *dv, hipStream_t s) {
 
 } // namespace
 
-void far_forward(FarPlan &P, const FarCall &c, int variant, double *dv, hipStream_t s) {
+size_t far_coef_doubles(const FarPlan &P, int variant, int T) {
+  const size_t stride = variant == FRPA_FAR_TERMS ? (size_t)P.mc_dim0 : (size_t)P.coef_stride;
+  return stride * (size_t)P.n_atoms * (size_t)T;
+}
+
+void far_forward(FarPlan &P, const FarCall &c, int variant, double *dv, hipStream_t s, double *coef_buf) {
   if (c.n_valid == 0 || c.n_tile == 0 || P.n_hc == 0)
     return;
   const int lama = P.lama;
   if (variant == FRPA_FAR_TERMS) {
     FRPA_REQUIRE(P.terms_ok, FRPA_ERR_UNSUPPORTED,
                  "index_cc differs from the compiled term table; use FRPA_FAR_HORNER");
-    P.d_mc.grow((size_t)P.mc_dim0 * P.n_atoms * c.T);
+    double *mc = coef_buf;
+    if (!mc) {
+      P.d_mc.grow(far_coef_doubles(P, variant, c.T));
+      mc = P.d_mc.ptr;
+    }
     hipLaunchKernelGGL(far_init_mc_kernel, dim3(P.n_atoms, c.n_tile), dim3(256), 0, s, P.n_atoms, P.mc_dim0, P.dim0,
-                       P.l_pot_max, lama, P.d_index_cc.ptr, P.d_cc.ptr, P.d_ncc.ptr, c.lmfd, c.moments, P.d_mc.ptr);
+                       P.l_pot_max, lama, P.d_index_cc.ptr, P.d_cc.ptr, P.d_ncc.ptr, c.lmfd, c.moments, mc);
     check_launch("far_init_mc_kernel");
-    launch_forward<FRPA_FAR_TERMS>(P, c, dv, s);
+    launch_forward<FRPA_FAR_TERMS>(P, c, mc, dv, s);
   } else if (variant == FRPA_FAR_HORNER) {
-    P.d_coef.grow((size_t)P.coef_stride * P.n_atoms * c.T);
+    double *coef = coef_buf;
+    if (!coef) {
+      P.d_coef.grow(far_coef_doubles(P, variant, c.T));
+      coef = P.d_coef.ptr;
+    }
     hipLaunchKernelGGL(far_init_horner_kernel, dim3(P.n_atoms, c.n_tile), dim3(256), 0, s, P.n_atoms,
                        P.coef_stride, P.dim0, P.l_pot_max, lama, P.d_index_cc.ptr, P.d_ncc.ptr, c.lmfd, c.moments,
-                       P.d_hc_ptr.ptr, P.d_hc_term.ptr, P.d_hc_w.ptr, P.d_coef.ptr);
+                       P.d_hc_ptr.ptr, P.d_hc_term.ptr, P.d_hc_w.ptr, coef);
     check_launch("far_init_horner_kernel");
     // points per lane of the Horner kernel (FRPA_FAR_POINTS_PER_LANE = 1, 2 or 4; default 4: l = 8
     // over 6M points x 1024 centres runs 73 / 46 / 45 ms)
     const char *e = getenv("FRPA_FAR_POINTS_PER_LANE");
     const int ppl = e ? atoi(e) : 4;
     if (ppl == 1)
-      launch_forward<FRPA_FAR_HORNER>(P, c, dv, s);
+      launch_forward<FRPA_FAR_HORNER>(P, c, coef, dv, s);
     else if (ppl == 2)
-      launch_horner<2>(P, c, dv, s);
+      launch_horner<2>(P, c, coef, dv, s);
     else
-      launch_horner<4>(P, c, dv, s);
+      launch_horner<4>(P, c, coef, dv, s);
   } else {
     FRPA_REQUIRE(false, FRPA_ERR_INVALID, "unknown far-field variant %d", variant);
   }
+}
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[SCRATCH_MAX_DEVICES];  // device address of `scratch` per device, resolved once
+
+// Address of the fixed coefficient array on the current device (looked up on the first call on it,
+// with that device current).
+double *far_scratch() {
+  int dev = -1;
+  FRPA_HIP(hipGetDevice(&dev));
+  FRPA_REQUIRE(dev >= 0 && dev < SCRATCH_MAX_DEVICES, FRPA_ERR_UNSUPPORTED, "device %d: at most %d devices", dev,
+               SCRATCH_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    FRPA_REQUIRE(e == hipSuccess && p, FRPA_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                 hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  return g_scratch[dev];
+}
+} // namespace
+
+void far_forward_noalloc(FarPlan &P, const FarCall &c, int variant, double *dv, hipStream_t s) {
+  FRPA_REQUIRE(variant == FRPA_FAR_TERMS || variant == FRPA_FAR_HORNER, FRPA_ERR_INVALID,
+               "unknown far-field variant %d", variant);
+  const size_t need = far_coef_doubles(P, variant, c.T);
+  FRPA_REQUIRE(need <= (size_t)FRPA_FAR_NOALLOC_SCRATCH_DOUBLES, FRPA_ERR_UNSUPPORTED,
+               "far forward without allocation: the coefficient table needs %zu doubles (%d per atom x %d atoms x "
+               "atom_tile_size %d), the fixed scratch holds %d",
+               need, variant == FRPA_FAR_TERMS ? P.mc_dim0 : P.coef_stride, P.n_atoms, c.T,
+               FRPA_FAR_NOALLOC_SCRATCH_DOUBLES);
+  if (c.n_valid == 0 || c.n_tile == 0 || P.n_hc == 0)
+    return;
+  far_forward(P, c, variant, dv, s, far_scratch());
 }
 
 void far_adjoint(FarPlan &P, const FarCall &c, const double *gV, double *gM, hipStream_t s) {

@@ -112,7 +112,14 @@ FarPlan *make_far_plan(int l_pot_max, int lama, int dim0, int n_centers, int n_h
                        int hfla, const int *species, const int *l_hartree, const int *chp, const int *center_to_atom,
                        const double *coords_center, const int *index_cc, const double *cc, const int *n_cc_lm_ijk,
                        const int *index_ijk_max_cc);
-void far_forward(FarPlan &P, const FarCall &c, int variant, double *dv, hipStream_t s);
+// Per-call coefficient table of a forward: multipole_c (FRPA_FAR_TERMS) or the merged Horner
+// coefficients, (stride, n_atoms, T) doubles.
+size_t far_coef_doubles(const FarPlan &P, int variant, int T);
+// coef_buf: where the per-call coefficient table goes (at least far_coef_doubles doubles); nullptr =
+// the plan's own buffer, allocated on first use and grown when a call needs more.
+void far_forward(FarPlan &P, const FarCall &c, int variant, double *dv, hipStream_t s, double *coef_buf = nullptr);
+// far_forward with the coefficient table in a fixed device array (no allocation; see frpa.h)
+void far_forward_noalloc(FarPlan &P, const FarCall &c, int variant, double *dv, hipStream_t s);
 void far_adjoint(FarPlan &P, const FarCall &c, const double *gV, double *gM, hipStream_t s);
 void set_near_tables(FarPlan &P, int n_species, const int *species_center, const int *n_grid,
                      const double *r_grid_min, const double *log_r_grid_inc);

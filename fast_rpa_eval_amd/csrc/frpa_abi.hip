@@ -627,6 +627,20 @@ int frpa_far_forward(frpa_far_plan *plan, int atom_tile_size, int n_tile, int va
   });
 }
 
+int frpa_far_forward_noalloc(frpa_far_plan *plan, int atom_tile_size, int n_tile, int variant, int n_full_points,
+                             int n_valid_points, const double *coord_points,
+                             const int *i_valid_point_2_i_full_points_map, const int *l_hartree_max_far_distance,
+                             const double *multipole_radius_sq, const double *outer_potential_radius,
+                             const double *multipole_moments, double *delta_v_hartree, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan && delta_v_hartree, FRPA_ERR_INVALID, "null argument");
+    FarCall c = far_call(atom_tile_size, n_tile, n_full_points, n_valid_points, coord_points,
+                         i_valid_point_2_i_full_points_map, l_hartree_max_far_distance, multipole_radius_sq,
+                         outer_potential_radius, multipole_moments);
+    far_forward_noalloc(*reinterpret_cast<FarPlan *>(plan), c, variant, delta_v_hartree, as_stream(stream));
+  });
+}
+
 int frpa_far_adjoint(frpa_far_plan *plan, int atom_tile_size, int n_tile, int n_full_points, int n_valid_points,
                      const double *coord_points, const int *i_valid_point_2_i_full_points_map,
                      const int *l_hartree_max_far_distance, const double *multipole_radius_sq,

@@ -237,6 +237,26 @@ int frpa_far_forward(frpa_far_plan *plan, int atom_tile_size, int n_tile, int va
                      double *delta_v_hartree, void *stream);
 
 /*
+ * frpa_far_forward without any allocation: same arguments, kernels and result bits, for callers that
+ * cannot allocate device memory during the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The per-call coefficient
+ * table ((n_cc(l_pot_max) for FRPA_FAR_TERMS, or the merged Horner count for FRPA_FAR_HORNER) x
+ * n_atoms x atom_tile_size doubles; the Horner count is 495 at l_pot_max = 8, 1001 at 10) goes into a
+ * fixed array of FRPA_FAR_NOALLOC_SCRATCH_DOUBLES doubles (32 MiB) that each device the library runs
+ * on holds until the process ends; a call needing more returns FRPA_ERR_UNSUPPORTED before launching
+ * anything.  The plan must exist (frpa_far_plan_create allocates; this call does not).  The first
+ * call on a device looks up the array's address there (the current device must be the stream's).
+ * Every call on a device shares that device's array: calls that overlap on one GPU (other streams,
+ * other threads) race; calls in one stream are ordered.
+ */
+#define FRPA_FAR_NOALLOC_SCRATCH_DOUBLES 4194304
+int frpa_far_forward_noalloc(frpa_far_plan *plan, int atom_tile_size, int n_tile, int variant, int n_full_points,
+                             int n_valid_points, const double *coord_points,
+                             const int *i_valid_point_2_i_full_points_map, const int *l_hartree_max_far_distance,
+                             const double *multipole_radius_sq, const double *outer_potential_radius,
+                             const double *multipole_moments, double *delta_v_hartree, void *stream);
+
+/*
  * Adjoint with respect to the multipole moments:
  *   grad_moments(lm, a, t) += sum_points gV(p, t) d delta_v(p, t) / d M(lm, a, t)
  * gV (n_full_points, T) device; grad_moments ((l_pot_max+1)^2, n_atoms, T) device, ACCUMULATED.

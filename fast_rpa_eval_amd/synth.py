@@ -216,13 +216,15 @@ def _batch_layout(nps, ncs, extra_rng):
 
 
 def make_rho1_problem(n_atoms, box_ang, n_batches, np_range, nc_range=None, config_k=1, fill_wave=True,
-                      batch_spread_bohr=1.5, zero_nc=(), zero_np=()):
+                      batch_spread_bohr=1.5, zero_nc=(), zero_np=(), host_atoms=None):
     """Batched rho1 inputs.
 
     nc_range None: natural compute sets (basis functions with max_batch |phi| > 1e-10, the rule of
     the small CPU-runnable configuration); otherwise nc ~ U{nc_range} nearest basis functions.
     np_range: int (fixed) or (lo, hi) inclusive.
     zero_nc / zero_np: batch indices forced to n_compute_c = 0 / n_points = 0 (edge cases).
+    host_atoms: atoms the batches are centred on, in turn (default: all atoms); a few atoms give a
+    few hundred batches the density of a full grid (tens of batches per atom).
     """
     sd = seeds(config_k)
     xyz = make_geometry(n_atoms, box_ang * ANG, 1.5 * ANG, sd["geometry"])
@@ -233,7 +235,8 @@ def make_rho1_problem(n_atoms, box_ang, n_batches, np_range, nc_range=None, conf
     else:
         nps = rng.integers(np_range[0], np_range[1] + 1, size=n_batches)
     nps[list(zero_np)] = 0
-    host = np.arange(n_batches) % n_atoms
+    hosts = np.arange(n_atoms) if host_atoms is None else np.asarray(host_atoms)
+    host = hosts[np.arange(n_batches) % len(hosts)]
     if nc_range is None:
         centres = xyz[host]
     else:

@@ -5,14 +5,23 @@
 // init_dfpt_device_data and used by the *_cu_host_ functions.  The same lifecycle is kept here:
 // init_dfpt_device_info creates the stream, init_dfpt_device_data builds the rho1 and far-field
 // plans and uploads the constant arrays; the *_cu_host_ functions upload their per-call arrays,
-// run on the device and copy the results back.  Errors print and exit(1), as DEV_CHECK
-// (device.hpp:19-42).
+// run on the device and copy the results back -- or, in the device-resident mode that the reference
+// selects with ALL_KERNELS_ON_GPU (evaluate_first_order_rho.cu:17,259-263,363-367), the rho1
+// function reads its per-call inputs from and leaves its outputs in device arrays that the caller
+// reaches through frpa_ref_device_array.  The reference's CPU functions on the same path
+// (evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_, sum_up_whole_potential_c_v3_
+// atoms_full_points_j_atom_tile_, evaluate_first_order_h_reduce_memory_c_v2_,
+// generate_first_order_rho_bias_part2_) run on the GPU too.  Errors print and exit(1), as DEV_CHECK
+// (device.hpp:19-42).  The implementations live in namespace frpa_ref; the C-linkage symbols are
+// defined below, the C++-linkage ones in reference_abi_cxx.hip.
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 
 #include "farfield_common.hpp"
+#include "reference_impl.hpp"
 #include "rho1_common.hpp"
 
 namespace {
@@ -23,6 +32,7 @@ struct RefState {
   bool info = false;
   hipStream_t stream = nullptr;
   bool data = false;
+  bool resident = false;  // ALL_KERNELS_ON_GPU: rho1 per-call arrays stay on the device
   int T = 1;
   int n_batches = 0, n_full = 0, n_basis = 0, nmb = 0, n_valid_points = 0, n_atoms = 0;
   std::unique_ptr<Rho1Plan> rho1;
@@ -39,6 +49,24 @@ struct RefState {
 
 RefState &ref() {
   static RefState s;
+  return s;
+}
+
+// State of the GPU-backed CPU symbols, which need no init_dfpt_* call: their own stream, plans kept
+// while the metadata they were built from is unchanged (a 64-bit hash of it), and buffers for the
+// arrays passed on every call.
+struct CpuSymState {
+  hipStream_t stream = nullptr;
+  uint64_t rho_key = 0, far_key = 0;
+  std::unique_ptr<Rho1Plan> rho1;
+  std::unique_ptr<FarPlan> far;
+  DevBuf<double> wave, gwave, P1, bias, gbias, rho, grad;
+  DevBuf<double> f_cp, f_mrsq, f_opr, f_M, f_dv, f_spl;
+  DevBuf<int> f_map, f_lmfd;
+};
+
+CpuSymState &cpu_sym() {
+  static CpuSymState s;
   return s;
 }
 
@@ -71,11 +99,152 @@ bool rho1_gradient_from_env() {
   return !(g && strcmp(g, "0") == 0);
 }
 
+// The reference's compile-time ATOM_TILE_SIZE (setting.h:19-20, default 1).
+int tile_size_from_env() {
+  const char *ts = getenv("FRPA_ATOM_TILE_SIZE");
+  const int T = ts ? atoi(ts) : 1;
+  FRPA_REQUIRE(T >= 1, FRPA_ERR_INVALID, "FRPA_ATOM_TILE_SIZE must be >= 1");
+  return T;
+}
+
+// 64-bit hash of a byte range (multiply-xorshift over 8-byte words), to tell whether metadata
+// passed again equals what a cached plan was built from.
+uint64_t hash_bytes(uint64_t h, const void *p, size_t n) {
+  const unsigned char *c = static_cast<const unsigned char *>(p);
+  auto mix = [](uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+  };
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    memcpy(&w, c + i, 8);
+    h = mix(h ^ w) + 0x9e3779b97f4a7c15ULL;
+  }
+  uint64_t w = 0;
+  memcpy(&w, c + i, n - i);
+  return mix(h ^ w ^ (uint64_t)n);
+}
+
+// Dense 1-based (n_centers_basis_I, n_batches) basis lists and int32 wave offsets -> the plan's CSR
+// lists and 64-bit offsets.
+Rho1Plan *plan_from_reference_arrays(int nb, int n_basis, int n_full, int nmb, int nbI, const int *n_point_batches,
+                                     const int *prefix, const int *map, const int *ncc, const int *i_basis_batches,
+                                     const int *wave_offset) {
+  std::vector<long long> ibp(nb + 1, 0), woff(nb + 1);
+  std::vector<int> ib;
+  for (int b = 0; b < nb; b++) {
+    const int nc = ncc[b];
+    FRPA_REQUIRE(nc >= 0 && nc <= nbI, FRPA_ERR_INVALID, "n_compute_c(%d) = %d", b, nc);
+    ib.insert(ib.end(), i_basis_batches + (size_t)nbI * b, i_basis_batches + (size_t)nbI * b + nc);
+    ibp[b + 1] = ibp[b] + nc;
+  }
+  for (int b = 0; b <= nb; b++)
+    woff[b] = wave_offset[b];
+  return make_rho1_plan(nb, n_basis, n_full, nmb, n_point_batches, prefix, map, ncc, ibp.data(), ib.data(),
+                        woff.data());
+}
+
+// rho1 forward of a reference entry point: precision and gradient from the environment; without the
+// gradient first_order_gradient_rho is zeroed (evaluate_first_order_rho.cpp:113-114).
+void rho1_run(Rho1Plan &plan, int T, int n_tile, const double *wave, const double *gwave, const double *P1,
+              const double *bias, const double *gbias, double *rho, double *grad, hipStream_t s) {
+  const bool want_grad = rho1_gradient_from_env();
+  const int prec = want_grad ? FRPA_PREC_FP64 : rho1_precision_from_env();
+  const int rc = frpa_rho1_forward(reinterpret_cast<frpa_rho1_plan *>(&plan), T, n_tile, prec, wave, P1, bias, rho,
+                                   want_grad ? gwave : nullptr, want_grad ? gbias : nullptr,
+                                   want_grad ? grad : nullptr, s);
+  FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
+  if (!want_grad)
+    fill_f64(grad, (size_t)3 * plan.nmb * plan.n_batches * T, 0.0, s);
+}
+
+// Per-call far/near arrays of one Hartree-sum call in device buffers.
+struct SumUpBufs {
+  DevBuf<double> *cp, *mrsq, *opr, *M, *dv, *spl;
+  DevBuf<int> *map, *lmfd;
+};
+
+// Both branches of the Hartree sum (sum_up_whole_potential.cpp:195-529 / .cu:548-770): uploads the
+// per-call arrays, near-field spline sum then far-field multipole sum accumulated into
+// delta_v_hartree, copies it back and synchronises.
+void sumup_run(FarPlan &P, const SumUpBufs &B, int T, int n_tile, int n_full_points, int n_valid_points,
+               int l_pot_max, int n_hartree_grid, int n_atoms, int n_species, const int *species_center_ptr,
+               const int *l_hartree_max_far_distance_ptr, const int *i_valid_point_2_i_full_points_map_ptr,
+               const double *coord_points_ptr, const double *multipole_radius_sq_ptr,
+               const double *outer_potential_radius_ptr, const double *multipole_moments_ptr,
+               const double *current_delta_v_hart_part_spl_tile_ptr, const int *n_grid_ptr,
+               const double *r_grid_min_ptr, const double *log_r_grid_inc_ptr, double *delta_v_hartree_ptr,
+               hipStream_t s) {
+  FRPA_REQUIRE(n_tile >= 1 && n_tile <= T, FRPA_ERR_INVALID, "j_atom tile %d outside [1, %d]", n_tile, T);
+  const int lp1 = l_pot_max + 1;
+  B.cp->grow((size_t)3 * n_full_points);
+  B.map->grow(std::max(n_valid_points, 1));
+  B.lmfd->grow((size_t)n_atoms * T);
+  B.mrsq->grow((size_t)n_atoms * T);
+  B.opr->grow((size_t)lp1 * n_atoms * T);
+  B.M->grow((size_t)lp1 * lp1 * n_atoms * T);
+  B.dv->grow((size_t)n_full_points * T);
+  FRPA_HIP(hipMemcpyAsync(B.cp->ptr, coord_points_ptr, (size_t)3 * n_full_points * 8, hipMemcpyHostToDevice, s));
+  FRPA_HIP(hipMemcpyAsync(B.map->ptr, i_valid_point_2_i_full_points_map_ptr, (size_t)n_valid_points * 4,
+                          hipMemcpyHostToDevice, s));
+  FRPA_HIP(hipMemcpyAsync(B.lmfd->ptr, l_hartree_max_far_distance_ptr, (size_t)n_atoms * T * 4, hipMemcpyHostToDevice,
+                          s));
+  FRPA_HIP(hipMemcpyAsync(B.mrsq->ptr, multipole_radius_sq_ptr, (size_t)n_atoms * T * 8, hipMemcpyHostToDevice, s));
+  FRPA_HIP(hipMemcpyAsync(B.opr->ptr, outer_potential_radius_ptr, (size_t)lp1 * n_atoms * T * 8,
+                          hipMemcpyHostToDevice, s));
+  FRPA_HIP(hipMemcpyAsync(B.M->ptr, multipole_moments_ptr, (size_t)lp1 * lp1 * n_atoms * T * 8,
+                          hipMemcpyHostToDevice, s));
+  FRPA_HIP(hipMemcpyAsync(B.dv->ptr, delta_v_hartree_ptr, (size_t)n_full_points * T * 8, hipMemcpyHostToDevice, s));
+  // near field (sum_up_close_part_cu): this call's per-species log grids and spline tile
+  const size_t n_spl = (size_t)lp1 * lp1 * 2 * n_hartree_grid * n_atoms * T;
+  FRPA_REQUIRE(species_center_ptr && n_grid_ptr && r_grid_min_ptr && log_r_grid_inc_ptr &&
+                   current_delta_v_hart_part_spl_tile_ptr && n_hartree_grid >= 2 && n_species == P.n_species,
+               FRPA_ERR_INVALID, "near field: grid arrays, spline tile and n_hartree_grid >= 2 are required");
+  set_near_tables(P, n_species, species_center_ptr, n_grid_ptr, r_grid_min_ptr, log_r_grid_inc_ptr);
+  B.spl->grow(n_spl);
+  FRPA_HIP(hipMemcpyAsync(B.spl->ptr, current_delta_v_hart_part_spl_tile_ptr, n_spl * 8, hipMemcpyHostToDevice, s));
+  const char *v = getenv("FRPA_FAR_VARIANT");
+  const int variant = (v && strcmp(v, "terms") == 0 && P.terms_ok) ? FRPA_FAR_TERMS : FRPA_FAR_HORNER;
+  FarCall c;
+  c.T = T;
+  c.n_tile = n_tile;
+  c.n_full = n_full_points;
+  c.n_valid = n_valid_points;
+  c.coord_points = B.cp->ptr;
+  c.map = B.map->ptr;
+  c.lmfd = B.lmfd->ptr;
+  c.mrsq = B.mrsq->ptr;
+  c.opr = B.opr->ptr;
+  c.moments = B.M->ptr;
+  c.spl = B.spl->ptr;
+  c.n_hgrid = n_hartree_grid;
+  near_forward(P, c, B.dv->ptr, s);
+  far_forward(P, c, variant, B.dv->ptr, s);
+  FRPA_HIP(hipMemcpyAsync(delta_v_hartree_ptr, B.dv->ptr, (size_t)n_full_points * T * 8, hipMemcpyDeviceToHost, s));
+  FRPA_HIP(hipStreamSynchronize(s));
+}
+
+hipStream_t cpu_sym_stream() {
+  CpuSymState &C = cpu_sym();
+  if (!C.stream) {
+    int n = 0;
+    FRPA_HIP(hipGetDeviceCount(&n));
+    FRPA_REQUIRE(n > 0, FRPA_ERR_HIP, "no HIP device");
+    FRPA_HIP(hipStreamCreate(&C.stream));
+  }
+  return C.stream;
+}
+
 } // namespace
 
-extern "C" {
+namespace frpa_ref {
 
-void init_dfpt_device_info(void) {
+void init_device_info() {
   run_or_die("init_dfpt_device_info", [] {
     RefState &S = ref();
     int n = 0;
@@ -88,7 +257,7 @@ void init_dfpt_device_info(void) {
   });
 }
 
-void free_dfpt_device_info(void) {
+void free_device_info() {
   RefState &S = ref();
   if (S.stream)
     (void)hipStreamDestroy(S.stream);
@@ -96,7 +265,7 @@ void free_dfpt_device_info(void) {
   S.info = false;
 }
 
-void init_dfpt_device_data(
+void init_device_data(
     int *n_my_batches_work_, int *n_full_points_, int *n_valid_points_, int *n_basis_, int *n_atoms_,
     int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_, int *l_pot_max_, int *n_max_radial_,
     int *n_hartree_grid_, int *n_centers_, int *n_centers_hartree_potential_, int *hartree_force_l_add_,
@@ -116,9 +285,10 @@ void init_dfpt_device_data(
   run_or_die("init_dfpt_device_data", [&] {
     RefState &S = ref();
     FRPA_REQUIRE(S.info, FRPA_ERR_STATE, "init_dfpt_device_info must be called first");
-    const char *ts = getenv("FRPA_ATOM_TILE_SIZE");
-    S.T = ts ? atoi(ts) : 1;
-    FRPA_REQUIRE(S.T >= 1, FRPA_ERR_INVALID, "FRPA_ATOM_TILE_SIZE must be >= 1");
+    S.T = tile_size_from_env();
+    const char *res = getenv("FRPA_ALL_KERNELS_ON_GPU");
+    if (res && res[0])
+      S.resident = res[0] == '1';
     const int nb = *n_my_batches_work_, nbI = *n_centers_basis_I_;
     S.n_batches = nb;
     S.n_full = *n_full_points_;
@@ -126,21 +296,10 @@ void init_dfpt_device_data(
     S.nmb = *n_max_batch_size_;
     S.n_valid_points = *n_valid_points_;
     S.n_atoms = *n_atoms_;
-    // dense (n_centers_basis_I, n_batches) 1-based lists -> CSR; int32 wave offsets -> int64
-    std::vector<long long> ibp(nb + 1, 0), woff(nb + 1);
-    std::vector<int> ib;
-    for (int b = 0; b < nb; b++) {
-      const int nc = n_compute_c_batches_ptr[b];
-      FRPA_REQUIRE(nc >= 0 && nc <= nbI, FRPA_ERR_INVALID, "n_compute_c(%d) = %d", b, nc);
-      ib.insert(ib.end(), i_basis_batches_ptr + (size_t)nbI * b, i_basis_batches_ptr + (size_t)nbI * b + nc);
-      ibp[b + 1] = ibp[b] + nc;
-    }
-    for (int b = 0; b <= nb; b++)
-      woff[b] = i_batch_2_wave_offset_ptr[b];
-    S.rho1.reset(make_rho1_plan(nb, S.n_basis, S.n_full, S.nmb, n_point_batches_ptr,
-                                    n_point_batches_prefix_sum_ptr, i_valid_point_2_i_full_points_map_ptr,
-                                    n_compute_c_batches_ptr, ibp.data(), ib.data(), woff.data()));
-    const size_t wl = nb ? (size_t)woff[nb] : 0;
+    S.rho1.reset(plan_from_reference_arrays(nb, S.n_basis, S.n_full, S.nmb, nbI, n_point_batches_ptr,
+                                            n_point_batches_prefix_sum_ptr, i_valid_point_2_i_full_points_map_ptr,
+                                            n_compute_c_batches_ptr, i_basis_batches_ptr, i_batch_2_wave_offset_ptr));
+    const size_t wl = nb ? (size_t)i_batch_2_wave_offset_ptr[nb] : 0;
     S.wave.upload(wave_batches_compress_ptr, wl);
     S.gwave.upload(gradient_wave_batches_compress_ptr, 3 * wl);
     S.avn.upload(atom_valid_n_compute_c_batches_ptr, (size_t)(S.n_atoms + 1) * nb);
@@ -149,44 +308,127 @@ void init_dfpt_device_data(
     S.gbias.alloc((size_t)3 * S.nmb * nb * S.T);
     S.rho.alloc((size_t)S.n_full * S.T);
     S.grad.alloc((size_t)3 * S.nmb * nb * S.T);
+    // as the reference's init (device_data.cu:185-187), the per-call arrays are allocated, not copied
     S.coord_points.upload(coord_points_ptr, (size_t)3 * S.n_full);
     if (*n_centers_hartree_potential_ > 0 && *n_atoms_ > 0)
       S.far.reset(make_far_plan(*l_pot_max_, *l_max_analytic_multipole_, *index_cc_dim_0_, *n_centers_,
-                                    *n_centers_hartree_potential_, *n_atoms_, *n_species_, *hartree_force_l_add_,
-                                    species_ptr, l_hartree_ptr, centers_hartree_potential_ptr, center_to_atom_ptr,
-                                    coords_center_ptr, index_cc_ptr, cc_ptr, n_cc_lm_ijk_ptr,
-                                    index_ijk_max_cc_ptr));
+                                *n_centers_hartree_potential_, *n_atoms_, *n_species_, *hartree_force_l_add_,
+                                species_ptr, l_hartree_ptr, centers_hartree_potential_ptr, center_to_atom_ptr,
+                                coords_center_ptr, index_cc_ptr, cc_ptr, n_cc_lm_ijk_ptr, index_ijk_max_cc_ptr));
     S.data = true;
   });
 }
 
-void free_dfpt_device_data(void) {
+void free_device_data() {
   RefState &S = ref();
   S.rho1.reset();
   S.far.reset();
-  S.f_spl.release();
-  S.wave.release();
-  S.gwave.release();
-  S.P1.release();
-  S.bias.release();
-  S.gbias.release();
-  S.dm.release();
-  S.avn.release();
-  for (DevBuf<double> *b : {&S.h_hw, &S.h_vs, &S.h_v2rr, &S.h_v2rs, &S.h_v2ss, &S.h_grho, &S.h_pref1, &S.h_grho1,
-                            &S.h_vhg, &S.h_rho1, &S.h_w, &S.h_H})
+  for (DevBuf<double> *b : {&S.f_spl, &S.wave, &S.gwave, &S.P1, &S.bias, &S.gbias, &S.dm, &S.h_hw, &S.h_vs,
+                            &S.h_v2rr, &S.h_v2rs, &S.h_v2ss, &S.h_grho, &S.h_pref1, &S.h_grho1, &S.h_vhg, &S.h_rho1,
+                            &S.h_w, &S.h_H, &S.rho, &S.grad, &S.coord_points, &S.f_cp, &S.f_mrsq, &S.f_opr, &S.f_M,
+                            &S.f_dv})
     b->release();
-  S.rho.release();
-  S.grad.release();
-  S.coord_points.release();
-  S.f_cp.release();
-  S.f_mrsq.release();
-  S.f_opr.release();
-  S.f_M.release();
-  S.f_dv.release();
+  S.avn.release();
   S.f_map.release();
   S.f_lmfd.release();
   S.data = false;
 }
+
+void sum_up_cu_host(
+    int j_atom_begin, int j_atom_end, int n_full_points, int l_max_analytic_multipole, int index_cc_dim_0,
+    int n_valid_points, int l_pot_max, int n_max_radial, int n_hartree_grid, int n_centers,
+    int n_centers_hartree_potential, int n_atoms, int hartree_force_l_add, int n_species, const int *n_radial_ptr,
+    const int *species_ptr, const int *species_center_ptr, const int *l_hartree_max_far_distance_ptr,
+    const int *i_valid_point_2_i_full_points_map_ptr, const double *coord_points_ptr,
+    const double *multipole_radius_sq_ptr, const double *outer_potential_radius_ptr,
+    const double *multipole_moments_ptr, const double *current_delta_v_hart_part_spl_tile_ptr,
+    const int *l_hartree_ptr, const int *n_grid_ptr, const int *n_cc_lm_ijk_ptr,
+    const int *centers_hartree_potential_ptr, const int *center_to_atom_ptr, const int *index_cc_ptr,
+    const int *index_ijk_max_cc_ptr, const double *cc_ptr, const double *coords_center_ptr,
+    const double *r_grid_min_ptr, const double *log_r_grid_inc_ptr, const double *scale_radial_ptr,
+    const double *partition_tab_ptr, double *delta_v_hartree_ptr) {
+  run_or_die("sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_", [&] {
+    RefState &S = ref();
+    FRPA_REQUIRE(S.data && S.far, FRPA_ERR_STATE, "init_dfpt_device_data (with hartree centres) must be called first");
+    FarPlan &P = *S.far;
+    FRPA_REQUIRE(l_pot_max == P.l_pot_max && n_atoms == P.n_atoms && n_centers_hartree_potential == P.n_hc,
+                 FRPA_ERR_INVALID, "sizes differ from init_dfpt_device_data");
+    SumUpBufs B{&S.f_cp, &S.f_mrsq, &S.f_opr, &S.f_M, &S.f_dv, &S.f_spl, &S.f_map, &S.f_lmfd};
+    sumup_run(P, B, S.T, j_atom_end - j_atom_begin + 1, n_full_points, n_valid_points, l_pot_max, n_hartree_grid,
+              n_atoms, n_species, species_center_ptr, l_hartree_max_far_distance_ptr,
+              i_valid_point_2_i_full_points_map_ptr, coord_points_ptr, multipole_radius_sq_ptr,
+              outer_potential_radius_ptr, multipole_moments_ptr, current_delta_v_hart_part_spl_tile_ptr, n_grid_ptr,
+              r_grid_min_ptr, log_r_grid_inc_ptr, delta_v_hartree_ptr, S.stream);
+  });
+}
+
+} // namespace frpa_ref
+
+extern "C" {
+
+void init_dfpt_device_info(void) { frpa_ref::init_device_info(); }
+
+void free_dfpt_device_info(void) { frpa_ref::free_device_info(); }
+
+void init_dfpt_device_data(
+    int *n_my_batches_work_, int *n_full_points_, int *n_valid_points_, int *n_basis_, int *n_atoms_,
+    int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_, int *l_pot_max_, int *n_max_radial_,
+    int *n_hartree_grid_, int *n_centers_, int *n_centers_hartree_potential_, int *hartree_force_l_add_,
+    int *n_species_, int *l_max_analytic_multipole_, int *index_cc_dim_0_,
+    int *basis_atom_ptr, int *batch_sizes_ptr, int *n_point_batches_ptr, int *n_point_batches_prefix_sum_ptr,
+    int *i_valid_point_2_i_full_points_map_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *i_batch_2_wave_offset_ptr, double *wave_batches_compress_ptr,
+    double *gradient_wave_batches_compress_ptr, double *density_matrix_ptr,
+    int *n_radial_ptr, int *species_ptr, int *species_center_ptr, double *coord_points_ptr, int *l_hartree_ptr,
+    int *n_grid_ptr, int *n_cc_lm_ijk_ptr, int *centers_hartree_potential_ptr, int *center_to_atom_ptr,
+    int *index_cc_ptr, int *index_ijk_max_cc_ptr, double *cc_ptr, double *coords_center_ptr,
+    double *r_grid_min_ptr, double *log_r_grid_inc_ptr, double *scale_radial_ptr, double *partition_tab_ptr,
+    double *first_order_density_matrix_ptr, double *first_order_rho_ptr, double *first_order_rho_bias_part2_ptr,
+    double *first_order_gradient_rho_ptr, double *first_order_gradient_rho_bias_batches_atoms_ptr,
+    int *l_hartree_max_far_distance_ptr, double *multipole_radius_sq_ptr, double *outer_potential_radius_ptr,
+    double *multipole_moments_ptr, double *current_delta_v_hart_part_spl_tile_ptr, double *delta_v_hartree_ptr) {
+  frpa_ref::init_device_data(
+      n_my_batches_work_, n_full_points_, n_valid_points_, n_basis_, n_atoms_, n_max_compute_ham_,
+      n_centers_basis_I_, n_max_batch_size_, l_pot_max_, n_max_radial_, n_hartree_grid_, n_centers_,
+      n_centers_hartree_potential_, hartree_force_l_add_, n_species_, l_max_analytic_multipole_, index_cc_dim_0_,
+      basis_atom_ptr, batch_sizes_ptr, n_point_batches_ptr, n_point_batches_prefix_sum_ptr,
+      i_valid_point_2_i_full_points_map_ptr, n_compute_c_batches_ptr, i_basis_batches_ptr,
+      atom_valid_n_compute_c_batches_ptr, i_batch_2_wave_offset_ptr, wave_batches_compress_ptr,
+      gradient_wave_batches_compress_ptr, density_matrix_ptr, n_radial_ptr, species_ptr, species_center_ptr,
+      coord_points_ptr, l_hartree_ptr, n_grid_ptr, n_cc_lm_ijk_ptr, centers_hartree_potential_ptr,
+      center_to_atom_ptr, index_cc_ptr, index_ijk_max_cc_ptr, cc_ptr, coords_center_ptr, r_grid_min_ptr,
+      log_r_grid_inc_ptr, scale_radial_ptr, partition_tab_ptr, first_order_density_matrix_ptr, first_order_rho_ptr,
+      first_order_rho_bias_part2_ptr, first_order_gradient_rho_ptr, first_order_gradient_rho_bias_batches_atoms_ptr,
+      l_hartree_max_far_distance_ptr, multipole_radius_sq_ptr, outer_potential_radius_ptr, multipole_moments_ptr,
+      current_delta_v_hart_part_spl_tile_ptr, delta_v_hartree_ptr);
+}
+
+void free_dfpt_device_data(void) { frpa_ref::free_device_data(); }
+
+void frpa_ref_set_all_kernels_on_gpu(int on) { ref().resident = on != 0; }
+
+double *frpa_ref_device_array(const char *name) {
+  RefState &S = ref();
+  if (!S.data || !name)
+    return nullptr;
+  const struct {
+    const char *n;
+    DevBuf<double> *b;
+  } tab[] = {{"first_order_density_matrix", &S.P1},
+             {"first_order_rho_bias_part2", &S.bias},
+             {"first_order_gradient_rho_bias_batches_atoms", &S.gbias},
+             {"first_order_rho", &S.rho},
+             {"first_order_gradient_rho", &S.grad},
+             {"wave_batches_compress", &S.wave},
+             {"gradient_wave_batches_compress", &S.gwave},
+             {"coord_points", &S.coord_points}};
+  for (const auto &e : tab)
+    if (strcmp(name, e.n) == 0)
+      return e.b->ptr;
+  return nullptr;
+}
+
+void *frpa_ref_stream(void) { return ref().stream; }
 
 void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
     int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
@@ -209,20 +451,81 @@ void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
                  "j_atom_end - j_atom_begin + 1 = %d outside [1, ATOM_TILE_SIZE = %d]", n_tile, S.T);
     hipStream_t s = S.stream;
     const size_t nbt = (size_t)S.nmb * S.n_batches * S.T;
-    FRPA_HIP(hipMemcpyAsync(S.P1.ptr, first_order_density_matrix_ptr, S.P1.n * 8, hipMemcpyHostToDevice, s));
-    FRPA_HIP(hipMemcpyAsync(S.bias.ptr, first_order_rho_bias_part2_ptr, nbt * 8, hipMemcpyHostToDevice, s));
-    FRPA_HIP(hipMemcpyAsync(S.gbias.ptr, first_order_gradient_rho_bias_batches_atoms_ptr, 3 * nbt * 8,
+    if (!S.resident) {  // evaluate_first_order_rho.cu:259-263
+      FRPA_HIP(hipMemcpyAsync(S.P1.ptr, first_order_density_matrix_ptr, S.P1.n * 8, hipMemcpyHostToDevice, s));
+      FRPA_HIP(hipMemcpyAsync(S.bias.ptr, first_order_rho_bias_part2_ptr, nbt * 8, hipMemcpyHostToDevice, s));
+      FRPA_HIP(hipMemcpyAsync(S.gbias.ptr, first_order_gradient_rho_bias_batches_atoms_ptr, 3 * nbt * 8,
+                              hipMemcpyHostToDevice, s));
+    }
+    rho1_run(*S.rho1, S.T, n_tile, S.wave.ptr, S.gwave.ptr, S.P1.ptr, S.bias.ptr, S.gbias.ptr, S.rho.ptr, S.grad.ptr,
+             s);
+    if (!S.resident) {  // evaluate_first_order_rho.cu:363-367; resident: results stay, no synchronisation
+      FRPA_HIP(hipMemcpyAsync(first_order_rho_ptr, S.rho.ptr, S.rho.n * 8, hipMemcpyDeviceToHost, s));
+      FRPA_HIP(hipMemcpyAsync(first_order_gradient_rho_ptr, S.grad.ptr, 3 * nbt * 8, hipMemcpyDeviceToHost, s));
+      FRPA_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
+void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_(
+    int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
+    int *n_basis_, int *n_atoms_, int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_,
+    int *batch_sizes_ptr, int *n_point_batches_ptr, int *n_point_batches_prefix_sum_ptr,
+    int *i_valid_point_2_i_full_points_map_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *i_batch_2_wave_offset_ptr, double *wave_batches_compress_ptr,
+    double *gradient_wave_batches_compress_ptr, double *first_order_density_matrix_ptr, double *first_order_rho_ptr,
+    double *first_order_rho_bias_part2_ptr, double *first_order_gradient_rho_ptr,
+    double *first_order_gradient_rho_bias_batches_atoms_ptr, int *basis_atom_ptr, double *partition_tab_ptr,
+    double *density_matrix_ptr) {
+  run_or_die("evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_", [&] {
+    CpuSymState &C = cpu_sym();
+    hipStream_t s = cpu_sym_stream();
+    const int nb = *n_my_batches_work_, n_full = *n_full_points_, n_basis = *n_basis_, nmb = *n_max_batch_size_;
+    const int nbI = *n_centers_basis_I_;
+    const int T = tile_size_from_env();
+    const int n_tile = *j_atom_end_ - *j_atom_begin_ + 1;
+    FRPA_REQUIRE(nb >= 0 && n_full >= 0 && n_basis >= 0 && nmb >= 0 && nbI >= 0, FRPA_ERR_INVALID, "bad sizes");
+    FRPA_REQUIRE(n_tile >= 1 && n_tile <= T, FRPA_ERR_INVALID,
+                 "j_atom_end - j_atom_begin + 1 = %d outside [1, ATOM_TILE_SIZE = %d]", n_tile, T);
+    // the plan is rebuilt when the batch metadata differs from the previous call's
+    uint64_t key = hash_bytes(0, n_point_batches_ptr, (size_t)nb * 4);
+    const int sizes[5] = {nb, n_full, n_basis, nmb, nbI};
+    key = hash_bytes(key, sizes, sizeof(sizes));
+    key = hash_bytes(key, n_point_batches_prefix_sum_ptr, (size_t)(nb + 1) * 4);
+    key = hash_bytes(key, n_compute_c_batches_ptr, (size_t)nb * 4);
+    key = hash_bytes(key, i_batch_2_wave_offset_ptr, (size_t)(nb + 1) * 4);
+    if (nb)
+      key = hash_bytes(key, i_valid_point_2_i_full_points_map_ptr, (size_t)n_point_batches_prefix_sum_ptr[nb] * 4);
+    for (int b = 0; b < nb; b++)
+      key = hash_bytes(key, i_basis_batches_ptr + (size_t)nbI * b,
+                       (size_t)std::max(0, std::min(n_compute_c_batches_ptr[b], nbI)) * 4);
+    if (!C.rho1 || key != C.rho_key) {
+      C.rho1.reset(plan_from_reference_arrays(nb, n_basis, n_full, nmb, nbI, n_point_batches_ptr,
+                                              n_point_batches_prefix_sum_ptr, i_valid_point_2_i_full_points_map_ptr,
+                                              n_compute_c_batches_ptr, i_basis_batches_ptr,
+                                              i_batch_2_wave_offset_ptr));
+      C.rho_key = key;
+    }
+    // every array is read afresh, as the CPU function reads it (evaluate_first_order_rho.cpp:113-209)
+    const size_t wl = nb ? (size_t)i_batch_2_wave_offset_ptr[nb] : 0;
+    const size_t nbt = (size_t)nmb * nb * T;
+    C.wave.grow(wl);
+    C.gwave.grow(3 * wl);
+    C.P1.grow((size_t)n_basis * n_basis * T);
+    C.bias.grow(nbt);
+    C.gbias.grow(3 * nbt);
+    C.rho.grow((size_t)n_full * T);
+    C.grad.grow(3 * nbt);
+    FRPA_HIP(hipMemcpyAsync(C.wave.ptr, wave_batches_compress_ptr, wl * 8, hipMemcpyHostToDevice, s));
+    FRPA_HIP(hipMemcpyAsync(C.gwave.ptr, gradient_wave_batches_compress_ptr, 3 * wl * 8, hipMemcpyHostToDevice, s));
+    FRPA_HIP(hipMemcpyAsync(C.P1.ptr, first_order_density_matrix_ptr, (size_t)n_basis * n_basis * T * 8,
                             hipMemcpyHostToDevice, s));
-    const bool grad = rho1_gradient_from_env();
-    const int prec = grad ? FRPA_PREC_FP64 : rho1_precision_from_env();
-    const int rc = frpa_rho1_forward(reinterpret_cast<frpa_rho1_plan *>(S.rho1.get()), S.T, n_tile, prec,
-                                     S.wave.ptr, S.P1.ptr, S.bias.ptr, S.rho.ptr, grad ? S.gwave.ptr : nullptr,
-                                     grad ? S.gbias.ptr : nullptr, grad ? S.grad.ptr : nullptr, s);
-    FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
-    if (!grad)  // evaluate_first_order_rho.cpp:113-114 zeroes both outputs
-      fill_f64(S.grad.ptr, 3 * nbt, 0.0, s);
-    FRPA_HIP(hipMemcpyAsync(first_order_rho_ptr, S.rho.ptr, S.rho.n * 8, hipMemcpyDeviceToHost, s));
-    FRPA_HIP(hipMemcpyAsync(first_order_gradient_rho_ptr, S.grad.ptr, 3 * nbt * 8, hipMemcpyDeviceToHost, s));
+    FRPA_HIP(hipMemcpyAsync(C.bias.ptr, first_order_rho_bias_part2_ptr, nbt * 8, hipMemcpyHostToDevice, s));
+    FRPA_HIP(hipMemcpyAsync(C.gbias.ptr, first_order_gradient_rho_bias_batches_atoms_ptr, 3 * nbt * 8,
+                            hipMemcpyHostToDevice, s));
+    rho1_run(*C.rho1, T, n_tile, C.wave.ptr, C.gwave.ptr, C.P1.ptr, C.bias.ptr, C.gbias.ptr, C.rho.ptr, C.grad.ptr, s);
+    FRPA_HIP(hipMemcpyAsync(first_order_rho_ptr, C.rho.ptr, (size_t)n_full * T * 8, hipMemcpyDeviceToHost, s));
+    FRPA_HIP(hipMemcpyAsync(first_order_gradient_rho_ptr, C.grad.ptr, 3 * nbt * 8, hipMemcpyDeviceToHost, s));
     FRPA_HIP(hipStreamSynchronize(s));
   });
 }
@@ -286,7 +589,7 @@ void evaluate_first_order_h_reduce_memory_c_v2_(
     const size_t nb = S.n_batches, nmb = S.nmb, nf = S.n_full, T = S.T, nbas = S.n_basis;
     const size_t wl = S.wave.n;
     auto up = [&](DevBuf<double> &b, const double *h, size_t n) {
-      b.alloc(std::max<size_t>(n, 1));
+      b.grow(n);
       if (n)
         FRPA_HIP(hipMemcpyAsync(b.ptr, h, n * 8, hipMemcpyHostToDevice, s));
     };
@@ -326,60 +629,65 @@ void sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_(
     const int *index_ijk_max_cc_ptr, const double *cc_ptr, const double *coords_center_ptr,
     const double *r_grid_min_ptr, const double *log_r_grid_inc_ptr, const double *scale_radial_ptr,
     const double *partition_tab_ptr, double *delta_v_hartree_ptr) {
-  run_or_die("sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_", [&] {
-    RefState &S = ref();
-    FRPA_REQUIRE(S.data && S.far, FRPA_ERR_STATE, "init_dfpt_device_data (with hartree centres) must be called first");
-    FarPlan &P = *S.far;
-    FRPA_REQUIRE(l_pot_max == P.l_pot_max && n_atoms == P.n_atoms &&
-                     n_centers_hartree_potential == P.n_hc,
-                 FRPA_ERR_INVALID, "sizes differ from init_dfpt_device_data");
-    const int n_tile = j_atom_end - j_atom_begin + 1;
-    const int T = S.T;
-    FRPA_REQUIRE(n_tile >= 1 && n_tile <= T, FRPA_ERR_INVALID, "j_atom tile %d outside [1, %d]", n_tile, T);
-    hipStream_t s = S.stream;
-    const int lp1 = l_pot_max + 1;
-    S.f_cp.alloc((size_t)3 * n_full_points);
-    S.f_map.alloc(std::max(n_valid_points, 1));
-    S.f_lmfd.alloc((size_t)n_atoms * T);
-    S.f_mrsq.alloc((size_t)n_atoms * T);
-    S.f_opr.alloc((size_t)lp1 * n_atoms * T);
-    S.f_M.alloc((size_t)lp1 * lp1 * n_atoms * T);
-    S.f_dv.alloc((size_t)n_full_points * T);
-    FRPA_HIP(hipMemcpyAsync(S.f_cp.ptr, coord_points_ptr, S.f_cp.n * 8, hipMemcpyHostToDevice, s));
-    FRPA_HIP(hipMemcpyAsync(S.f_map.ptr, i_valid_point_2_i_full_points_map_ptr, (size_t)n_valid_points * 4,
-                            hipMemcpyHostToDevice, s));
-    FRPA_HIP(hipMemcpyAsync(S.f_lmfd.ptr, l_hartree_max_far_distance_ptr, S.f_lmfd.n * 4, hipMemcpyHostToDevice, s));
-    FRPA_HIP(hipMemcpyAsync(S.f_mrsq.ptr, multipole_radius_sq_ptr, S.f_mrsq.n * 8, hipMemcpyHostToDevice, s));
-    FRPA_HIP(hipMemcpyAsync(S.f_opr.ptr, outer_potential_radius_ptr, S.f_opr.n * 8, hipMemcpyHostToDevice, s));
-    FRPA_HIP(hipMemcpyAsync(S.f_M.ptr, multipole_moments_ptr, S.f_M.n * 8, hipMemcpyHostToDevice, s));
-    FRPA_HIP(hipMemcpyAsync(S.f_dv.ptr, delta_v_hartree_ptr, S.f_dv.n * 8, hipMemcpyHostToDevice, s));
-    // near field (sum_up_close_part_cu): this call's per-species log grids and spline tile
-    const size_t n_spl = (size_t)lp1 * lp1 * 2 * n_hartree_grid * n_atoms * T;
-    FRPA_REQUIRE(species_center_ptr && n_grid_ptr && r_grid_min_ptr && log_r_grid_inc_ptr &&
-                     current_delta_v_hart_part_spl_tile_ptr && n_hartree_grid >= 2 && n_species == P.n_species,
-                 FRPA_ERR_INVALID, "near field: grid arrays, spline tile and n_hartree_grid >= 2 are required");
-    set_near_tables(P, n_species, species_center_ptr, n_grid_ptr, r_grid_min_ptr, log_r_grid_inc_ptr);
-    S.f_spl.alloc(n_spl);
-    FRPA_HIP(hipMemcpyAsync(S.f_spl.ptr, current_delta_v_hart_part_spl_tile_ptr, n_spl * 8, hipMemcpyHostToDevice, s));
-    const char *v = getenv("FRPA_FAR_VARIANT");
-    const int variant = (v && strcmp(v, "terms") == 0 && P.terms_ok) ? FRPA_FAR_TERMS : FRPA_FAR_HORNER;
-    FarCall c;
-    c.T = T;
-    c.n_tile = n_tile;
-    c.n_full = n_full_points;
-    c.n_valid = n_valid_points;
-    c.coord_points = S.f_cp.ptr;
-    c.map = S.f_map.ptr;
-    c.lmfd = S.f_lmfd.ptr;
-    c.mrsq = S.f_mrsq.ptr;
-    c.opr = S.f_opr.ptr;
-    c.moments = S.f_M.ptr;
-    c.spl = S.f_spl.ptr;
-    c.n_hgrid = n_hartree_grid;
-    near_forward(P, c, S.f_dv.ptr, s);
-    far_forward(P, c, variant, S.f_dv.ptr, s);
-    FRPA_HIP(hipMemcpyAsync(delta_v_hartree_ptr, S.f_dv.ptr, S.f_dv.n * 8, hipMemcpyDeviceToHost, s));
-    FRPA_HIP(hipStreamSynchronize(s));
+  frpa_ref::sum_up_cu_host(
+      j_atom_begin, j_atom_end, n_full_points, l_max_analytic_multipole, index_cc_dim_0, n_valid_points, l_pot_max,
+      n_max_radial, n_hartree_grid, n_centers, n_centers_hartree_potential, n_atoms, hartree_force_l_add, n_species,
+      n_radial_ptr, species_ptr, species_center_ptr, l_hartree_max_far_distance_ptr,
+      i_valid_point_2_i_full_points_map_ptr, coord_points_ptr, multipole_radius_sq_ptr, outer_potential_radius_ptr,
+      multipole_moments_ptr, current_delta_v_hart_part_spl_tile_ptr, l_hartree_ptr, n_grid_ptr, n_cc_lm_ijk_ptr,
+      centers_hartree_potential_ptr, center_to_atom_ptr, index_cc_ptr, index_ijk_max_cc_ptr, cc_ptr,
+      coords_center_ptr, r_grid_min_ptr, log_r_grid_inc_ptr, scale_radial_ptr, partition_tab_ptr,
+      delta_v_hartree_ptr);
+}
+
+void sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_(
+    const int *j_atom_begin_, const int *j_atom_end_, const int *n_full_points_, const int *l_max_analytic_multipole_,
+    const int *index_cc_dim_0_, const int *n_valid_points_, const int *l_pot_max_, const int *n_max_radial_,
+    const int *n_hartree_grid_, const int *n_centers_, const int *n_centers_hartree_potential_, const int *n_atoms_,
+    const int *hartree_force_l_add_, const int *n_species_, const int *n_radial_ptr, const int *species_ptr,
+    const int *species_center_ptr, const int *l_hartree_max_far_distance_ptr,
+    const int *i_valid_point_2_i_full_points_map_ptr, const double *coord_points_ptr,
+    const double *multipole_radius_sq_ptr, const double *outer_potential_radius_ptr,
+    const double *multipole_moments_ptr, const double *current_delta_v_hart_part_spl_tile_ptr,
+    const int *l_hartree_ptr, const int *n_grid_ptr, const int *n_cc_lm_ijk_ptr,
+    const int *centers_hartree_potential_ptr, const int *center_to_atom_ptr, const int *index_cc_ptr,
+    const int *index_ijk_max_cc_ptr, const double *cc_ptr, const double *coords_center_ptr,
+    const double *r_grid_min_ptr, const double *log_r_grid_inc_ptr, const double *scale_radial_ptr,
+    const double *partition_tab_ptr, double *delta_v_hartree_ptr) {
+  run_or_die("sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_", [&] {
+    CpuSymState &C = cpu_sym();
+    hipStream_t s = cpu_sym_stream();
+    const int T = tile_size_from_env();
+    const int l_pot_max = *l_pot_max_, lama = *l_max_analytic_multipole_, dim0 = *index_cc_dim_0_;
+    const int n_centers = *n_centers_, n_hc = *n_centers_hartree_potential_, n_atoms = *n_atoms_;
+    const int n_species = *n_species_, hfla = *hartree_force_l_add_;
+    FRPA_REQUIRE(n_centers >= 0 && n_hc >= 0 && n_atoms >= 1 && n_species >= 1 && lama >= 0 && dim0 >= 1,
+                 FRPA_ERR_INVALID, "bad sizes");
+    // the plan (centre records, term tables) is rebuilt when the geometry or tables differ
+    const int sizes[7] = {l_pot_max, lama, dim0, n_centers, n_hc, n_atoms, n_species};
+    uint64_t key = hash_bytes(0, sizes, sizeof(sizes));
+    key = hash_bytes(key, &hfla, 4);
+    key = hash_bytes(key, species_ptr, (size_t)n_atoms * 4);
+    key = hash_bytes(key, l_hartree_ptr, (size_t)n_species * 4);
+    key = hash_bytes(key, centers_hartree_potential_ptr, (size_t)n_hc * 4);
+    key = hash_bytes(key, center_to_atom_ptr, (size_t)n_centers * 4);
+    key = hash_bytes(key, coords_center_ptr, (size_t)3 * n_centers * 8);
+    key = hash_bytes(key, index_cc_ptr, (size_t)dim0 * 6 * 4);
+    key = hash_bytes(key, cc_ptr, (size_t)dim0 * 8);
+    key = hash_bytes(key, n_cc_lm_ijk_ptr, (size_t)(lama + 1) * 4);
+    key = hash_bytes(key, index_ijk_max_cc_ptr, (size_t)3 * (lama + 1) * 4);
+    if (!C.far || key != C.far_key) {
+      C.far.reset(make_far_plan(l_pot_max, lama, dim0, n_centers, n_hc, n_atoms, n_species, hfla, species_ptr,
+                                l_hartree_ptr, centers_hartree_potential_ptr, center_to_atom_ptr, coords_center_ptr,
+                                index_cc_ptr, cc_ptr, n_cc_lm_ijk_ptr, index_ijk_max_cc_ptr));
+      C.far_key = key;
+    }
+    SumUpBufs B{&C.f_cp, &C.f_mrsq, &C.f_opr, &C.f_M, &C.f_dv, &C.f_spl, &C.f_map, &C.f_lmfd};
+    sumup_run(*C.far, B, T, *j_atom_end_ - *j_atom_begin_ + 1, *n_full_points_, *n_valid_points_, l_pot_max,
+              *n_hartree_grid_, n_atoms, n_species, species_center_ptr, l_hartree_max_far_distance_ptr,
+              i_valid_point_2_i_full_points_map_ptr, coord_points_ptr, multipole_radius_sq_ptr,
+              outer_potential_radius_ptr, multipole_moments_ptr, current_delta_v_hart_part_spl_tile_ptr, n_grid_ptr,
+              r_grid_min_ptr, log_r_grid_inc_ptr, delta_v_hartree_ptr, s);
   });
 }
 

@@ -312,14 +312,22 @@ int frpa_synth_wave(frpa_rho1_plan *plan, int n_atoms, const double *atom_xyz, c
 /* Reference entry points (drop-in names; host arrays; exit(1) on error)                       */
 /* ------------------------------------------------------------------------------------------ */
 
-/* fast_rap_src_dfpt_gpu_eval/device.hpp:191-192 (C++ linkage there; C linkage here). */
+/* The reference declares init_dfpt_device_info / free_dfpt_device_info / init_dfpt_device_data /
+ * free_dfpt_device_data and sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_ with
+ * C++ linkage.  The library exports each of them twice: with C linkage (declared here) and with C++
+ * linkage and the reference's parameter lists (the mangled names a host compiled against the
+ * reference's device.hpp, device_data.hpp and sum_up_whole_potential.h links to); both forms run
+ * the same code. */
+
+/* fast_rap_src_dfpt_gpu_eval/device.hpp:191-192. */
 void init_dfpt_device_info(void);
 void free_dfpt_device_info(void);
 
-/* fast_rap_src_dfpt_gpu_eval/device_data.hpp:159-227 (C++ linkage there; C linkage here).
- * Uploads the constant arrays once and builds the rho1 and far-field plans.  The reference's
- * compile-time ATOM_TILE_SIZE is taken from the environment variable FRPA_ATOM_TILE_SIZE
- * (default 1), since this signature has no argument for it. */
+/* fast_rap_src_dfpt_gpu_eval/device_data.hpp:159-227.
+ * Uploads the constant arrays once and builds the rho1 and far-field plans; allocates (does not
+ * copy) the per-call rho1 arrays, as the reference.  The reference's compile-time ATOM_TILE_SIZE is
+ * taken from the environment variable FRPA_ATOM_TILE_SIZE (default 1), since this signature has no
+ * argument for it; FRPA_ALL_KERNELS_ON_GPU=1 selects the device-resident mode below. */
 void init_dfpt_device_data(
     int *n_my_batches_work_, int *n_full_points_, int *n_valid_points_, int *n_basis_, int *n_atoms_,
     int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_, int *l_pot_max_, int *n_max_radial_,
@@ -339,10 +347,28 @@ void init_dfpt_device_data(
     double *multipole_moments_ptr, double *current_delta_v_hart_part_spl_tile_ptr, double *delta_v_hartree_ptr);
 void free_dfpt_device_data(void);
 
+/* Device-resident mode, the run-time counterpart of the reference's compile-time ALL_KERNELS_ON_GPU
+ * (evaluate_first_order_rho.cu:17,259-263,363-367; its runner fills and reads the global devPs
+ * arrays, direct_test/evaluate_first_order_rho_run.cpp:163-166,200-202): on, the rho1 _cu_host_
+ * function reads first_order_density_matrix and the two bias arrays from device arrays and leaves
+ * first_order_rho and first_order_gradient_rho in device arrays, without copies and without
+ * synchronising; its host pointer arguments are not used.  frpa_ref_device_array returns the device
+ * address of the array of that name (first_order_density_matrix, first_order_rho_bias_part2,
+ * first_order_gradient_rho_bias_batches_atoms, first_order_rho, first_order_gradient_rho,
+ * wave_batches_compress, gradient_wave_batches_compress, coord_points; the layouts and sizes of the
+ * reference's arguments with ATOM_TILE_SIZE = FRPA_ATOM_TILE_SIZE), NULL before
+ * init_dfpt_device_data or for another name; frpa_ref_stream the stream the entry points run on
+ * (the reference's devInfo.stream).  Off by default (the environment variable FRPA_ALL_KERNELS_ON_GPU=1
+ * at init_dfpt_device_data, or this setter, turns it on). */
+void frpa_ref_set_all_kernels_on_gpu(int on);
+double *frpa_ref_device_array(const char *name);
+void *frpa_ref_stream(void);
+
 /* fast_rap_src_dfpt_gpu_eval/evaluate_first_order_rho.h:37-69.  Runs on the device-resident
  * batch data of init_dfpt_device_data; uploads first_order_density_matrix and the bias arrays from
  * the given host pointers and downloads first_order_rho and first_order_gradient_rho into them
- * (the reference's non-ALL_KERNELS_ON_GPU mode, evaluate_first_order_rho.cu:240-244,362-366).
+ * (the reference's non-ALL_KERNELS_ON_GPU mode, evaluate_first_order_rho.cu:240-244,362-366), or
+ * works on the device arrays only in the device-resident mode above.
  * first_order_gradient_rho is computed (GGA, FP64) unless FRPA_RHO1_GRADIENT=0 (then left zero);
  * without it FRPA_RHO1_PRECISION=ozaki / ozaki4 selects the INT8 split for first_order_rho. */
 void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
@@ -355,6 +381,44 @@ void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
     double *first_order_rho_bias_part2_ptr, double *first_order_gradient_rho_ptr,
     double *first_order_gradient_rho_bias_batches_atoms_ptr, int *basis_atom_ptr, double *partition_tab_ptr,
     double *density_matrix_ptr);
+
+/* fast_rap_src_dfpt_gpu_eval/evaluate_first_order_rho.h:3-35: the reference's CPU function (same
+ * name, C linkage, same 28 arguments, host arrays), run on the GPU.  Needs no init_dfpt_* call: it
+ * keeps a plan of its own, rebuilt when the batch metadata differs from the previous call's, and
+ * uploads the wave arrays, first_order_density_matrix and the bias arrays on every call (the CPU
+ * function reads them afresh); first_order_rho is overwritten (zeroed, then the tile's
+ * perturbations written) and first_order_gradient_rho computed, as evaluate_first_order_rho.cpp:
+ * 113-209, with the precision switches of the _cu_host_ function. */
+void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_(
+    int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
+    int *n_basis_, int *n_atoms_, int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_,
+    int *batch_sizes_ptr, int *n_point_batches_ptr, int *n_point_batches_prefix_sum_ptr,
+    int *i_valid_point_2_i_full_points_map_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *i_batch_2_wave_offset_ptr, double *wave_batches_compress_ptr,
+    double *gradient_wave_batches_compress_ptr, double *first_order_density_matrix_ptr, double *first_order_rho_ptr,
+    double *first_order_rho_bias_part2_ptr, double *first_order_gradient_rho_ptr,
+    double *first_order_gradient_rho_bias_batches_atoms_ptr, int *basis_atom_ptr, double *partition_tab_ptr,
+    double *density_matrix_ptr);
+
+/* fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.h:3-48: the reference's CPU function (same name,
+ * C linkage, scalars by pointer), run on the GPU: both branches (near-field splines, far-field
+ * multipoles) accumulated into delta_v_hartree (n_full_points, ATOM_TILE_SIZE), as
+ * sum_up_whole_potential.cpp:195-529.  Needs no init_dfpt_* call: it keeps a far-field plan of its
+ * own, rebuilt when the geometry or term tables differ from the previous call's. */
+void sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_(
+    const int *j_atom_begin_, const int *j_atom_end_, const int *n_full_points_, const int *l_max_analytic_multipole_,
+    const int *index_cc_dim_0_, const int *n_valid_points_, const int *l_pot_max_, const int *n_max_radial_,
+    const int *n_hartree_grid_, const int *n_centers_, const int *n_centers_hartree_potential_, const int *n_atoms_,
+    const int *hartree_force_l_add_, const int *n_species_, const int *n_radial_ptr, const int *species_ptr,
+    const int *species_center_ptr, const int *l_hartree_max_far_distance_ptr,
+    const int *i_valid_point_2_i_full_points_map_ptr, const double *coord_points_ptr,
+    const double *multipole_radius_sq_ptr, const double *outer_potential_radius_ptr,
+    const double *multipole_moments_ptr, const double *current_delta_v_hart_part_spl_tile_ptr,
+    const int *l_hartree_ptr, const int *n_grid_ptr, const int *n_cc_lm_ijk_ptr,
+    const int *centers_hartree_potential_ptr, const int *center_to_atom_ptr, const int *index_cc_ptr,
+    const int *index_ijk_max_cc_ptr, const double *cc_ptr, const double *coords_center_ptr,
+    const double *r_grid_min_ptr, const double *log_r_grid_inc_ptr, const double *scale_radial_ptr,
+    const double *partition_tab_ptr, double *delta_v_hartree_ptr);
 
 /* fast_rap_src_dfpt_gpu_eval/generate_first_order_rho_bias_part2.cpp:9-30 (a CPU function there;
  * same name, C linkage, 21 pointer arguments).  Runs on the GPU with the batch data that
@@ -388,7 +452,7 @@ void evaluate_first_order_h_reduce_memory_c_v2_(
     double *v2rho2_batches_ptr, double *v2rhosigma_batches_ptr, double *v2sigma2_batches_ptr,
     double *partition_tab_ptr, int *not_use);
 
-/* fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.h:50-95 (C++ linkage there; C linkage here).
+/* fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.h:50-95.
  * Both branches, as the reference (sum_up_whole_potential.cu:641-769): uploads the per-call arrays
  * (including current_delta_v_hart_part_spl_tile), runs the near-field spline sum and the far-field
  * multipole sum, accumulates into delta_v_hartree and synchronises.  species_center, n_grid,

@@ -44,3 +44,19 @@ def test_error_reporting_without_gpu_work():
     assert rc == 1 and b"plan is null" in lib.frpa_last_error()
     rc = lib.frpa_far_forward(None, 1, 1, 1, 0, 0, None, None, None, None, None, None, None, None)
     assert rc == 1
+
+
+def test_cxx_host_links_against_reference_declarations():
+    """tests/cxx/reference_host.cpp declares the entry points with the reference's linkage (C++ for
+    init_dfpt_device_info/data, free_*, the Hartree _cu_host_; C for the rho1 functions and the CPU
+    Hartree sum); linked with -z now, every one of them resolves from libfrpa.so at load time (no
+    GPU call is made)."""
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    subprocess.run(["make", "-s", "-C", os.path.join(here, "cxx")], check=True)
+    r = subprocess.run([os.path.join(here, "cxx", "reference_host"), "--link-only"], capture_output=True, text=True)
+    assert r.returncode == 0 and "linked" in r.stdout, (r.returncode, r.stdout, r.stderr)
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    for mangled in ("_Z21init_dfpt_device_infov", "_Z21free_dfpt_device_infov", "_Z21free_dfpt_device_datav",
+                    "_Z21init_dfpt_device_dataPi", "_Z66sum_up_whole_potential_c_v3_atoms_full_points_j_atom_tile_cu_host_ii"):
+        assert mangled in out, mangled

@@ -81,17 +81,17 @@ void run_or_die(const char *where, F &&f) {
     die(where);
 }
 
-// FRPA_RHO1_PRECISION: fp64 (default), ozaki (INT8 split, 5 digits), ozaki4 (4 digits).
-// FRPA_RHO1_GRADIENT=0: first_order_gradient_rho is not computed (left zero; LDA callers).  The
-// GGA gradient needs W in FP64, so with the gradient requested the FP64 kernel runs whatever the
-// precision.
+// FRPA_RHO1_PRECISION: ozaki (INT8 split with 5 digits, ~1e-12 of max|rho1| and of max|grad rho1|;
+// the default), ozaki4 (4 digits, ~1e-9), fp64 (FP64 MFMA).  FRPA_RHO1_GRADIENT=0: first_order_
+// gradient_rho is not computed (left zero; LDA callers), which lets the INT8 split multiply only the
+// symmetric half of P1.
 int rho1_precision_from_env() {
   const char *p = getenv("FRPA_RHO1_PRECISION");
-  if (p && strcmp(p, "ozaki") == 0)
-    return FRPA_PREC_OZAKI;
+  if (p && strcmp(p, "fp64") == 0)
+    return FRPA_PREC_FP64;
   if (p && strcmp(p, "ozaki4") == 0)
     return FRPA_PREC_OZAKI4;
-  return FRPA_PREC_FP64;
+  return FRPA_PREC_OZAKI;
 }
 
 bool rho1_gradient_from_env() {
@@ -154,7 +154,7 @@ Rho1Plan *plan_from_reference_arrays(int nb, int n_basis, int n_full, int nmb, i
 void rho1_run(Rho1Plan &plan, int T, int n_tile, const double *wave, const double *gwave, const double *P1,
               const double *bias, const double *gbias, double *rho, double *grad, hipStream_t s) {
   const bool want_grad = rho1_gradient_from_env();
-  const int prec = want_grad ? FRPA_PREC_FP64 : rho1_precision_from_env();
+  const int prec = rho1_precision_from_env();
   const int rc = frpa_rho1_forward(reinterpret_cast<frpa_rho1_plan *>(&plan), T, n_tile, prec, wave, P1, bias, rho,
                                    want_grad ? gwave : nullptr, want_grad ? gbias : nullptr,
                                    want_grad ? grad : nullptr, s);

@@ -17,6 +17,11 @@
 // and only the 64 x 64 tile pairs J >= I of P' are multiplied (half the MFMAs and half the P1
 // gathers of the full product).  The ddot, the bias and the scatter stay FP64.
 //
+// GGA gradient (GRAD): grad rho1(j, p) = 2 sum_i dphi(i, j, p) W(i, p) with W = P1loc phi for P1 as
+// given (evaluate_first_order_rho.cpp:161-200), so that launch multiplies every tile pair (I, J) of
+// the digits of P1 itself -- twice the MFMAs and gathers of the symmetric form -- and takes
+// rho1 = sum_i phi_i W_i and the gradient from the same W; dphi is read in FP64 in the epilogue.
+//
 // Data: P1 is relabeled by first appearance over the batches (get_relabel), so a batch's compute set
 // is a few long runs of consecutive labels and the per-tile gathers touch few cache lines.  Per call
 // the P' digits are written as one int64 per element (its S digits) in the new labels; the phi
@@ -146,10 +151,11 @@ __global__ void p1_rowscale_kernel(const unsigned long long *__restrict__ rowbit
   ainv[i] = mx > 0.0 ? mx / 127.0 : 1.0;
 }
 
-// Digits of P'(pinv[x], pinv[y]) / ainv[x] at x + n y, digit s in byte s of one 8-byte word (one
-// gather load per element); one 64 x 64 tile of the new labels per block; the transposed tile
-// P1(pinv[y], pinv[x]) goes through LDS so both reads run down columns.
-template <int S>
+// Digits of P'(pinv[x], pinv[y]) / ainv[x] (FULL: of P1(pinv[x], pinv[y]) / ainv[x]) at x + n y,
+// digit s in byte s of one 8-byte word (one gather load per element); one 64 x 64 tile of the new
+// labels per block; the transposed tile P1(pinv[y], pinv[x]) goes through LDS so both reads run down
+// columns.  |P1(r, c)| and |P'(r, c)| are both at most bound_r, the row scale.
+template <int S, bool FULL>
 __global__ __launch_bounds__(256) void p1_digits_kernel(const double *__restrict__ P1, const int *__restrict__ pinv,
                                                         const double *__restrict__ ainv, int n_basis,
                                                         unsigned long long *__restrict__ adig) {
@@ -159,7 +165,7 @@ __global__ __launch_bounds__(256) void p1_digits_kernel(const double *__restrict
   const double *Pt = P1 + nn * t;
   const int a = threadIdx.x & 63, b0 = threadIdx.x >> 6;
   // transposed read: thread (j = a, i = b0 + 4u) loads P1(pinv[y0 + j], pinv[x0 + i])
-  {
+  if (!FULL) {
     const int y = y0 + a;
     const int cy = y < n_basis ? pinv[y] : 0;
 #pragma unroll 4
@@ -179,9 +185,14 @@ __global__ __launch_bounds__(256) void p1_digits_kernel(const double *__restrict
     const int j = b0 + 4 * u, y = y0 + j;
     if (y >= n_basis)
       break;
-    double v = 0.5 * (Pt[rx + (size_t)n_basis * pinv[y]] + tr[a][j]);
-    if (x == y)
-      v *= 0.5;
+    double v;
+    if (FULL) {
+      v = Pt[rx + (size_t)n_basis * pinv[y]];
+    } else {
+      v = 0.5 * (Pt[rx + (size_t)n_basis * pinv[y]] + tr[a][j]);
+      if (x == y)
+        v *= 0.5;
+    }
     double z = v * inv;
     unsigned long long packed = 0;
 #pragma unroll
@@ -226,14 +237,23 @@ __device__ __forceinline__ void repack_a(const unsigned long long (&e)[16], v4i 
 // LDS: three staged phi-digit chunks -- buffer 2 holds the diagonal chunk J = I for the whole row
 // tile (its digits also give phi(i, p) for the ddot, so the FP64 wave is not read), buffers 0 and 1
 // alternate for J > I.
-template <int S, int NPG, int NG, int H>
+// GRAD: the gradient launch (every column tile J, starting at J = I so that buffer 2 still holds
+// the diagonal chunk; P1's own digits; dphi from g.gwave in the epilogue).
+struct GradArgs {
+  const double *gwave;  // gradient_wave_batches_compress: batch block at 3 wave_off, (nc_pad, 3, np)
+  const double *gbias;  // first_order_gradient_rho_bias_batches_atoms (3, nmb, n_batches, T), nullable
+  double *grad;         // first_order_gradient_rho (3, nmb, n_batches, T)
+  const int *srow_all;  // caller's local row of every sorted row (Relabel::d_srow)
+};
+
+template <int S, int NPG, int NG, int H, bool GRAD>
 __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *red, const BatchDesc &d, int t,
                                             int pc, int npc, int gp0, const int *__restrict__ sib,
                                             const int *__restrict__ map, const unsigned long long *__restrict__ At,
                                             const double *__restrict__ ai,
                                             const signed char *__restrict__ Bg, const double *__restrict__ binv,
                                             int n_basis, const double *__restrict__ bias, double *__restrict__ rho,
-                                            int n_full, int nmb, int n_batches_all, int n_tile) {
+                                            int n_full, int nmb, int n_batches_all, int n_tile, const GradArgs &ga) {
   constexpr int THREADS = 256 * H;
   constexpr int PCH = 16 * H * NPG;             // point slots per chunk
   constexpr int BUF = S * PCH * BROW;           // bytes per LDS buffer
@@ -245,9 +265,17 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
   const int lr = lane & 15, lk = lane >> 4;
   const int nT = ncK / 64;
   double racc[NG > 0 ? NG : 1];
+  double gacc[GRAD ? 3 : 1][NG > 0 ? NG : 1];
 #pragma unroll
-  for (int g = 0; g < NG; g++)
+  for (int g = 0; g < NG; g++) {
     racc[g] = 0.0;
+#pragma unroll
+    for (int j = 0; j < (GRAD ? 3 : 1); j++)
+      gacc[j][g] = 0.0;
+  }
+  const int ncp = (nc + 3) & ~3;
+  const double *GW = GRAD ? ga.gwave + 3 * d.wave_off : nullptr;
+  const int *srow = GRAD ? ga.srow_all + d.ib_off : nullptr;
 
   v4i bq[QPT];
   auto load_b = [&](int c) {
@@ -288,12 +316,13 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
         acc[l][g] = v4i{0, 0, 0, 0};
 
     unsigned long long e[16];
-    // A fragments: row irow, k = 64 c + 16 lk + q, q < 16; P'(i, k) = 0 below the diagonal (k < i)
+    // A fragments: row irow, k = 64 c + 16 lk + q, q < 16; P'(i, k) = 0 below the diagonal (k < i);
+    // GRAD: the full P1
     auto load_a = [&](int c, int buf) {
 #pragma unroll
       for (int q = 0; q < 16; q++) {
         const int jb = jbs[64 * buf + 16 * lk + q];
-        const bool ok = iok && jb >= 0 && (c > I || 64 * c + 16 * lk + q >= irow);
+        const bool ok = iok && jb >= 0 && (GRAD || c > I || 64 * c + 16 * lk + q >= irow);
         e[q] = ok ? At[arow + (size_t)n_basis * jb] : 0ull;
       }
     };
@@ -304,11 +333,14 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
     v4i a[S];
     load_a(I, 2);
     repack_a<S>(e, a);
-    for (int c = I; c < nT; c++) {
-      const int cur = c == I ? 2 : (c - I - 1) & 1, nxt = (c - I) & 1;
-      const bool more = c + 1 < nT;
+    // step st visits column tile c = I + st (GRAD: (I + st) mod nT, every tile)
+    const int nsteps = GRAD ? nT : nT - I;
+    for (int st = 0; st < nsteps; st++) {
+      const int cn = GRAD ? (I + st + 1) % nT : I + st + 1;  // the next step's tile
+      const int cur = st == 0 ? 2 : (st - 1) & 1, nxt = st & 1;
+      const bool more = st + 1 < nsteps;
       if (more)
-        load_b(c + 1);
+        load_b(cn);
       const signed char *B = Bs + cur * BUF;
 #pragma unroll
       for (int g = 0; g < NG; g++) {
@@ -323,10 +355,10 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
             acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
       }
       if (more)
-        store_b(c + 1, nxt);
+        store_b(cn, nxt);
       __syncthreads();
       if (more) {
-        load_a(c + 1, nxt);
+        load_a(cn, nxt);
         repack_a<S>(e, a);
       }
     }
@@ -336,16 +368,19 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
     // rebuilt from the diagonal chunk's digits (rows 16 rg + 4 lk .. + 3 are one dword per plane):
     //   rho(p) += W(i, p) phi(i, p) = (sum_l 254^-l C_l / sa_i) (sum_s b_s 254^-s) / sb_p^2
     double rinv[4];
+    int grow[4];  // GRAD: caller's local row of sorted row i (-1 past nc)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int i = 64 * I + 16 * rg + 4 * lk + r;
       rinv[r] = i < nc ? ai[sib[i]] : 0.0;
+      if (GRAD)
+        grow[r] = i < nc ? srow[i] : -1;
     }
     const signed char *BD = Bs + 2 * BUF;
 #pragma unroll
     for (int g = 0; g < NG; g++) {
       const int p = 16 * (gp0 + g) + lr;
-      double part = 0.0;
+      double part = 0.0, gp[3] = {0.0, 0.0, 0.0};
       if (p < npc) {
         unsigned dw[S];
 #pragma unroll
@@ -362,55 +397,93 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
           for (int sl = S - 2; sl >= 0; sl--)
             ph = ph * (1.0 / RADIX) + (double)(signed char)(dw[sl] >> (8 * r));
           part += wv * rinv[r] * ph;
+          if (GRAD && grow[r] >= 0) {
+            // W(i, p) sb_p = wv / sa_i; dphi(i, j, p) at grow + ncp (j + 3 p)
+            const double *gq = GW + grow[r] + (size_t)ncp * 3 * (pc + p);
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+              gp[j] += wv * rinv[r] * gq[(size_t)ncp * j];
+          }
         }
         const double bi = binv[d.pstart + pc + p];
         part *= bi * bi;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          gp[j] *= bi;
       }
       part += __shfl_xor(part, 16);
       part += __shfl_xor(part, 32);
       racc[g] += part;
+      if (GRAD) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          gp[j] += __shfl_xor(gp[j], 16);
+          gp[j] += __shfl_xor(gp[j], 32);
+          gacc[j][g] += gp[j];
+        }
+      }
     }
   }
 
-  double(*rd)[PCH] = reinterpret_cast<double(*)[PCH]>(red);
+  // per-point sums over the four row groups; GRAD puts them (4 values per point and row group) in
+  // the staging buffers 0 and 1, which no wave reads after the last tile's K loop
+  double(*rd)[PCH] = reinterpret_cast<double(*)[PCH]>(GRAD ? reinterpret_cast<double *>(Bs) : red);
   __syncthreads();
   if (lk == 0) {
 #pragma unroll
-    for (int g = 0; g < NG; g++)
+    for (int g = 0; g < NG; g++) {
       rd[rg][16 * (gp0 + g) + lr] = racc[g];
+      if (GRAD) {
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          rd[4 * (j + 1) + rg][16 * (gp0 + g) + lr] = gacc[j][g];
+      }
+    }
   }
   __syncthreads();
   for (int p = tid; p < npc; p += THREADS) {
     const size_t bi = (size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
-    double sum = 2.0 * (rd[0][p] + rd[1][p] + rd[2][p] + rd[3][p]);
+    // symmetric form: 2 sum_i phi_i sum_{k >= i} P'(i, k) phi_k; GRAD: sum_i phi_i W_i
+    double sum = (GRAD ? 1.0 : 2.0) * (rd[0][p] + rd[1][p] + rd[2][p] + rd[3][p]);
     if (bias)
       sum += bias[bi];
     if (t < n_tile)
       rho[(size_t)map[d.pstart + pc + p] + (size_t)n_full * t] = sum;
+    if (GRAD) {
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const int q = 4 * (j + 1);
+        double gv = 2.0 * (rd[q][p] + rd[q + 1][p] + rd[q + 2][p] + rd[q + 3][p]);
+        if (ga.gbias)
+          gv -= ga.gbias[j + 3 * bi];
+        ga.grad[j + 3 * bi] = gv;
+      }
+    }
   }
 }
 
-template <int S, int NPG, int H, int... Gs>
+template <int S, int NPG, int H, bool GRAD, int... Gs>
 __device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>, int ng, signed char *Bs, int *jbs,
                                                double *red, const BatchDesc &d, int t, int pc, int npc, int gp0,
                                                const int *sib, const int *map, const unsigned long long *At,
                                                const double *ai, const signed char *Bg,
                                                const double *binv, int n_basis, const double *bias, double *rho,
-                                               int n_full, int nmb, int n_batches_all, int n_tile) {
-  ((ng == Gs ? ozaki_chunk<S, NPG, Gs, H>(Bs, jbs, red, d, t, pc, npc, gp0, sib, map, At, ai, Bg, binv, n_basis,
-                                       bias, rho, n_full, nmb, n_batches_all, n_tile)
+                                               int n_full, int nmb, int n_batches_all, int n_tile,
+                                               const GradArgs &ga) {
+  ((ng == Gs ? ozaki_chunk<S, NPG, Gs, H, GRAD>(Bs, jbs, red, d, t, pc, npc, gp0, sib, map, At, ai, Bg, binv, n_basis,
+                                             bias, rho, n_full, nmb, n_batches_all, n_tile, ga)
              : (void)0),
    ...);
 }
 
-template <int S, int NPG, int H>
+template <int S, int NPG, int H, bool GRAD>
 __global__ __launch_bounds__(256 * H, 2) void rho1_ozaki_kernel(
     const BatchDesc *__restrict__ descs, const int *__restrict__ order, int *__restrict__ counters, int n_pos, int T,
     const int *__restrict__ sib_all, const int *__restrict__ map, const unsigned long long *__restrict__ adig,
     const double *__restrict__ ainv, const signed char *__restrict__ bdig,
     const long long *__restrict__ boff, const double *__restrict__ binv, int n_basis,
     const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all, int n_tile,
-    int items) {
+    int items, GradArgs ga) {
   constexpr int PCH = 16 * H * NPG;
   __shared__ __attribute__((aligned(16))) signed char Bs[3 * S * PCH * BROW];
   __shared__ int jbs[192];
@@ -438,8 +511,9 @@ __global__ __launch_bounds__(256 * H, 2) void rho1_ozaki_kernel(
       const int ngt = (npc + 15) >> 4;       // groups of 16 points in this chunk
       const int n0 = H == 2 ? (ngt + 1) >> 1 : ngt;  // the first half takes the extra group
       const int ng = half ? ngt - n0 : n0, gp0 = half ? n0 : 0;
-      ozaki_dispatch<S, NPG, H>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc, gp0, sib,
-                             map, adig + nn, ai, Bg, binv, n_basis, bias, rho, n_full, nmb, n_batches_all, n_tile);
+      ozaki_dispatch<S, NPG, H, GRAD>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc,
+                                      gp0, sib, map, adig + nn, ai, Bg, binv, n_basis, bias, rho, n_full, nmb,
+                                      n_batches_all, n_tile, ga);
     }
   }
 }
@@ -467,13 +541,13 @@ void prepare(Rho1Plan &P, OzakiState &st, const double *wave, hipStream_t s) {
   st.bytes = off[P.n_valid] + (long long)nvp * 8;
 }
 
-template <int S, int NPG, int H>
+template <int S, int NPG, int H, bool GRAD>
 void launch(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s) {
   OzakiState &st = *P.ozaki;
   const Relabel &R = get_relabel(P);
   static int per_cu = -1;
   if (per_cu < 0) {
-    FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_ozaki_kernel<S, NPG, H>, 256 * H, 0));
+    FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_ozaki_kernel<S, NPG, H, GRAD>, 256 * H, 0));
     per_cu = std::max(per_cu, 1);
   }
   const bool items = cfg.order == 2;
@@ -481,14 +555,15 @@ void launch(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s)
     build_item_order(P, 16 * H * NPG);
   const int n_pos = items ? P.n_items : P.n_valid;
   const LaunchArgs la = launch_args(P, cfg, c.T, per_cu, s, n_pos);
-  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG, H>), la.grid, dim3(256 * H), 0, s, P.d_desc.ptr, la.order,
+  const GradArgs ga{c.gwave, c.gbias, c.grad, R.d_srow.ptr};
+  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG, H, GRAD>), la.grid, dim3(256 * H), 0, s, P.d_desc.ptr, la.order,
                      la.counters, n_pos, c.T, R.d_sib.ptr, P.d_map.ptr, st.d_adig.ptr, st.d_ainv.ptr,
                      st.d_bdig.ptr, st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full, P.nmb,
-                     P.n_batches, c.n_tile, (int)items);
+                     P.n_batches, c.n_tile, (int)items, ga);
   check_launch("rho1_ozaki_kernel");
 }
 
-template <int S>
+template <int S, bool GRAD>
 void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   OzakiState &st = *P.ozaki;
   const Relabel &R = get_relabel(P);
@@ -508,7 +583,7 @@ void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   hipLaunchKernelGGL(p1_rowscale_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, st.d_rowbits.ptr,
                      R.d_pinv.ptr, n, nr, st.d_ainv.ptr);
   check_launch("p1_rowscale_kernel");
-  hipLaunchKernelGGL(p1_digits_kernel<S>, dim3((n + 63) / 64, (n + 63) / 64, c.T), dim3(256), 0, s, c.P1,
+  hipLaunchKernelGGL((p1_digits_kernel<S, GRAD>), dim3((n + 63) / 64, (n + 63) / 64, c.T), dim3(256), 0, s, c.P1,
                      R.d_pinv.ptr, st.d_ainv.ptr, n, st.d_adig.ptr);
   check_launch("p1_digits_kernel");
   // point-chunk width 16 H NPG: NPG at most the widest whose accumulators, prefetched digits and
@@ -517,15 +592,15 @@ void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   // four-wave block (H = 1, 64-point passes, two blocks per CU).
   const LaunchCfg &cfg = P.cfg[S == 5 ? FRPA_PREC_OZAKI : FRPA_PREC_OZAKI4];
   if (cfg.npg == 2) {
-    launch<S, 4, 1>(P, c, cfg, s);
+    launch<S, 4, 1, GRAD>(P, c, cfg, s);
   } else if constexpr (S == 5) {
-    launch<S, 4, 2>(P, c, cfg, s);
+    launch<S, 4, 2, GRAD>(P, c, cfg, s);
   } else {
     const int npg = cfg.npg ? cfg.npg : (P.max_np <= 128 ? 4 : 5);
     if (npg == 4)
-      launch<S, 4, 2>(P, c, cfg, s);
+      launch<S, 4, 2, GRAD>(P, c, cfg, s);
     else
-      launch<S, 5, 2>(P, c, cfg, s);
+      launch<S, 5, 2, GRAD>(P, c, cfg, s);
   }
 }
 
@@ -545,15 +620,16 @@ long long ozaki_prepare_wave(Rho1Plan &P, const double *wave, int S, hipStream_t
 }
 
 void rho1_forward_ozaki(Rho1Plan &P, const Rho1Call &c, int S, hipStream_t s) {
-  FRPA_REQUIRE(!c.grad, FRPA_ERR_UNSUPPORTED, "the INT8-split path computes rho1 only (no GGA gradient)");
   if (P.n_valid == 0)
     return;
   if (!P.ozaki || P.ozaki->wave != c.wave || P.ozaki->S != S)
     ozaki_prepare_wave(P, c.wave, S, s);
+  const bool grad = c.grad != nullptr;
+  FRPA_REQUIRE(!grad || c.gwave, FRPA_ERR_INVALID, "gradient output without gradient_wave");
   if (P.ozaki->S == 4)
-    forward<4>(P, c, s);
+    grad ? forward<4, true>(P, c, s) : forward<4, false>(P, c, s);
   else
-    forward<5>(P, c, s);
+    grad ? forward<5, true>(P, c, s) : forward<5, false>(P, c, s);
 }
 
 } // namespace frpa

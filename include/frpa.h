@@ -72,7 +72,9 @@ int frpa_rho1_plan_destroy(frpa_rho1_plan *plan);
 
 /*
  * Forward (fast_rap_src_dfpt_gpu_eval/evaluate_first_order_rho.cu:188-371, same maths as
- * evaluate_first_order_rho.cpp:113-209).  Device arrays:
+ * evaluate_first_order_rho.cpp:113-209).  Every precision computes the GGA gradient when
+ * gradient_wave is given; the INT8 split then multiplies every tile pair of P1 (twice the products of
+ * rho1 alone, which only needs the symmetric half).  Device arrays:
  *   wave                       wave_batches_compress
  *   density_matrix             first_order_density_matrix (n_basis, n_basis, T)
  *   rho_bias (nullable)        first_order_rho_bias_part2 (n_max_batch_size, n_batches, T)
@@ -369,8 +371,10 @@ void *frpa_ref_stream(void);
  * the given host pointers and downloads first_order_rho and first_order_gradient_rho into them
  * (the reference's non-ALL_KERNELS_ON_GPU mode, evaluate_first_order_rho.cu:240-244,362-366), or
  * works on the device arrays only in the device-resident mode above.
- * first_order_gradient_rho is computed (GGA, FP64) unless FRPA_RHO1_GRADIENT=0 (then left zero);
- * without it FRPA_RHO1_PRECISION=ozaki / ozaki4 selects the INT8 split for first_order_rho. */
+ * Precision from the environment variable FRPA_RHO1_PRECISION: ozaki (the default: INT8 split with 5
+ * digits, ~1e-12 of max|rho1| and of max|grad rho1|), ozaki4 (4 digits) or fp64 (FP64 MFMA);
+ * first_order_gradient_rho is computed (GGA) unless FRPA_RHO1_GRADIENT=0 (then left zero, and the INT8
+ * split multiplies only the symmetric half of P1). */
 void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
     int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
     int *n_basis_, int *n_atoms_, int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_,

@@ -94,6 +94,30 @@ def test_rho1_1024atom_shapes_64bit_offsets(shape1024, precision, tol):
     assert np.all(np.isfinite(got[valid])) and err <= tol
 
 
+def test_rho1_gradient_1024atom_shapes(shape1024, oracle):
+    """The GGA gradient at the same shapes: FP64 within 1e-12, the INT8 split (every tile pair of P1)
+    within 1e-10, against the oracle; the wave buffer is passed from its first batch block (32-bit
+    offsets) so that the gradient wave (three times the wave) stays a few GB."""
+    D = shape1024
+    prob = D["prob"]
+    gwave_h = synth.random_like_wave(prob, 3, 77)
+    rho_ref, g_ref = oracle.rho1(prob, np.asfortranarray(D["P1"].view(prob.n_basis, prob.n_basis).cpu().numpy().T)
+                                 [:, :, None], gwave=gwave_h)
+    plan = api.Rho1Plan.from_problem(prob)
+    wave = D["wave"][OFFSET_BASE:]
+    gw = api.to_device(gwave_h)
+    gb = torch.zeros(3 * prob.n_max_batch_size * prob.n_batches, dtype=torch.float64, device="cuda")
+    for precision, tol in ((api.FP64, 1e-12), (api.OZAKI, 1e-10)):
+        rho, grad = plan.forward(wave, D["P1"], precision=precision, gradient_wave=gw, grad_bias=gb)
+        rho = rho.cpu().numpy()
+        grad = grad.cpu().numpy().reshape(g_ref.shape, order="F")
+        e1 = np.abs(rho - rho_ref[:, 0]).max() / np.abs(rho_ref).max()
+        e2 = np.abs(grad - g_ref).max() / np.abs(g_ref).max()
+        print(f"[rho1 + gradient 1024 atoms precision {precision}] rho1 rel {e1:.3e}, gradient rel {e2:.3e}",
+              flush=True)
+        assert e1 <= tol and e2 <= tol
+
+
 @pytest.mark.parametrize("precision,merge,tol", [(api.FP64, None, 1e-12), (api.OZAKI, "0.8", 1e-10),
                                                  (api.OZAKI, "0", 1e-10)])
 def test_adjoint_1024atom_shapes_64bit_offsets(shape1024, precision, merge, tol, monkeypatch):

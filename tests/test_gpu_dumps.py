@@ -1,8 +1,9 @@
 """Replay of a pair of dumps through the reference-named entry points (the reference's direct-test
 runner, direct_test/evaluate_first_order_rho_run.cpp): dumps written from synthetic problems, read
 back, value arrays filled at random as the runner does (wave padding rows included), then
-init_dfpt_device_data, the rho1 entry point (GGA gradient, bias, ATOM_TILE_SIZE = 2) and the whole
-Hartree sum, against the CPU oracle on the same read arrays."""
+init_dfpt_device_data, the rho1 entry point (GGA gradient, bias, ATOM_TILE_SIZE = 2; its default
+precision, the INT8 split with 5 digits, within 1e-10) and the whole Hartree sum (FP64, within
+1e-12), against the CPU oracle on the same read arrays."""
 import numpy as np
 import pytest
 
@@ -32,4 +33,4 @@ def test_replay_dumps_through_drop_in(cuda, oracle, tmp_path):
     vref = oracle.near(sv, delta_v=oracle.far(sv, delta_v=vals["delta_v_hartree"].copy(order="F")))
     vrel = np.abs(got["delta_v_hartree"] - vref).max() / np.abs(vref - vals["delta_v_hartree"]).max()
     print(f"[dump replay] rho1 rel {rel:.2e}, gradient rel {grel:.2e}, delta_v rel {vrel:.2e}", flush=True)
-    assert rel <= 1e-12 and grel <= 1e-12 and vrel <= 1e-12
+    assert rel <= 1e-10 and grel <= 1e-10 and vrel <= 1e-12

@@ -107,15 +107,21 @@ def test_reference_entry_points_against_oracle(cuda, oracle, monkeypatch):
             I("batch_sizes"), I("npb"), I("pref"), I("map"), I("ncc"), d["ibasis"], d["avn"], d["woff"], d["wave"],
             d["gwave"], d["P1"], d["rho"], d["bias"], d["grad"], d["gbias"], I("basis_atom"), zd, zd)
 
-    # a short tile: j_atom_begin = 4, j_atom_end = 4 -> one perturbation of the two
-    rho1_call(4, 4)
+    # a short tile: j_atom_begin = 4, j_atom_end = 4 -> one perturbation of the two; the default
+    # precision (INT8 split, 5 digits, GGA gradient on every tile pair) within 1e-10, FP64 within 1e-12
     ref, gref = oracle.rho1(prob, keep["P1"], T=T, bias=keep["bias"], gwave=keep["gwave"], gbias=keep["gbias"],
                             j_atom_begin=4, j_atom_end=4)
-    rel = np.abs(keep["rho"] - ref).max() / np.abs(ref).max()
-    grel = np.abs(keep["grad"] - gref).max() / np.abs(gref).max()
-    print(f"[reference rho1 entry] rel {rel:.3e}, gradient rel {grel:.3e}", flush=True)
-    assert rel <= 1e-12 and grel <= 1e-12
-    assert np.all(keep["rho"][:, 1] == 0.0)
+    for prec, tol in ((None, 1e-10), ("fp64", 1e-12)):
+        if prec:
+            monkeypatch.setenv("FRPA_RHO1_PRECISION", prec)
+        keep["rho"][:] = -7.0
+        rho1_call(4, 4)
+        rel = np.abs(keep["rho"] - ref).max() / np.abs(ref).max()
+        grel = np.abs(keep["grad"] - gref).max() / np.abs(gref).max()
+        print(f"[reference rho1 entry, precision {prec or 'default'}] rel {rel:.3e}, gradient rel {grel:.3e}",
+              flush=True)
+        assert rel <= tol and grel <= tol
+        assert np.all(keep["rho"][:, 1] == 0.0)
 
     # LDA caller without the gradient, INT8 split: rho1 within 1e-10, gradient zeroed
     monkeypatch.setenv("FRPA_RHO1_GRADIENT", "0")

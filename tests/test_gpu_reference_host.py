@@ -6,7 +6,9 @@ sum_up_whole_potential.h:3-48) links against libfrpa.so and runs the direct-test
 the rho1 GPU entry point, the rho1 CPU function's symbol, the device-resident (ALL_KERNELS_ON_GPU)
 mode with the caller filling and reading the device arrays, the Hartree GPU entry point and the
 Hartree CPU function's symbol.  Every output against the CPU oracle (ATOM_TILE_SIZE = 2 with a short
-tile, GGA gradient and bias, near + far field accumulated into delta_v_hartree): within 1e-12."""
+tile, GGA gradient and bias, near + far field accumulated into delta_v_hartree): rho1 and its gradient
+at the entry points' default precision (the INT8 split, 5 digits) within 1e-10, the Hartree sum
+(FP64) within 1e-12."""
 import os
 import subprocess
 
@@ -98,7 +100,7 @@ def test_cxx_host_against_oracle(cuda, oracle, tmp_path):
         rel = np.abs(rho - ref).max() / np.abs(ref).max()
         grel = np.abs(grad - gref).max() / np.abs(gref).max()
         print(f"[reference_host rho1 {mode}] rel {rel:.3e}, gradient rel {grel:.3e}", flush=True)
-        assert rel <= 1e-12 and grel <= 1e-12
+        assert rel <= 1e-10 and grel <= 1e-10
         assert np.all(rho[:, 1] == 0.0)  # the tile's second perturbation is outside [jb, je]
     vfar = oracle.far(fp, delta_v=np.asfortranarray(dv0.copy()), j_atom_begin=jb, j_atom_end=je)
     vref = oracle.near(fp, delta_v=vfar.copy(order="F"), j_atom_begin=jb, j_atom_end=je)

@@ -125,16 +125,31 @@ def test_rho1_int8_split_against_oracle(cuda, oracle, precision, tol):
         _check(got, ref, rel=tol, what=f"rho1 int8 split (precision {precision}, config {k})")
 
 
-def test_rho1_int8_split_rejects_gradient_and_reports_bytes(cuda):
+@pytest.mark.parametrize("precision,tol", [(api.OZAKI, 1e-10), (api.OZAKI4, 1e-8)])
+def test_rho1_int8_split_gradient_against_oracle(cuda, oracle, precision, tol):
+    """GGA gradient on the INT8 split (evaluate_first_order_rho.cpp:177-200): grad rho1 = 2 sum_i dphi_i
+    W_i - gbias with W = P1 phi for P1 as given (nonsymmetric here, so the symmetric half-product
+    would be wrong), every tile pair; nc up to 5 row tiles, np over several passes, T = 3 with a short
+    tile (rho1 of the perturbations outside it stays zero; the gradient is written for all T, as the
+    reference does), batches with n_compute_c = 0 and with no points."""
+    prob = synth.make_rho1_problem(24, 14.0, 80, (1, 300), (1, 300), config_k=8, zero_nc=(5,), zero_np=(9,))
+    assert (prob.n_compute_c_batches > 256).any() and (prob.n_point_batches > 256).any()
+    rng = np.random.default_rng(19)
+    T = 3
+    P1 = np.asfortranarray(rng.uniform(-1, 1, size=(prob.n_basis, prob.n_basis, T)))
+    bias, gbias = _bias(prob, T, 13)
+    gwave = synth.random_like_wave(prob, 3, 14)
+    ref, gref = oracle.rho1(prob, P1, T=T, bias=bias, gwave=gwave, gbias=gbias, j_atom_begin=2, j_atom_end=3)
+    got, ggot = _run_gpu(prob, P1, T, 2, bias=bias, gwave=gwave, gbias=gbias, precision=precision)
+    assert np.all(got[:, 2] == 0.0)
+    _check(got, ref, rel=tol, what=f"rho1 int8 split with gradient (precision {precision})")
+    _check(ggot, gref, rel=tol, what=f"grad rho1 int8 split (precision {precision})")
+
+
+def test_rho1_int8_split_reports_digit_bytes(cuda):
     prob = synth.make_rho1_problem(6, 9.0, 10, (10, 50), (5, 60), config_k=9)
     plan = api.Rho1Plan.from_problem(prob)
-    wave = api.to_device(prob.wave)
-    assert plan.prepare_wave(wave, api.OZAKI) > 0
-    P1 = api.to_device(np.asfortranarray(np.eye(prob.n_basis)[:, :, None]))
-    gw = api.to_device(synth.random_like_wave(prob, 3, 1))
-    from fast_rpa_eval_amd._lib import FrpaError
-    with pytest.raises(FrpaError, match="rho1 only"):
-        plan.forward(wave, P1, precision=api.OZAKI, gradient_wave=gw, grad_bias=torch.zeros_like(gw))
+    assert plan.prepare_wave(api.to_device(prob.wave), api.OZAKI) > 0
 
 
 def test_rho1_int8_split_nonsymmetric_multi_tile(cuda, oracle):
@@ -169,6 +184,11 @@ def test_rho1_launch_configs_bit_identical(cuda, oracle, precision, npgs):
     oref, _ = oracle.rho1(prob, P1h, T=T)
     tol = 1e-12 if precision == api.FP64 else 1e-10 if precision == api.OZAKI else 1e-9
     _check(ref.cpu().numpy().reshape((prob.n_full_points, T), order="F"), oref, rel=tol, what="default launch")
+    # the GGA gradient launch (every tile pair for the INT8 split) under the same configurations
+    gw = api.to_device(synth.random_like_wave(prob, 3, 24))
+    gb = torch.zeros(3 * prob.n_max_batch_size * prob.n_batches * T, dtype=torch.float64, device="cuda")
+    gref_rho, gref = (x.clone() for x in plan.forward(wave, P1, T=T, precision=precision, gradient_wave=gw,
+                                                      grad_bias=gb))
     orders = (0, 1) if precision == api.FP64 else (0, 1, 2)  # 2: point passes as separate items (INT8)
     for order in orders:
         for persistent in (0, 1, 16):
@@ -176,6 +196,9 @@ def test_rho1_launch_configs_bit_identical(cuda, oracle, precision, npgs):
                 plan.set_launch(precision, order, persistent, npg)
                 got = plan.forward(wave, P1, T=T, precision=precision)
                 assert torch.equal(got, ref), f"order {order} persistent {persistent} npg {npg}"
+                r2, g2 = plan.forward(wave, P1, T=T, precision=precision, gradient_wave=gw, grad_bias=gb)
+                assert torch.equal(r2, gref_rho) and torch.equal(g2, gref), \
+                    f"gradient launch: order {order} persistent {persistent} npg {npg}"
     if precision == api.FP64:
         with pytest.raises(RuntimeError, match="order"):
             plan.set_launch(precision, 2, 0, 0)

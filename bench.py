@@ -439,14 +439,32 @@ PROFILED_KERNEL = {  # bench kernel (path) -> kernel name prefix in the rocprofv
 }
 
 
-def read_counter_table(path):
+def read_counter_table(path, counters=("FETCH_SIZE", "WRITE_SIZE")):
     """{(kernel, counter): mean per dispatch} from a summarize_profile counter table."""
     out = {}
     for ln in open(path):
         parts = ln.split()
-        if len(parts) >= 5 and parts[-4] in ("FETCH_SIZE", "WRITE_SIZE"):
+        if len(parts) >= 5 and parts[-4] in counters:
             out[(" ".join(parts[:-4]), parts[-4])] = float(parts[-1])
     return out
+
+
+F64_COUNTERS = ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_TRANS_F64")
+
+
+def load_f64_counters(workload, kernel):
+    """Executed FP64 VALU work of one launch of `kernel` from profiles/counters_fp64_<workload>.txt:
+    64 lanes x (2 per v_fma_f64 + 1 per add / mul / transcendental) per wave instruction (lanes masked
+    off still count), and the instruction counts; None if the table is absent."""
+    path = os.path.join(ROOT, "profiles", f"counters_fp64_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    c = {k[1]: v for k, v in read_counter_table(path, F64_COUNTERS).items() if k[0].startswith(kernel)}
+    if "SQ_INSTS_VALU_FMA_F64" not in c:
+        return None
+    flop = 64.0 * (2.0 * c["SQ_INSTS_VALU_FMA_F64"] + c.get("SQ_INSTS_VALU_ADD_F64", 0.0) +
+                   c.get("SQ_INSTS_VALU_MUL_F64", 0.0) + c.get("SQ_INSTS_VALU_TRANS_F64", 0.0))
+    return {"executed_flop": flop, "wave_instructions": c}
 
 
 def traffic_table(fetch_txt, write_txt, cf_txt, cw_txt, copy_bytes=2 ** 30):
@@ -518,52 +536,72 @@ def compare_precisions(D, rho_fp64, steps):
     return res
 
 
-def cpu_baseline(D, threads, rho_s=12.0, far_s=6.0):
-    """CPU FP64 oracle on a bounded sample of the same inputs: random batches (rho1) and the first
-    far-field points, each sized from a short pilot run to take about rho_s / far_s seconds."""
+def host_cores():
+    """Threads the CPU baseline uses: OMP_NUM_THREADS if set (the GPU host's share), else every CPU in
+    this process's affinity mask; and the size of that mask."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = int(env) if env.isdigit() and int(env) > 0 else aff
+    return min(n, aff), aff
+
+
+def cpu_baseline(D, rho_s=2.0, far_s=1.0, runs=5):
+    """The CPU FP64 oracle (the reference's loops restated in C: its dgemm as a plain loop nest, no
+    BLAS) on bounded samples of the same inputs: random batches (rho1) and the first far-field points.
+    Two legs -- all host cores (OpenMP across batches / points) and one thread (the reference's own
+    build links the sequential MKL, task1_atom_tiling/version3_rho_with_bias/Makefile:17) -- each sized
+    by a pilot run (also the warm-up) to about rho_s + far_s seconds, then the median of `runs` runs."""
     from fast_rpa_eval_amd import synth
     from oracle import oracle
-    oracle.set_threads(threads)
     prob, fp = D["prob"], D["fp"]
     rng = np.random.default_rng(0)
     # the device P1 is column-major: element (r, c) at r + n_basis * c
     P1 = np.asfortranarray(D["P1"].view(prob.n_basis, prob.n_basis).cpu().numpy().T)[:, :, None]
     perm = rng.permutation(prob.n_batches)
+    cores, aff = host_cores()
 
-    def run_rho(nb):
+    def rho_sample(nb):
         sel = np.sort(perm[:nb])
-        sub = subproblem(prob, sel, D["wave"])
-        t0 = time.time()
-        out, _ = oracle.rho1(sub, P1)
-        return sel, sub, out, time.time() - t0
+        return sel, subproblem(prob, sel, D["wave"])
 
-    nb = min(256, prob.n_batches)
-    sel, sub, rho_cpu, t = run_rho(nb)
-    nb2 = int(min(prob.n_batches, max(nb, nb * rho_s / max(t, 1e-3))))
-    if nb2 > nb:
-        sel, sub, rho_cpu, t_rho = run_rho(nb2)
-    else:
-        t_rho = t
-
-    def run_far(n):
+    def far_sample(n):
         fsub = synth.FarProblem(**{**fp.__dict__})
         fsub.i_valid_point_2_i_full_points_map = fp.i_valid_point_2_i_full_points_map[:n].copy()
         fsub.n_valid_points = n
+        return fsub
+
+    def timed(f):
         t0 = time.time()
-        out = oracle.far(fsub)
+        out = f()
         return out, time.time() - t0
 
-    nfs = min(fp.n_valid_points, 2000)
-    dv_cpu, t = run_far(nfs)
-    nfs2 = int(min(fp.n_valid_points, max(nfs, nfs * far_s / max(t, 1e-3))))
-    if nfs2 > nfs:
-        nfs = nfs2
-        dv_cpu, t_far = run_far(nfs)
-    else:
-        t_far = t
-    pts_rho = int(sub.n_point_batches.sum())
-    return dict(rho_pts=pts_rho, rho_s=t_rho, far_pts=nfs, far_s=t_far, sel=sel, sub=sub, rho_cpu=rho_cpu,
-                dv_cpu=dv_cpu, nfs=nfs)
+    legs = {}
+    for name, threads in (("all_cores", cores), ("single_thread", 1)):
+        oracle.set_threads(threads)
+        nb = min(16 * threads, prob.n_batches)
+        sel, sub = rho_sample(nb)
+        _, t = timed(lambda: oracle.rho1(sub, P1))
+        nb = int(min(prob.n_batches, max(1, nb * rho_s / max(t, 1e-3))))
+        sel, sub = rho_sample(nb)
+        nf = min(fp.n_valid_points, 200 * threads)
+        _, t = timed(lambda: oracle.far(far_sample(nf)))
+        nf = int(min(fp.n_valid_points, max(64, nf * far_s / max(t, 1e-3))))
+        fsub = far_sample(nf)
+        t_rho, t_far = [], []
+        for _ in range(runs):
+            (rho_cpu, _), tr = timed(lambda: oracle.rho1(sub, P1))
+            dv_cpu, tf = timed(lambda: oracle.far(fsub))
+            t_rho.append(tr)
+            t_far.append(tf)
+        pts_rho = int(sub.n_point_batches.sum())
+        mr, mf = float(np.median(t_rho)), float(np.median(t_far))
+        legs[name] = dict(threads=threads, rho_pts=pts_rho, far_pts=nf, rho_s=mr, far_s=mf,
+                          value=(pts_rho + nf) / (mr + mf), rho_runs_s=t_rho, far_runs_s=t_far,
+                          sel=sel, sub=sub, rho_cpu=rho_cpu, dv_cpu=dv_cpu, nfs=nf)
+        log(f"CPU oracle, {threads} thread(s): rho1 {nb} batches {mr:.2f}s, far {nf} points {mf:.2f}s "
+            f"(median of {runs}) -> {legs[name]['value']:.0f} points/s")
+    legs["affinity_cpus"] = aff
+    return legs
 
 
 def subproblem(prob, sel, wave_dev):
@@ -590,6 +628,58 @@ def subproblem(prob, sel, wave_dev):
     return sub
 
 
+def rooflines(r, path, D, w, tuned, workload):
+    """{kernel: roofline record} for one path's per-kernel times r["per"] (see the roofline of main)."""
+    from fast_rpa_eval_amd import api
+    prob, fp = D["prob"], D["fp"]
+    traffic = load_traffic(workload) or {}
+    rl = {}
+    for name, tms in r["per"].items():
+        t = tms / 1e3
+        prof = PROFILED_KERNEL[(name, path)]
+        if name == "rho1" and path == "fast":
+            # the INT8 kernel's template arguments follow the chosen launch: npg 2 = four-wave workgroups
+            m = re.search(r"npg=(\d+)", tuned.get(path) or "")
+            prof += "<5,4,1" if (m and int(m.group(1)) == 2) else "<5,4,2"
+        tr = next((v for k, v in traffic.items() if k.startswith(prof)), None)
+        if name == "far":
+            from fast_rpa_eval_amd.multipole_tables import horner_fma_count
+            pairs = far_pairs(D)
+            per_pair = (2.0 * horner_fma_count(w["l_far"]) + 25) if r["var"] == api.FAR_HORNER else \
+                (2.0 * fp.tables["n_cc_lm_ijk"][w["l_far"]] + 25)
+            ach = pairs * per_pair / t / 1e12
+            rl[name] = {"kernel": prof, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                        "frac": ach / PEAK_FP64, "traffic": tr, "pairs": pairs, "flop_per_pair": per_pair,
+                        "note": "FP64 VALU (v_fma_f64); the FP64 vector and matrix peaks are equal on MI355X"}
+            ex = load_f64_counters(workload, prof)
+            if ex:
+                rl[name]["executed"] = {
+                    "flop_per_launch": ex["executed_flop"], "achieved": ex["executed_flop"] / t / 1e12,
+                    "frac": ex["executed_flop"] / t / 1e12 / PEAK_FP64,
+                    "wave_instructions": ex["wave_instructions"],
+                    "note": f"executed FP64 VALU instructions per launch (rocprofv3 counters, profiles/"
+                            f"counters_fp64_{workload}.txt) over this run's kernel time"}
+            continue
+        prec = r["prec"] if name == "rho1" else r["adj_prec"]
+        S = {api.OZAKI: 5, api.OZAKI4: 4}.get(prec)
+        ops, nbytes = work_model(prob, name, prec, S)
+        peak_ops = PEAK_INT8 if S else PEAK_FP64
+        t_ops = ops / (peak_ops * 1e12)
+        t_mem = nbytes / (PEAK_HBM * 1e9)
+        if t_mem > t_ops:
+            ach = nbytes / t / 1e9
+            rl[name] = {"kernel": prof, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM, "unit": "GB/s",
+                        "frac": ach / PEAK_HBM, "traffic": tr, "algorithmic_bytes": nbytes}
+        else:
+            ach = ops / t / 1e12
+            rl[name] = {"kernel": prof, "bound": "mfma", "achieved": ach, "peak": peak_ops,
+                        "unit": "TOP/s" if S else "TFLOP/s", "frac": ach / peak_ops, "traffic": tr,
+                        "algorithmic_bytes": nbytes}
+        rl[name]["algorithmic_ops"] = ops
+        rl[name]["other_bound_frac"] = (ops / t / 1e12 / peak_ops) if t_mem > t_ops else (nbytes / t / 1e9 / PEAK_HBM)
+    return rl
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -597,7 +687,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU oracle in the side measurements (0 = OMP_NUM_THREADS, else "
+                         "every CPU of the affinity mask)")
     ap.add_argument("--paths", default="fast,fp64")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default rho1 launch configuration")
     ap.add_argument("--compare-precisions", type=int, default=1,
@@ -630,7 +722,13 @@ def main():
     ap.add_argument("--traffic-table", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="only print the HBM traffic table from four summaries made by --summarize-profile "
                          "(kernel FETCH_SIZE, kernel WRITE_SIZE, 1 GiB copy FETCH_SIZE, copy WRITE_SIZE)")
+    ap.add_argument("--config3", type=int, default=1,
+                    help="1: also run the 512-atom, 20k-batch rho1 workload (n_compute 80-800) in every precision, "
+                         "with and without the GGA gradient: ms and errors against the FP64 path (reported as "
+                         "rho1_512atoms_precisions, outside the step; run last)")
     a = ap.parse_args()
+    if a.cpu_threads <= 0:
+        a.cpu_threads = host_cores()[0]
     if a.traffic_table:
         print(traffic_table(*a.traffic_table))
         return
@@ -729,6 +827,36 @@ def main():
         line["fp64_value"] = pts / (res["fp64"]["ms"] / 1e3)
         line["fp64_ms_per_step"] = res["fp64"]["ms"]
         line["speedup_vs_fp64"] = res["fp64"]["ms"] / fast["ms"]
+    if "fp64" in res and "fast" in res and set(res["fp64"]["per"]) == set(res["fast"]["per"]) and "far" in names:
+        # the FP64 baseline with the same far-field kernel as the fast path (FP64 Horner), so that the
+        # ratio measures the precision change of rho1 and its adjoint alone; per-kernel eager times
+        f, d = res["fast"]["per"], res["fp64"]["per"]
+        t_f = sum(f.values())
+        t_d = t_f - f["far"] + sum(v for k, v in d.items() if k != "far")
+        line["speedup_vs_fp64_same_far"] = {
+            "value": t_d / t_f, "fast_ms": t_f, "fp64_ms": t_d,
+            "note": "FP64 rho1 + FP64 adjoint + the FP64 Horner far field against the fast path, from the "
+                    "per-kernel eager times; speedup_vs_fp64 also counts the far field's change of formulation "
+                    "(term by term in the plain FP64 path, Horner in both paths here).  The FP64 rho1 kernel "
+                    "multiplies the full P1 (as the reference); the INT8 split only the upper tile pairs of its "
+                    "symmetric part"}
+    if "fast" in res and have_ozaki:
+        # the one-off conversion of the wave into the INT8 digits of the forward and of the adjoint
+        # (frpa_rho1_prepare_wave rebuilds both once the adjoint has made its own), kept while the wave
+        # is unchanged; outside the timed step
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        nbytes = D["plan"].prepare_wave(D["wave"], api.OZAKI)
+        e1.record()
+        torch.cuda.synchronize()
+        one_ms = e0.elapsed_time(e1)
+        line["one_off"] = {"ms": one_ms, "bytes": nbytes, "wave_bytes": int(prob.wave_offset[-1]) * 8,
+                           "ms_per_step_with_one_off": fast["ms"] + one_ms,
+                           "note": "phi_digits_kernel (forward) + h1_digits_kernel and h1_pointmax_kernel (adjoint) "
+                                   "and their host set-up, once per wave (the wave is fixed across the CPSCF "
+                                   "iterations and perturbations that reuse it); bytes = device memory the digits "
+                                   "hold"}
+        log(f"one-off digit conversion: {one_ms:.1f} ms, {nbytes / 1e9:.1f} GB")
     if "fp64" in res and "fast" in res and all(k in res["fast"]["per"] for k in ("rho1", "far")):
         f, d = res["fast"]["per"], res["fp64"]["per"]
         t_f, t_d = f["rho1"] + f["far"], d["rho1"] + d["far"]
@@ -755,49 +883,16 @@ def main():
     if a.compare_precisions and "fp64" in outs and "rho1" in names:
         line["rho1_precision_comparison"] = compare_precisions(D, outs["fp64"]["rho"], a.steps)
 
-    # roofline of every kernel of the fast path: algorithmic work / measured time against the peak of
-    # whichever resource binds the kernel at its algorithmic work; the dominant kernel is "roofline"
-    traffic = load_traffic(a.workload) or {}
-    rl = {}
-    for name, tms in fast["per"].items():
-        t = tms / 1e3
-        path = "fast" if "fast" in res else "fp64"
-        prof = PROFILED_KERNEL[(name, path)]
-        if name == "rho1" and path == "fast":
-            # the INT8 kernel's template arguments follow the chosen launch: npg 2 = four-wave workgroups
-            m = re.search(r"npg=(\d+)", tuned.get(path) or "")
-            prof += "<5,4,1>" if (m and int(m.group(1)) == 2) else "<5,4,2>"
-        tr = next((v for k, v in traffic.items() if k.startswith(prof)), None)
-        if name == "far":
-            from fast_rpa_eval_amd.multipole_tables import horner_fma_count
-            pairs = far_pairs(D)
-            per_pair = (2.0 * horner_fma_count(w["l_far"]) + 25) if fast["var"] == api.FAR_HORNER else \
-                (2.0 * fp.tables["n_cc_lm_ijk"][w["l_far"]] + 25)
-            ach = pairs * per_pair / t / 1e12
-            rl[name] = {"kernel": prof, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                        "frac": ach / PEAK_FP64, "traffic": tr, "pairs": pairs, "flop_per_pair": per_pair,
-                        "note": "FP64 VALU (v_fma_f64); the FP64 vector and matrix peaks are equal on MI355X"}
-            continue
-        prec = fast["prec"] if name == "rho1" else fast["adj_prec"]
-        S = {api.OZAKI: 5, api.OZAKI4: 4}.get(prec)
-        ops, nbytes = work_model(prob, name, prec, S)
-        peak_ops = PEAK_INT8 if S else PEAK_FP64
-        t_ops = ops / (peak_ops * 1e12)
-        t_mem = nbytes / (PEAK_HBM * 1e9)
-        if t_mem > t_ops:
-            ach = nbytes / t / 1e9
-            rl[name] = {"kernel": prof, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM, "unit": "GB/s",
-                        "frac": ach / PEAK_HBM, "traffic": tr, "algorithmic_bytes": nbytes}
-        else:
-            ach = ops / t / 1e12
-            rl[name] = {"kernel": prof, "bound": "mfma", "achieved": ach, "peak": peak_ops,
-                        "unit": "TOP/s" if S else "TFLOP/s", "frac": ach / peak_ops, "traffic": tr,
-                        "algorithmic_bytes": nbytes}
-        rl[name]["algorithmic_ops"] = ops
-        rl[name]["other_bound_frac"] = (ops / t / 1e12 / peak_ops) if t_mem > t_ops else (nbytes / t / 1e9 / PEAK_HBM)
+    # roofline of every kernel of each path: algorithmic work / measured time against the peak of
+    # whichever resource binds the kernel at its algorithmic work; the fast path's dominant kernel is
+    # "roofline"
+    rls = {p: rooflines(res[p], p, D, w, tuned, a.workload) for p in res}
+    rl = rls["fast" if "fast" in res else "fp64"]
     dom = max(fast["per"], key=fast["per"].get)
     line["roofline"] = rl[dom]
     line["roofline_all"] = rl
+    if "fp64" in rls and "fast" in rls:
+        line["roofline_fp64_path"] = rls["fp64"]
 
     if a.near:
         N = make_near_inputs(D, w)
@@ -859,23 +954,29 @@ def main():
         torch.cuda.empty_cache()
 
     if not a.no_cpu:
-        cb = cpu_baseline(D, a.cpu_threads)
-        cpu_pps = (cb["rho_pts"] + cb["far_pts"]) / (cb["rho_s"] + cb["far_s"])
+        legs = cpu_baseline(D)
+        cb, c1 = legs["all_cores"], legs["single_thread"]
         line["cpu_baseline"] = {
-            "value": cpu_pps, "unit": "points/s", "cores": a.cpu_threads, "kind": "port",
-            "sample": f"{len(cb['sel'])} random batches ({cb['rho_pts']} rho1 points, {cb['rho_s']:.1f}s) + "
-                      f"{cb['far_pts']} far-field points x {fp.n_atoms} centres ({cb['far_s']:.1f}s), "
-                      f"CPU FP64 oracle, OpenMP {a.cpu_threads} threads"}
-        # FP64 GPU path against the oracle on the sample
-        if "fp64" in outs:
+            "value": cb["value"], "unit": "points/s", "cores": cb["threads"], "kind": "port",
+            "statistic": "median of 5 runs after a pilot run",
+            "sample": f"{len(cb['sel'])} random batches ({cb['rho_pts']} rho1 points, {cb['rho_s']:.2f}s) + "
+                      f"{cb['far_pts']} far-field points x {fp.n_atoms} centres ({cb['far_s']:.2f}s), CPU FP64 "
+                      f"oracle (the reference's dgemm as a plain loop nest, no BLAS), OpenMP {cb['threads']} "
+                      f"threads of {legs['affinity_cpus']} CPUs in the affinity mask",
+            "single_thread": {"value": c1["value"], "cores": 1,
+                              "sample": f"{len(c1['sel'])} random batches ({c1['rho_pts']} rho1 points, "
+                                        f"{c1['rho_s']:.2f}s) + {c1['far_pts']} far-field points "
+                                        f"({c1['far_s']:.2f}s), median of 5"},
+            "runs_s": {"all_cores": {"rho1": cb["rho_runs_s"], "far": cb["far_runs_s"]},
+                       "single_thread": {"rho1": c1["rho_runs_s"], "far": c1["far_runs_s"]}}}
+        # the GPU paths against the oracle on the sample
+        if "fp64" in outs or "fast" in outs:
             ref = cb["rho_cpu"][:, 0]
             sub = cb["sub"]
-            got = outs["fp64"]["rho"].cpu().numpy()
             mp_sub = sub.i_valid_point_2_i_full_points_map[:sub.n_point_batches_prefix_sum[-1]]
-            line["rho1_fp64_vs_oracle_rel"] = float(np.abs(got[mp_sub] - ref[mp_sub]).max() / np.abs(ref[mp_sub]).max())
-            if "fast" in outs:
-                gotf = outs["fast"]["rho"].cpu().numpy()
-                line["rho1_fast_vs_oracle_rel"] = float(np.abs(gotf[mp_sub] - ref[mp_sub]).max() /
+            for p in outs:
+                got = outs[p]["rho"].cpu().numpy()
+                line[f"rho1_{p}_vs_oracle_rel"] = float(np.abs(got[mp_sub] - ref[mp_sub]).max() /
                                                         np.abs(ref[mp_sub]).max())
             mpf = fp.i_valid_point_2_i_full_points_map[:cb["nfs"]]
             dref = cb["dv_cpu"][mpf, 0]
@@ -895,7 +996,146 @@ def main():
             log(f"H1 GGA: {line['h1_gga']['ms']:.3f} ms, {line['h1_gga']['roofline']['achieved']:.1f} TF/s")
         except torch.cuda.OutOfMemoryError as e:
             log(f"H1 GGA skipped: {e}")
+    if a.config3:
+        D.clear()
+        import gc
+        gc.collect()
+        torch.cuda.empty_cache()
+        try:
+            line["rho1_512atoms_precisions"] = bench_config3(a.steps, a.warmup, a.no_cpu, a.cpu_threads)
+        except torch.cuda.OutOfMemoryError as e:
+            log(f"512-atom precision leg skipped: {e}")
+    leftover = child_processes()
+    if leftover:
+        line["child_processes_at_exit"] = leftover
     print(json.dumps(line), flush=True)
+
+
+def child_processes():
+    """Child processes of this one still alive at the end (their command lines; the pool of the
+    oracle's OpenMP runtime is threads, not processes).  Each is ended and waited for, so that none
+    outlives the bench."""
+    try:
+        import psutil
+    except ImportError:
+        return []
+    kids = psutil.Process().children(recursive=True)
+    out = []
+    for k in kids:
+        try:
+            out.append(" ".join(k.cmdline())[:200])
+            k.terminate()
+        except psutil.Error:
+            pass
+    psutil.wait_procs(kids, timeout=5)
+    if out:
+        log(f"child processes at exit (ended): {out}")
+    return out
+
+
+C3_WORKLOAD = "rho1_512atoms_20kbatches_far200k_l6"
+
+
+def bench_config3(steps, warmup, no_cpu, threads):
+    """rho1 on the 512-atom, 20k-batch workload (points 50-200, n_compute 80-800 nearest basis
+    functions, P1 = N(0,1) exp(-d/2 bohr) symmetric) in every precision: ms per call (HIP events on
+    torch's current stream, default launch of each precision after one autotune of the INT8 split and
+    FP64), and per precision the error against the FP64 GPU path over all points: max |d rho1| / max
+    |rho1|, the median over points with |rho1| > 1e-3 max of |d rho1(p)| / |rho1(p)|, and the maximum
+    of |d rho1(p)| / cond(p) with cond(p) = sum_ij |phi_i(p)| |P1_ij| |phi_j(p)| (the FP64 path run on
+    |phi| and |P1|).  With the GGA gradient (random gradient wave): FP64 against the INT8 split.  The
+    FP64 path is checked against the CPU oracle on a sample of batches."""
+    import torch
+
+    from fast_rpa_eval_amd import api, synth
+    from fast_rpa_eval_amd._lib import FrpaError
+    w3 = WORKLOADS[C3_WORKLOAD]
+    t0 = time.time()
+    prob = synth.make_rho1_problem(w3["n_atoms"], w3["box_ang"], w3["n_batches"], w3["np_range"], w3["nc_range"],
+                                   config_k=w3["config_k"], fill_wave=False)
+    plan = api.Rho1Plan.from_problem(prob)
+    wl = int(prob.wave_offset[-1])
+    wave = plan.synth_wave(prob.atom_xyz, prob.basis, api.to_device(prob.coord_points), wl)
+    g = torch.Generator(device="cuda").manual_seed(synth.seeds(w3["config_k"])["p1"])
+    nb = prob.n_basis
+    A = torch.randn(nb, nb, generator=g, device="cuda", dtype=torch.float64)
+    A = torch.triu(A) + torch.triu(A, 1).T
+    bx = torch.from_numpy(prob.atom_xyz[prob.basis.atom]).cuda()
+    A *= torch.exp(-torch.cdist(bx, bx) / 2.0)
+    P1 = A.contiguous().view(-1)
+    del A, bx
+    pts = int(prob.n_point_batches[prob.n_compute_c_batches > 0].sum())
+    log(f"512-atom workload ready: {prob.n_batches} batches, {nb} basis, {pts} points, wave {wl * 8 / 1e9:.1f} GB "
+        f"({time.time() - t0:.1f}s)")
+    valid = torch.from_numpy(prob.valid_map().astype(np.int64)).cuda()
+    out = torch.empty(prob.n_full_points, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        for _ in range(max(warmup, 1)):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    tuned = {}
+    for prec in (api.FP64, api.OZAKI):
+        tuned[prec] = plan.autotune(wave, P1, precision=prec, reps=2).split(";")[0]
+    ref = plan.forward(wave, P1, precision=api.FP64)[valid].clone()
+    cond = plan.forward(wave.abs(), P1.abs(), precision=api.FP64)[valid].clone()
+    big = ref.abs() > 1e-3 * ref.abs().max()
+    res = {"workload": f"{C3_WORKLOAD}: {prob.n_batches} batches, {nb} basis functions, n_compute 80-800, "
+                       f"{pts} points", "points": pts,
+           "launch": {"fp64": tuned[api.FP64], "int8_split_s5": tuned[api.OZAKI]}, "precisions": {}}
+    for name, prec in (("fp64", api.FP64), ("int8_split_s5", api.OZAKI), ("int8_split_s4", api.OZAKI4),
+                       ("fp32_mfma", api.FP32), ("bf16x3", api.BF16X3)):
+        try:
+            ms = timed(lambda: plan.forward(wave, P1, precision=prec, out=out))
+        except FrpaError as e:
+            res["precisions"][name] = {"error": str(e)}
+            continue
+        d = (out[valid] - ref).abs()
+        res["precisions"][name] = {
+            "ms": ms, "points_per_s": pts / (ms / 1e3), "max_rel_err": float(d.max() / ref.abs().max()),
+            "median_pointwise_rel_err": float((d[big] / ref[big].abs()).median()),
+            "max_cond_normalised_err": float((d / cond).max())}
+        log(f"512 atoms, {name}: {res['precisions'][name]}")
+    fp64_ms = res["precisions"]["fp64"]["ms"]
+    for v in res["precisions"].values():
+        if "ms" in v:
+            v["speedup_vs_fp64"] = fp64_ms / v["ms"]
+    # GGA gradient: FP64 and the INT8 split (every tile pair of P1) on a random gradient wave
+    gw = torch.empty(3 * wl, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0, generator=g)
+    gb = torch.zeros(3 * prob.n_max_batch_size * prob.n_batches, dtype=torch.float64, device="cuda")
+    gout = torch.empty_like(gb)
+    gga = {}
+    for name, prec in (("fp64", api.FP64), ("int8_split_s5", api.OZAKI)):
+        ms = timed(lambda: plan.forward(wave, P1, precision=prec, out=out, gradient_wave=gw, grad_bias=gb,
+                                        grad_out=gout))
+        gga[name] = {"ms": ms, "rho": out[valid].clone(), "grad": gout.clone()}
+    r_ref, g_ref = gga["fp64"]["rho"], gga["fp64"]["grad"]
+    res["with_gradient"] = {
+        n: {"ms": v["ms"], "points_per_s": pts / (v["ms"] / 1e3), "speedup_vs_fp64": gga["fp64"]["ms"] / v["ms"],
+            "rho1_max_rel_err": float((v["rho"] - r_ref).abs().max() / r_ref.abs().max()),
+            "grad_max_rel_err": float((v["grad"] - g_ref).abs().max() / g_ref.abs().max())}
+        for n, v in gga.items()}
+    log(f"512 atoms with the gradient: {res['with_gradient']}")
+    del gw, gb, gout, gga
+    if not no_cpu:
+        from oracle import oracle
+        pick = np.sort(np.random.default_rng(3).choice(prob.n_batches, size=64, replace=False))
+        sub = subproblem(prob, pick, wave)
+        oracle.set_threads(threads)
+        P1h = np.asfortranarray(P1.view(nb, nb).cpu().numpy().T)[:, :, None]
+        rc, _ = oracle.rho1(sub, P1h)
+        got = plan.forward(wave, P1, precision=api.FP64).cpu().numpy()
+        mp = sub.valid_map()
+        res["fp64_vs_oracle_rel"] = float(np.abs(got[mp] - rc[mp, 0]).max() / np.abs(rc[mp, 0]).max())
+    return res
 
 
 def summarize_profile(d):

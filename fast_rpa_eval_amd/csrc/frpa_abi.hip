@@ -536,8 +536,10 @@ int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wa
     const int S = precision == FRPA_PREC_OZAKI ? 5 : 4;
     long long b = ozaki_prepare_wave(P, wave, S, as_stream(stream));
     // the adjoint keeps digits of its own: rebuild them too once the adjoint has made them
-    if (adjoz_built(P))
+    if (adjoz_built(P)) {
       adjoz_prepare(P, S, wave, as_stream(stream));
+      b += adjoz_bytes(P);
+    }
     if (bytes)
       *bytes = b;
   });

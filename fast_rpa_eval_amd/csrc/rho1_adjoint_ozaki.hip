@@ -791,6 +791,13 @@ void adjoz_invalidate(Rho1Plan &P) {
 
 bool adjoz_built(const Rho1Plan &P) { return P.adjoz && P.adjoz->wave; }
 
+long long adjoz_bytes(const Rho1Plan &P) {
+  if (!adjoz_built(P))
+    return 0;
+  const AdjOzState &st = *P.adjoz;
+  return st.bytes + (long long)(st.d_finv.n + st.d_pmax.n) * 8;
+}
+
 const double *h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *g, unsigned long long *Hs,
                             hipStream_t s) {
   FRPA_REQUIRE(S == 4 || S == 5, FRPA_ERR_INVALID, "INT8 split supports 4 or 5 slices, not %d", S);

@@ -130,6 +130,7 @@ const double *h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const
 void adjoz_prepare(Rho1Plan &P, int S, const double *wave, hipStream_t s);
 void adjoz_invalidate(Rho1Plan &P);
 bool adjoz_built(const Rho1Plan &P);
+long long adjoz_bytes(const Rho1Plan &P);  // device bytes of the adjoint's digits, scales and point maxima
 // Slot waits of the INT8 adjoint that gave up since the last read (read and cleared; synchronises s)
 int adjoz_take_wait_failures(Rho1Plan &P, hipStream_t s);
 

@@ -137,7 +137,8 @@ int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int per
  * first INT8-split call and reused while the same wave pointer is passed: after changing the wave
  * in place (or passing new contents at the same address), call this function, which rebuilds the
  * forward's digits and, once the adjoint has made its own, the adjoint's.  Optional otherwise.
- * Returns the device bytes held by the forward's digits in *bytes (nullable). */
+ * Returns in *bytes (nullable) the device bytes the digits it made hold (the forward's, plus the
+ * adjoint's when it rebuilt them). */
 int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wave, long long *bytes,
                            void *stream);
 

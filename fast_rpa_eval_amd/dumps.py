@@ -97,6 +97,32 @@ def read(path, kind):
     return d
 
 
+def array_lengths(kind, scalars):
+    """Element count of every array of a dump, as the scalars fix it: the reference's host program
+    writes the arrays with these shapes (the atom tile of the sum-up arrays is j_atom_end -
+    j_atom_begin + 1; i_valid_point_2_i_full_points_map of the rho dump has one entry per full grid
+    point, of the sum-up dump one per valid point)."""
+    s = scalars
+    if kind == "rho":
+        nb, nbI = s["n_my_batches_work"], s["n_centers_basis_I"]
+        return dict(basis_atom=s["n_basis"], batch_sizes=nb, n_point_batches=nb, n_point_batches_prefix_sum=nb + 1,
+                    i_valid_point_2_i_full_points_map=s["n_full_points"], n_compute_c_batches=nb,
+                    i_basis_batches=nbI * nb, atom_valid_n_compute_c_batches=(s["n_atoms"] + 1) * nb,
+                    i_batch_2_wave_offset=nb + 1)
+    if kind == "sumup":
+        T = s["j_atom_end"] - s["j_atom_begin"] + 1
+        na, ns, nc = s["n_atoms"], s["n_species"], s["n_centers"]
+        lp1, lama1, d0 = s["l_pot_max"] + 1, s["l_max_analytic_multipole"] + 1, s["index_cc_dim_0"]
+        return dict(n_radial=ns, species=na, species_center=nc, l_hartree_max_far_distance=na * T,
+                    i_valid_point_2_i_full_points_map=s["n_valid_points"], coord_points=3 * s["n_full_points"],
+                    multipole_radius_sq=na * T, outer_potential_radius=lp1 * na * T,
+                    multipole_moments=lp1 * lp1 * na * T, l_hartree=ns, n_grid=ns, n_cc_lm_ijk=lama1,
+                    centers_hartree_potential=s["n_centers_hartree_potential"], center_to_atom=nc,
+                    index_cc=d0 * 6, index_ijk_max_cc=3 * lama1, cc=d0, coords_center=3 * nc, r_grid_min=ns,
+                    log_r_grid_inc=ns, scale_radial=ns, partition_tab=s["n_full_points"])
+    raise ValueError(kind)
+
+
 def file_names(nproc):
     """(rho meta dump, sum-up dump) names for a run on nproc MPI ranks (rank 0 writes them)."""
     return f"first_order_rho.meta.test.nproc_{nproc}.bin", f"sumup.test.nproc_{nproc}.bin"

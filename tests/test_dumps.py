@@ -141,3 +141,29 @@ def test_read_dumps_drive_the_oracle_like_the_problems(tmp_path, oracle):
     v = sumup_view(sd, T, spl)
     assert np.array_equal(oracle.far(fp), oracle.far(v))
     assert np.array_equal(oracle.near(fp), oracle.near(v))
+
+
+def test_writer_byte_count_matches_the_reference_dumps(tmp_path):
+    """The reference's FHI-aims dumps (Git-LFS pointers: fragment_save_bin/case_18 .. case_32, 8 cases)
+    record each file's byte size, and its direct-test log echoes the scalars read from each (fixture
+    tests/golden/dump_sizes.json, made by make_dump_sizes.py).  A dump written with those scalars and
+    arrays of the lengths they fix (dumps.array_lengths) has exactly the recorded size, for both
+    formats and every case: the scalar count and type, every array's element count, element type and
+    size_t count prefix agree with the reference's files."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dump_sizes.json")))
+    assert len(fx["cases"]) == 8
+    for case, e in fx["cases"].items():
+        for kind in ("rho", "sumup"):
+            sc = e[f"{kind}_scalars"]
+            _, arrs = dumps._layout(kind)
+            lens = dumps.array_lengths(kind, sc)
+            arrays = {name: np.zeros(lens[name], dtype=np.int32 if t == "i" else np.float64) for name, t in arrs}
+            path = tmp_path / f"{kind}_{case}.bin"
+            dumps.write(path, dumps.Dump(kind, dict(sc), arrays))
+            size = os.path.getsize(path)
+            assert size == e[f"{kind}_bytes"], (case, kind, size, e[f"{kind}_bytes"])
+            back = dumps.read(path, kind)
+            assert back.scalars == sc
+            os.remove(path)

@@ -2,7 +2,10 @@
 
 LDA part: closed form H1_ij = sum_p (H_prefactor1 + w v1) phi_i phi_j scattered to (ib_i, ib_j), and
 the adjoint identity <w v1, rho1(P1)> = <H1, P1>_F tying it to the rho1 oracle.
-GGA part: an einsum restatement on tiny inputs (prefactors of :144-189).
+GGA part: the sigma1 / pref2 / pref3 terms are the derivative of the XC potential matrix of a model
+functional in the direction (rho1, grad rho1) (central differences); the Hessian (pref5) term is the
+derivative of its bilinear form under a displacement of the perturbed atom (central differences of
+the synthetic basis); plus an einsum restatement on tiny inputs.
 """
 import numpy as np
 import pytest
@@ -111,3 +114,168 @@ def test_h1_gga_terms_against_einsum(prob, oracle):
         ib = prob.ibasis[prob.ibasis_ptr[b]:prob.ibasis_ptr[b + 1]] - 1
         ref[np.ix_(ib, ib)] += C @ phi.T + phi @ C.T
     np.testing.assert_allclose(H[:, :, 0], ref, rtol=1e-11, atol=1e-11)
+
+
+def _batch(prob, b):
+    nc, npt = int(prob.n_compute_c_batches[b]), int(prob.n_point_batches[b])
+    ib = prob.ibasis[prob.ibasis_ptr[b]:prob.ibasis_ptr[b + 1]] - 1
+    s = prob.n_point_batches_prefix_sum[b]
+    mp = prob.i_valid_point_2_i_full_points_map[s:s + npt]
+    ncp = (nc + 3) // 4 * 4
+    return nc, npt, ncp, ib, mp
+
+
+def _per_batch(prob, k, fill):
+    """(k, nmb) block per batch: component 0 from fill(b, p) at the batch's points, the other
+    components and the padding random (the reference reads component 0 only)."""
+    rng = np.random.default_rng(99 + k)
+    nmb, nb = prob.n_max_batch_size, prob.n_batches
+    out = rng.uniform(-5, 5, size=(k, nmb, nb))
+    for b in range(nb):
+        nc, npt, _, _, mp = _batch(prob, b)
+        out[0, :npt, b] = fill(mp)
+    return np.asfortranarray(out).ravel(order="F")
+
+
+def test_h1_gga_terms_are_the_derivative_of_the_xc_potential_matrix(prob, oracle):
+    """For a GGA functional f(rho, sigma), sigma = |grad rho|^2, the XC potential matrix is
+    V_ij[rho] = sum_p w [f_rho phi_i phi_j + 2 f_sigma grad rho . grad(phi_i phi_j)], and the GGA part of
+    the first-order H (evaluate_first_order_H.cpp:144-189: sigma1, pref2, pref3 with the Hartree and
+    H_prefactor1 terms zero) is its derivative d/de V[rho0 + e rho1] at e = 0, with grad rho1 moving
+    grad rho.  Model f = a rho^2 + b rho sigma + c sigma^2 + d rho^3 sigma, whose second derivatives
+    (v2rho2, v2rhosigma, v2sigma2) and f_sigma (vsigma) are the oracle's inputs at rho0; the derivative
+    is taken by central differences of V (e = 1e-5).  The per-batch XC blocks carry random values in
+    the components the reference does not read."""
+    rng = np.random.default_rng(7)
+    A, B, C, Dd = 0.7, -0.3, 0.2, 0.05
+    f_r = lambda r, s: 2 * A * r + B * s + 3 * Dd * r * r * s
+    f_s = lambda r, s: B * r + 2 * C * s + Dd * r ** 3
+    f_rr = lambda r, s: 2 * A + 6 * Dd * r * s
+    f_rs = lambda r, s: B + 3 * Dd * r * r
+    f_ss = lambda r, s: 2 * C + 0 * r
+    nf, nmb, nb = prob.n_full_points, prob.n_max_batch_size, prob.n_batches
+    rho0 = rng.uniform(0.2, 1.5, nf)
+    g0 = rng.uniform(-1, 1, (3, nf))
+    rho1 = rng.uniform(-1, 1, nf)
+    g1 = rng.uniform(-1, 1, (3, nf))
+    w = rng.uniform(0.1, 1.0, nf)
+    s0 = np.sum(g0 * g0, axis=0)
+    gw = synth.random_like_wave(prob, 3, 8)
+    a = _zero_arrays(prob, 1)
+    a.update(gwave=gw, partition_tab=w, gradient_rho=np.asfortranarray(g0),
+             first_order_rho=np.asfortranarray(rho1[:, None]),
+             vsigma=_per_batch(prob, 3, lambda q: f_s(rho0[q], s0[q])),
+             v2rho2=_per_batch(prob, 3, lambda q: f_rr(rho0[q], s0[q])),
+             v2rhosigma=_per_batch(prob, 6, lambda q: f_rs(rho0[q], s0[q])),
+             v2sigma2=_per_batch(prob, 6, lambda q: f_ss(rho0[q], s0[q])))
+    gr1 = np.zeros((3, nmb, nb, 1), order="F")
+    for b in range(nb):
+        _, npt, _, _, mp = _batch(prob, b)
+        gr1[:, :npt, b, 0] = g1[:, mp]
+    a["grad_rho1"] = gr1
+    H = np.zeros((prob.n_basis, prob.n_basis, 1), order="F")
+    oracle.h1(prob, 1, 1, 1, 1, H, a)
+
+    def V(e):
+        out = np.zeros((prob.n_basis, prob.n_basis))
+        for b in range(nb):
+            nc, npt, ncp, ib, mp = _batch(prob, b)
+            if nc == 0 or npt == 0:
+                continue
+            phi = prob.wave[prob.wave_offset[b]:prob.wave_offset[b + 1]].reshape((ncp, npt), order="F")[:nc]
+            dphi = gw[3 * prob.wave_offset[b]:3 * prob.wave_offset[b + 1]].reshape((ncp, 3, npt), order="F")[:nc]
+            r = rho0[mp] + e * rho1[mp]
+            g = g0[:, mp] + e * g1[:, mp]
+            s = np.sum(g * g, axis=0)
+            gd = np.einsum("jp,ijp->ip", g, dphi)  # grad rho . grad phi_i
+            loc = (phi * (w[mp] * f_r(r, s))) @ phi.T
+            loc += (gd * (2 * w[mp] * f_s(r, s))) @ phi.T + (phi * (2 * w[mp] * f_s(r, s))) @ gd.T
+            out[np.ix_(ib, ib)] += loc
+        return out
+
+    e = 1e-5
+    fd = (V(e) - V(-e)) / (2 * e)
+    err = np.abs(H[:, :, 0] - fd).max()
+    print(f"[h1 gga vs d/de V_xc] max|err| {err:.3e}, max|H1| {np.abs(fd).max():.3e}")
+    assert np.abs(fd).max() > 1.0
+    assert err <= 1e-7 * np.abs(fd).max()
+
+
+def _fd_derivs(prob, ib, pts, xyz, h1=1e-5, h2=1e-4):
+    """First derivatives (nc, 3, np) and the six second derivatives (nc, 6, np) in the order
+    xx, xy, xz, yy, yz, zz of the synthetic basis at pts, by central differences."""
+    f = lambda d: synth.eval_basis(xyz, prob.basis, ib, pts + d)
+    E = np.eye(3)
+    g = np.stack([(f(h1 * E[j]) - f(-h1 * E[j])) / (2 * h1) for j in range(3)], axis=1)
+    f0 = f(np.zeros(3))
+    hs = []
+    for a, c in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)):
+        if a == c:
+            hs.append((f(h2 * E[a]) - 2 * f0 + f(-h2 * E[a])) / (h2 * h2))
+        else:
+            hs.append((f(h2 * (E[a] + E[c])) - f(h2 * (E[a] - E[c])) - f(h2 * (E[c] - E[a])) +
+                       f(-h2 * (E[a] + E[c]))) / (4 * h2 * h2))
+    return g, np.stack(hs, axis=1)
+
+
+@pytest.mark.parametrize("j_coord", [1, 2, 3])
+def test_h1_hessian_term_is_the_derivative_under_an_atom_displacement(oracle, j_coord):
+    """The Hessian term of the first-order H (evaluate_first_order_H.cpp:170-189: C -= sum_c pref5_c
+    d2phi_{c, j_coord} on the rows of the perturbed atom, pref5 = 2 vsigma w grad rho0), alone (every
+    other XC input zero): it is d/dR of M_ij(R) = sum_p [D_i phi_j + phi_i D_j], D_k = pref5 . grad
+    phi_k(r_p; R) for the basis functions of the perturbed atom moved by R along j_coord (0 for the
+    others), with phi held fixed.  The oracle gets the synthetic basis' true first and second
+    derivatives (central differences) in the wave layouts; M is differentiated by moving the atom
+    (central differences, 1e-4 bohr).  A wrong Hessian component (index_hessian), row range or sign
+    fails it."""
+    prob = synth.make_rho1_problem(5, 8.0, 14, (10, 40), None, config_k=23)
+    rng = np.random.default_rng(11)
+    nf, nmb, nb = prob.n_full_points, prob.n_max_batch_size, prob.n_batches
+    ja = 2  # 1-based perturbed atom
+    wl = int(prob.wave_offset[-1])
+    gw, hw = np.full(3 * wl, np.nan), np.full(6 * wl, np.nan)
+    for b in range(nb):
+        nc, npt, ncp, ib, mp = _batch(prob, b)
+        if nc == 0 or npt == 0:
+            continue
+        g, hs = _fd_derivs(prob, ib, prob.coord_points[:, mp].T, prob.atom_xyz)
+        gw[3 * prob.wave_offset[b]:3 * prob.wave_offset[b + 1]].reshape((ncp, 3, npt), order="F")[:nc] = g
+        hw[6 * prob.wave_offset[b]:6 * prob.wave_offset[b + 1]].reshape((ncp, 6, npt), order="F")[:nc] = hs
+    w = rng.uniform(0.1, 1.0, nf)
+    g0 = rng.uniform(-1, 1, (3, nf))
+    vs = rng.uniform(-1, 1, nf)
+    a = _zero_arrays(prob, 1)
+    a.update(gwave=gw, hwave=hw, partition_tab=w, gradient_rho=np.asfortranarray(g0),
+             vsigma=_per_batch(prob, 3, lambda q: vs[q]))
+    for k in ("v2rho2", "v2rhosigma", "v2sigma2"):  # read at component 0 only, which must be zero here
+        kk = 3 if k == "v2rho2" else 6
+        a[k] = _per_batch(prob, kk, lambda q: 0.0 * q)
+    H = np.zeros((prob.n_basis, prob.n_basis, 1), order="F")
+    oracle.h1(prob, 1, ja, ja, j_coord, H, a)
+
+    def M(delta):
+        xyz = prob.atom_xyz.copy()
+        xyz[ja - 1, j_coord - 1] += delta
+        out = np.zeros((prob.n_basis, prob.n_basis))
+        for b in range(nb):
+            nc, npt, ncp, ib, mp = _batch(prob, b)
+            if nc == 0 or npt == 0:
+                continue
+            own = prob.basis.atom[ib] == ja - 1
+            if not own.any():
+                continue
+            pts = prob.coord_points[:, mp].T
+            phi = prob.wave[prob.wave_offset[b]:prob.wave_offset[b + 1]].reshape((ncp, npt), order="F")[:nc]
+            g, _ = _fd_derivs(prob, ib[own], pts, xyz)
+            pref5 = 2 * vs[mp] * w[mp] * g0[:, mp]
+            D = np.zeros((nc, npt))
+            D[own] = np.einsum("jp,ijp->ip", pref5, g)
+            out[np.ix_(ib, ib)] += D @ phi.T + phi @ D.T
+        return out
+
+    h = 1e-4
+    fd = (M(h) - M(-h)) / (2 * h)
+    err = np.abs(H[:, :, 0] - fd).max()
+    print(f"[h1 hessian term, j_coord {j_coord}] max|err| {err:.3e}, max|H1| {np.abs(fd).max():.3e}")
+    assert np.abs(fd).max() > 0.01
+    assert err <= 1e-6 * np.abs(fd).max()

@@ -2,8 +2,9 @@
 
 Checks that do not retype the oracle's loops: a brute-force sum_ij phi_i P_ij phi_j per point,
 closed forms for rank-one and unit P1, linearity, the scatter map, the zero/bias/gradient semantics
-against an einsum restatement on tiny inputs, NaN padding rows, and a planted transposition that
-the brute-force check must catch.
+against an einsum restatement on tiny inputs, NaN padding rows, a planted transposition that
+the brute-force check must catch, and the GGA gradient as the point derivative of rho1 (central
+differences of the synthetic basis).
 """
 import numpy as np
 import pytest
@@ -130,3 +131,57 @@ def test_rho1_bias_tiles_and_empty_batches(oracle):
                 np.testing.assert_allclose(rho[mp, 0], ref, rtol=1e-12, atol=1e-12)
             gref = 2 * np.einsum("icp,ip->cp", G, W) - gbias[:, :npt, b, t]
             np.testing.assert_allclose(grad[:, :npt, b, t], gref, rtol=1e-12, atol=1e-12)
+
+
+def _fd_grad(prob, ib, pts, atom_xyz=None, h=1e-5):
+    """d phi_ib / d r_j at pts by central differences of the synthetic basis, (nc, 3, np)."""
+    xyz = prob.atom_xyz if atom_xyz is None else atom_xyz
+    out = np.empty((len(ib), 3, len(pts)))
+    for j in range(3):
+        e = np.zeros(3)
+        e[j] = h
+        out[:, j, :] = (synth.eval_basis(xyz, prob.basis, ib, pts + e) -
+                        synth.eval_basis(xyz, prob.basis, ib, pts - e)) / (2 * h)
+    return out
+
+
+def test_gradient_is_the_point_derivative_of_rho1(oracle):
+    """The oracle's GGA output grad rho1(j, p) = 2 sum_i dphi(i, j, p) W(i, p) - gbias
+    (evaluate_first_order_rho.cpp:177-200) is the derivative of rho1(r) = phi(r)^T P1 phi(r) in the point
+    coordinate r_j for a symmetric P1, when gradient_wave holds the basis functions' true derivatives
+    (here central differences of the synthetic basis) in its (nc_pad, 3, np) layout: compared with
+    central differences of the brute-force rho1 at displaced points.  A transposed gradient layout, a
+    dropped factor 2, a wrong point or row index fails it."""
+    prob = synth.make_rho1_problem(4, 8.0, 12, 20, None, config_k=21)
+    P1 = synth.make_p1(prob.atom_xyz, prob.basis, 41)
+    P = P1[:, :, 0]
+    wl = int(prob.wave_offset[-1])
+    gw = np.full(3 * wl, np.nan)
+    for b in range(prob.n_batches):
+        nc, npt = int(prob.n_compute_c_batches[b]), int(prob.n_point_batches[b])
+        if nc == 0 or npt == 0:
+            continue
+        ib, _, mp = _batch_phi(prob, b)
+        ncp = (nc + 3) // 4 * 4
+        blk = gw[3 * prob.wave_offset[b]:3 * prob.wave_offset[b + 1]].reshape((ncp, 3, npt), order="F")
+        blk[:nc] = _fd_grad(prob, ib, prob.coord_points[:, mp].T)
+    _, grad = oracle.rho1(prob, P1, gwave=gw)
+    h = 1e-5
+    worst, scale = 0.0, 0.0
+    for b in range(prob.n_batches):
+        if prob.n_compute_c_batches[b] == 0:
+            continue
+        ib, _, mp = _batch_phi(prob, b)
+        pts = prob.coord_points[:, mp].T
+        Pl = P[np.ix_(ib, ib)]
+        for j in range(3):
+            e = np.zeros(3)
+            e[j] = h
+            rp = synth.eval_basis(prob.atom_xyz, prob.basis, ib, pts + e)
+            rm = synth.eval_basis(prob.atom_xyz, prob.basis, ib, pts - e)
+            fd = (np.einsum("ip,ij,jp->p", rp, Pl, rp) - np.einsum("ip,ij,jp->p", rm, Pl, rm)) / (2 * h)
+            got = grad[j, :len(mp), b, 0]
+            worst = max(worst, float(np.abs(got - fd).max()))
+            scale = max(scale, float(np.abs(fd).max()))
+    assert scale > 0.1
+    assert worst <= 1e-7 * scale, (worst, scale)

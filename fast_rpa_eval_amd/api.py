@@ -164,6 +164,17 @@ class Rho1Plan:
             _dp(gradient_wave), _dp(density_matrix), _dp(out), _stream(stream)), "frpa_rho1_bias_part2")
         return out
 
+    def bias_inline(self, wave, gradient_wave, density_matrix, atom_valid_n_compute_c_batches, n_atoms,
+                    j_atom_begin, bias, n_tile=1, j_coord=1, T=None, stream=None):
+        """The v6 variant's inline bias -sum_{i in V} grad phi_i sum_k DM_ik phi_k
+        (version6_rho_with_bias_calculate/evaluate_first_order_rho.cpp:179-231), ADDED in place to the
+        device tensor `bias` (n_max_batch_size * n_batches * T); returns bias."""
+        T = n_tile if T is None else T
+        check(lib().frpa_rho1_bias_inline(
+            self.handle, T, j_atom_begin, n_tile, j_coord, n_atoms, _dp(atom_valid_n_compute_c_batches), _dp(wave),
+            _dp(gradient_wave), _dp(density_matrix), _dp(bias), _stream(stream)), "frpa_rho1_bias_inline")
+        return bias
+
     def h1(self, wave, gradient_wave, hessian_wave, xc, atom_valid_n_compute_c_batches, n_atoms, j_atom_begin,
            n_tile=1, j_coord=1, T=None, H=None, stream=None):
         """Full first-order H (LDA + GGA terms of evaluate_first_order_H), accumulated into H

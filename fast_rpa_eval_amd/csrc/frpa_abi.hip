@@ -669,6 +669,18 @@ int frpa_rho1_bias_part2(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_be
   });
 }
 
+int frpa_rho1_bias_inline(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord,
+                          int n_atoms, const int *atom_valid_n_compute_c_batches, const double *wave,
+                          const double *gradient_wave, const double *density_matrix, double *first_order_rho_bias,
+                          void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
+    rho1_bias_part2(*reinterpret_cast<Rho1Plan *>(plan), atom_tile_size, j_atom_begin, n_tile, j_coord, n_atoms,
+                    atom_valid_n_compute_c_batches, wave, gradient_wave, density_matrix, first_order_rho_bias,
+                    as_stream(stream), 1, true);
+  });
+}
+
 int frpa_rho1_h1(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord, int n_atoms,
                  const int *atom_valid_n_compute_c_batches, const double *wave, const double *gradient_wave,
                  const double *hessian_wave, const double *vsigma_batches, const double *gradient_rho,

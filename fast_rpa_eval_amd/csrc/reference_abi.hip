@@ -38,6 +38,8 @@ struct RefState {
   std::unique_ptr<Rho1Plan> rho1;
   std::unique_ptr<FarPlan> far;
   DevBuf<double> wave, gwave, P1, bias, gbias, rho, grad, dm;
+  DevBuf<double> dm0;       // density_matrix of init_dfpt_device_data (device_data.cu:328)
+  DevBuf<double> bias_inl;  // bias input plus the v6 inline bias
   // per-call arrays of the first-order H entry point
   DevBuf<double> h_hw, h_vs, h_v2rr, h_v2rs, h_v2ss, h_grho, h_pref1, h_grho1, h_vhg, h_rho1, h_w, h_H;
   DevBuf<int> avn;  // atom_valid_n_compute_c_batches (n_atoms + 1, n_batches)
@@ -60,7 +62,8 @@ struct CpuSymState {
   uint64_t rho_key = 0, far_key = 0;
   std::unique_ptr<Rho1Plan> rho1;
   std::unique_ptr<FarPlan> far;
-  DevBuf<double> wave, gwave, P1, bias, gbias, rho, grad;
+  DevBuf<double> wave, gwave, P1, bias, gbias, rho, grad, dm;
+  DevBuf<int> avn;
   DevBuf<double> f_cp, f_mrsq, f_opr, f_M, f_dv, f_spl;
   DevBuf<int> f_map, f_lmfd;
 };
@@ -97,6 +100,16 @@ int rho1_precision_from_env() {
 bool rho1_gradient_from_env() {
   const char *g = getenv("FRPA_RHO1_GRADIENT");
   return !(g && strcmp(g, "0") == 0);
+}
+
+// FRPA_RHO1_INLINE_BIAS=1: the rho1 entry points compute as the v6 variant of the reference's rho1
+// function (task4_common_calculation_eliminated/version6_rho_with_bias_calculate/
+// evaluate_first_order_rho.cpp:179-231, .cu.hpp:202-250), which adds to rho1 the inline bias
+// -sum_{i in V} grad_{j_coord} phi_i sum_k DM_ik phi_k on top of the bias input, with the
+// density_matrix of init_dfpt_device_data (GPU entry point) or of the call (CPU function's symbol).
+bool rho1_inline_bias_from_env() {
+  const char *v = getenv("FRPA_RHO1_INLINE_BIAS");
+  return v && strcmp(v, "1") == 0;
 }
 
 // The reference's compile-time ATOM_TILE_SIZE (setting.h:19-20, default 1).
@@ -161,6 +174,20 @@ void rho1_run(Rho1Plan &plan, int T, int n_tile, const double *wave, const doubl
   FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
   if (!want_grad)
     fill_f64(grad, (size_t)3 * plan.nmb * plan.n_batches * T, 0.0, s);
+}
+
+// The v6 inline bias: bias_out = bias_in + inline bias of DM for the tile's atoms (device arrays);
+// returns bias_out, the bias input of the forward.
+const double *rho1_inline_bias(Rho1Plan &plan, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms,
+                               const int *avn, const double *wave, const double *gwave, const double *DM,
+                               const double *bias_in, DevBuf<double> &bias_out, hipStream_t s) {
+  const size_t nbt = (size_t)plan.nmb * plan.n_batches * T;
+  bias_out.grow(nbt);
+  FRPA_HIP(hipMemcpyAsync(bias_out.ptr, bias_in, nbt * 8, hipMemcpyDeviceToDevice, s));
+  const int rc = frpa_rho1_bias_inline(reinterpret_cast<frpa_rho1_plan *>(&plan), T, j_atom_begin, n_tile, j_coord,
+                                       n_atoms, avn, wave, gwave, DM, bias_out.ptr, s);
+  FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
+  return bias_out.ptr;
 }
 
 // Per-call far/near arrays of one Hartree-sum call in device buffers.
@@ -308,7 +335,10 @@ void init_device_data(
     S.gbias.alloc((size_t)3 * S.nmb * nb * S.T);
     S.rho.alloc((size_t)S.n_full * S.T);
     S.grad.alloc((size_t)3 * S.nmb * nb * S.T);
-    // as the reference's init (device_data.cu:185-187), the per-call arrays are allocated, not copied
+    // as the reference's init (device_data.cu:185-187), the per-call arrays are allocated, not copied;
+    // the density matrix is copied (device_data.cu:328), for the v6 inline bias
+    if (density_matrix_ptr && S.n_basis > 0)
+      S.dm0.upload(density_matrix_ptr, (size_t)S.n_basis * S.n_basis);
     S.coord_points.upload(coord_points_ptr, (size_t)3 * S.n_full);
     if (*n_centers_hartree_potential_ > 0 && *n_atoms_ > 0)
       S.far.reset(make_far_plan(*l_pot_max_, *l_max_analytic_multipole_, *index_cc_dim_0_, *n_centers_,
@@ -323,7 +353,8 @@ void free_device_data() {
   RefState &S = ref();
   S.rho1.reset();
   S.far.reset();
-  for (DevBuf<double> *b : {&S.f_spl, &S.wave, &S.gwave, &S.P1, &S.bias, &S.gbias, &S.dm, &S.h_hw, &S.h_vs,
+  for (DevBuf<double> *b : {&S.f_spl, &S.wave, &S.gwave, &S.P1, &S.bias, &S.gbias, &S.dm, &S.dm0, &S.bias_inl,
+                            &S.h_hw, &S.h_vs,
                             &S.h_v2rr, &S.h_v2rs, &S.h_v2ss, &S.h_grho, &S.h_pref1, &S.h_grho1, &S.h_vhg, &S.h_rho1,
                             &S.h_w, &S.h_H, &S.rho, &S.grad, &S.coord_points, &S.f_cp, &S.f_mrsq, &S.f_opr, &S.f_M,
                             &S.f_dv})
@@ -421,6 +452,7 @@ double *frpa_ref_device_array(const char *name) {
              {"first_order_gradient_rho", &S.grad},
              {"wave_batches_compress", &S.wave},
              {"gradient_wave_batches_compress", &S.gwave},
+             {"density_matrix", &S.dm0},
              {"coord_points", &S.coord_points}};
   for (const auto &e : tab)
     if (strcmp(name, e.n) == 0)
@@ -457,8 +489,13 @@ void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
       FRPA_HIP(hipMemcpyAsync(S.gbias.ptr, first_order_gradient_rho_bias_batches_atoms_ptr, 3 * nbt * 8,
                               hipMemcpyHostToDevice, s));
     }
-    rho1_run(*S.rho1, S.T, n_tile, S.wave.ptr, S.gwave.ptr, S.P1.ptr, S.bias.ptr, S.gbias.ptr, S.rho.ptr, S.grad.ptr,
-             s);
+    const double *bias = S.bias.ptr;
+    if (rho1_inline_bias_from_env()) {
+      FRPA_REQUIRE(S.dm0.ptr, FRPA_ERR_STATE, "FRPA_RHO1_INLINE_BIAS=1 needs the density_matrix of init_dfpt_device_data");
+      bias = rho1_inline_bias(*S.rho1, S.T, *j_atom_begin_, n_tile, *j_coord_, S.n_atoms, S.avn.ptr, S.wave.ptr,
+                              S.gwave.ptr, S.dm0.ptr, S.bias.ptr, S.bias_inl, s);
+    }
+    rho1_run(*S.rho1, S.T, n_tile, S.wave.ptr, S.gwave.ptr, S.P1.ptr, bias, S.gbias.ptr, S.rho.ptr, S.grad.ptr, s);
     if (!S.resident) {  // evaluate_first_order_rho.cu:363-367; resident: results stay, no synchronisation
       FRPA_HIP(hipMemcpyAsync(first_order_rho_ptr, S.rho.ptr, S.rho.n * 8, hipMemcpyDeviceToHost, s));
       FRPA_HIP(hipMemcpyAsync(first_order_gradient_rho_ptr, S.grad.ptr, 3 * nbt * 8, hipMemcpyDeviceToHost, s));
@@ -523,6 +560,21 @@ void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_(
     FRPA_HIP(hipMemcpyAsync(C.bias.ptr, first_order_rho_bias_part2_ptr, nbt * 8, hipMemcpyHostToDevice, s));
     FRPA_HIP(hipMemcpyAsync(C.gbias.ptr, first_order_gradient_rho_bias_batches_atoms_ptr, 3 * nbt * 8,
                             hipMemcpyHostToDevice, s));
+    if (rho1_inline_bias_from_env()) {  // version6_rho_with_bias_calculate/evaluate_first_order_rho.cpp:179-231
+      const int n_atoms = *n_atoms_;
+      FRPA_REQUIRE(density_matrix_ptr && atom_valid_n_compute_c_batches_ptr && n_atoms >= 1, FRPA_ERR_INVALID,
+                   "FRPA_RHO1_INLINE_BIAS=1 needs density_matrix and atom_valid_n_compute_c_batches");
+      C.dm.grow((size_t)n_basis * n_basis);
+      C.avn.grow((size_t)(n_atoms + 1) * nb);
+      FRPA_HIP(hipMemcpyAsync(C.dm.ptr, density_matrix_ptr, (size_t)n_basis * n_basis * 8, hipMemcpyHostToDevice, s));
+      FRPA_HIP(hipMemcpyAsync(C.avn.ptr, atom_valid_n_compute_c_batches_ptr, (size_t)(n_atoms + 1) * nb * 4,
+                              hipMemcpyHostToDevice, s));
+      // accumulated into C.bias, the device copy of this call's bias input
+      const int rc = frpa_rho1_bias_inline(reinterpret_cast<frpa_rho1_plan *>(C.rho1.get()), T, *j_atom_begin_,
+                                           n_tile, *j_coord_, n_atoms, C.avn.ptr, C.wave.ptr, C.gwave.ptr, C.dm.ptr,
+                                           C.bias.ptr, s);
+      FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
+    }
     rho1_run(*C.rho1, T, n_tile, C.wave.ptr, C.gwave.ptr, C.P1.ptr, C.bias.ptr, C.gbias.ptr, C.rho.ptr, C.grad.ptr, s);
     FRPA_HIP(hipMemcpyAsync(first_order_rho_ptr, C.rho.ptr, (size_t)n_full * T * 8, hipMemcpyDeviceToHost, s));
     FRPA_HIP(hipMemcpyAsync(first_order_gradient_rho_ptr, C.grad.ptr, 3 * nbt * 8, hipMemcpyDeviceToHost, s));

@@ -27,6 +27,12 @@
 // instead of two gathers through the n_basis^2 matrix (whose transposed half touches one line per
 // lane).  When the tile has more rows than the table holds, the kernel gathers from DM directly;
 // both give the same sums bit for bit.
+//
+// The same kernels compute the inline bias of the v6 variant of the rho1 function
+// (task4_common_calculation_eliminated/version6_rho_with_bias_calculate/evaluate_first_order_rho.cpp:
+// 179-231): acc_bias(p, t) = sum_{k in V_t} phi'(k, p) sum_i DM(k, i) phi(i, p), the one-sided matrix
+// (sides = 1: A(k, i) = DM(k, i) only), ADDED to the bias array it is given (accumulate), so that a
+// copy of the bias input becomes the input of the forward.
 #include <algorithm>
 #include <cstdlib>
 
@@ -101,11 +107,12 @@ __global__ __launch_bounds__(1024) void b2_scan_kernel(int *__restrict__ rowmap,
     rows[0] = part[1023];
 }
 
-// tab[c * cap + l] = DM(rows_l, c) + DM(c, rows_l) for l < R, every column c; 64 x 64 tiles, the
-// DM(c, rows_l) half read along c and transposed through LDS.  Nothing is written when R > cap.
+// tab[c * cap + l] = DM(rows_l, c) + DM(c, rows_l) (sides = 2) or DM(rows_l, c) (sides = 1) for l < R,
+// every column c; 64 x 64 tiles, the DM(c, rows_l) half read along c and transposed through LDS.
+// Nothing is written when R > cap.
 __global__ __launch_bounds__(256) void b2_table_kernel(const double *__restrict__ DM, int n_basis,
                                                       const int *__restrict__ rows, double *__restrict__ tab,
-                                                      int cap) {
+                                                      int cap, int sides) {
   __shared__ double sT[64][65];
   const int R = rows[0];
   const int c0 = blockIdx.x * 64, l0 = blockIdx.y * 64;
@@ -114,17 +121,19 @@ __global__ __launch_bounds__(256) void b2_table_kernel(const double *__restrict_
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   for (int li = ty; li < 64; li += 4) {
     const int l = l0 + li, c = c0 + tx;
-    sT[li][tx] = (l < R && c < n_basis) ? DM[c + (size_t)n_basis * rows[1 + l]] : 0.0;
+    sT[li][tx] = (sides == 2 && l < R && c < n_basis) ? DM[c + (size_t)n_basis * rows[1 + l]] : 0.0;
   }
   __syncthreads();
   for (int ci = ty; ci < 64; ci += 4) {
     const int c = c0 + ci, l = l0 + tx;
     if (l < R && c < n_basis)
-      tab[(size_t)c * cap + l] = DM[rows[1 + l] + (size_t)n_basis * c] + sT[tx][ci];
+      tab[(size_t)c * cap + l] = sides == 2 ? DM[rows[1 + l] + (size_t)n_basis * c] + sT[tx][ci]
+                                            : DM[rows[1 + l] + (size_t)n_basis * c];
   }
 }
 
 // QUADS: every wave block starts on a 4-double boundary, so a lane's four phi values are one load.
+// sides: 2 for DM + DM^T, 1 for DM; accumulate: add the sums to bias instead of storing them.
 template <bool QUADS>
 __global__ __launch_bounds__(256, 2) void bias2_kernel(const BatchDesc *__restrict__ descs,
                                                    const int *__restrict__ ibasis0, const int *__restrict__ avn,
@@ -132,8 +141,9 @@ __global__ __launch_bounds__(256, 2) void bias2_kernel(const BatchDesc *__restri
                                                    const double *__restrict__ wave, const double *__restrict__ gwave,
                                                    const double *__restrict__ DM, int n_basis,
                                                    const double *__restrict__ tab, const int *__restrict__ rowmap,
-                                                   const int *__restrict__ rows, int cap,
-                                                   double *__restrict__ bias, int nmb, int n_batches_all) {
+                                                   const int *__restrict__ rows, int cap, int sides,
+                                                   int accumulate, double *__restrict__ bias, int nmb,
+                                                   int n_batches_all) {
   __shared__ double red[B2_TMAX][B2_PASS];
   __shared__ int s_vs[B2_TMAX], s_off[B2_TMAX + 1];
   const BatchDesc d = descs[blockIdx.x];
@@ -235,7 +245,8 @@ __global__ __launch_bounds__(256, 2) void bias2_kernel(const BatchDesc *__restri
             for (int s = 0; s < 4; s++) {
               const long long jb = jj[s];
               an[rt][s] = (rt < nrt && arow[rt] >= 0 && jb >= 0)
-                              ? DM[arow[rt] + (size_t)n_basis * jb] + DM[jb + (size_t)n_basis * arow[rt]]
+                              ? (sides == 2 ? DM[arow[rt] + (size_t)n_basis * jb] + DM[jb + (size_t)n_basis * arow[rt]]
+                                            : DM[arow[rt] + (size_t)n_basis * jb])
                               : 0.0;
             }
         }
@@ -342,7 +353,8 @@ __global__ __launch_bounds__(256, 2) void bias2_kernel(const BatchDesc *__restri
     __syncthreads();
     for (int e = tid; e < n_tile * npp; e += 256) {
       const int t = e / npp, p = e % npp;
-      bias[(size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t)] = red[t][p];
+      double *o = bias + (size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
+      *o = accumulate ? *o + red[t][p] : red[t][p];
     }
     __syncthreads();
   }
@@ -351,7 +363,9 @@ __global__ __launch_bounds__(256, 2) void bias2_kernel(const BatchDesc *__restri
 } // namespace
 
 void rho1_bias_part2(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,
-                     const double *wave, const double *gwave, const double *DM, double *bias, hipStream_t s) {
+                     const double *wave, const double *gwave, const double *DM, double *bias, hipStream_t s,
+                     int sides, bool accumulate) {
+  FRPA_REQUIRE(sides == 1 || sides == 2, FRPA_ERR_INVALID, "bias: sides %d is not 1 or 2", sides);
   FRPA_REQUIRE(n_tile >= 1 && n_tile <= T && n_tile <= B2_TMAX, FRPA_ERR_INVALID,
                "bias part 2: n_tile %d outside [1, min(T = %d, %d)]", n_tile, T, B2_TMAX);
   FRPA_REQUIRE(j_coord >= 1 && j_coord <= 3, FRPA_ERR_INVALID, "bias part 2: j_coord %d outside [1, 3]", j_coord);
@@ -383,7 +397,7 @@ void rho1_bias_part2(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coo
     hipLaunchKernelGGL(b2_scan_kernel, dim3(1), dim3(1024), 0, s, P.d_b2_rowmap.ptr, P.d_b2_rows.ptr, P.n_basis);
     check_launch("b2_scan_kernel");
     hipLaunchKernelGGL(b2_table_kernel, dim3((P.n_basis + 63) / 64, cap / 64), dim3(256), 0, s, DM, P.n_basis,
-                       P.d_b2_rows.ptr, P.d_b2_tab.ptr, cap);
+                       P.d_b2_rows.ptr, P.d_b2_tab.ptr, cap, sides);
     check_launch("b2_table_kernel");
     tab = P.d_b2_tab.ptr;
   }
@@ -391,7 +405,7 @@ void rho1_bias_part2(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coo
   auto kern = quads ? bias2_kernel<true> : bias2_kernel<false>;
   hipLaunchKernelGGL(kern, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, P.d_ibasis0.ptr, avn, n_atoms,
                      j_atom_begin, n_tile, j_coord, wave, gwave, DM, P.n_basis, tab, P.d_b2_rowmap.ptr,
-                     P.d_b2_rows.ptr, cap, bias, P.nmb, P.n_batches);
+                     P.d_b2_rows.ptr, cap, sides, accumulate ? 1 : 0, bias, P.nmb, P.n_batches);
   check_launch("bias2_kernel");
 }
 

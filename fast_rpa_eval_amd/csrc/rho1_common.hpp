@@ -149,9 +149,11 @@ void synth_wave(const Rho1Plan &P, int n_atoms, const double *atom_xyz, const in
                 hipStream_t s);
 
 // first_order_rho_bias_part2 (nmb, n_batches, T) for the atoms j_atom_begin .. + n_tile - 1 (rho1_bias2.hip);
-// avn = atom_valid_n_compute_c_batches (n_atoms + 1, n_batches) on the device.
+// avn = atom_valid_n_compute_c_batches (n_atoms + 1, n_batches) on the device.  sides = 1 with
+// accumulate: the v6 inline bias (DM one-sided), added to bias.
 void rho1_bias_part2(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,
-                     const double *wave, const double *gwave, const double *DM, double *bias, hipStream_t s);
+                     const double *wave, const double *gwave, const double *DM, double *bias, hipStream_t s,
+                     int sides = 2, bool accumulate = false);
 
 // Full first-order H (LDA + GGA terms of evaluate_first_order_H.cpp:13-222) for the perturbations
 // t < n_tile (atoms j_atom_begin + t), accumulated into H (n_basis, n_basis, T) (rho1_adjoint.hip).

@@ -199,6 +199,22 @@ int frpa_rho1_bias_part2(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_be
                          const double *gradient_wave, const double *density_matrix,
                          double *first_order_rho_bias_part2, void *stream);
 
+/*
+ * The inline bias of the v6 variant of the rho1 function
+ * (task4_common_calculation_eliminated/version6_rho_with_bias_calculate/evaluate_first_order_rho.cpp:
+ * 179-231, its second dgemm DM(V, :) phi and the acc_bias loop), ADDED to first_order_rho_bias:
+ *   first_order_rho_bias(p, b, t) += -sum_{i in V} grad_{j_coord} phi(i, p) sum_k DM(i, k) phi(k, p)
+ * with V, DM(i, k) and the arrays as for frpa_rho1_bias_part2 (one side of the density matrix where
+ * part 2 has both: inline(DM) + inline(DM^T) = part 2 of DM).  v6 adds this term to rho1 on top of
+ * its bias input; passing a copy of the bias input through this call and then to frpa_rho1_forward
+ * gives v6's rho1.  Entries of batches with n_compute_c = 0, of points beyond a batch's n_points and
+ * of perturbations t >= n_tile are unchanged.  Same limits and table as frpa_rho1_bias_part2.
+ */
+int frpa_rho1_bias_inline(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord,
+                          int n_atoms, const int *atom_valid_n_compute_c_batches, const double *wave,
+                          const double *gradient_wave, const double *density_matrix, double *first_order_rho_bias,
+                          void *stream);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Far-field Hartree multipole sum                                                            */
 /* ------------------------------------------------------------------------------------------ */
@@ -358,7 +374,8 @@ void free_dfpt_device_data(void);
  * synchronising; its host pointer arguments are not used.  frpa_ref_device_array returns the device
  * address of the array of that name (first_order_density_matrix, first_order_rho_bias_part2,
  * first_order_gradient_rho_bias_batches_atoms, first_order_rho, first_order_gradient_rho,
- * wave_batches_compress, gradient_wave_batches_compress, coord_points; the layouts and sizes of the
+ * wave_batches_compress, gradient_wave_batches_compress, coord_points, density_matrix (the copy made
+ * by init_dfpt_device_data, device_data.cu:328, used by the inline bias); the layouts and sizes of the
  * reference's arguments with ATOM_TILE_SIZE = FRPA_ATOM_TILE_SIZE), NULL before
  * init_dfpt_device_data or for another name; frpa_ref_stream the stream the entry points run on
  * (the reference's devInfo.stream).  Off by default (the environment variable FRPA_ALL_KERNELS_ON_GPU=1
@@ -375,7 +392,11 @@ void *frpa_ref_stream(void);
  * Precision from the environment variable FRPA_RHO1_PRECISION: ozaki (the default: INT8 split with 5
  * digits, ~1e-12 of max|rho1| and of max|grad rho1|), ozaki4 (4 digits) or fp64 (FP64 MFMA);
  * first_order_gradient_rho is computed (GGA) unless FRPA_RHO1_GRADIENT=0 (then left zero, and the INT8
- * split multiplies only the symmetric half of P1). */
+ * split multiplies only the symmetric half of P1).  FRPA_RHO1_INLINE_BIAS=1 selects the v6 variant
+ * (task4_common_calculation_eliminated/version6_rho_with_bias_calculate/evaluate_first_order_rho.cpp:
+ * 179-231): rho1 also gets the inline bias of frpa_rho1_bias_inline for atoms j_atom_begin..j_atom_end
+ * and coordinate j_coord, from the density_matrix of init_dfpt_device_data; the bias input arrays are
+ * not modified. */
 void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
     int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
     int *n_basis_, int *n_atoms_, int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_,
@@ -393,7 +414,9 @@ void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_(
  * uploads the wave arrays, first_order_density_matrix and the bias arrays on every call (the CPU
  * function reads them afresh); first_order_rho is overwritten (zeroed, then the tile's
  * perturbations written) and first_order_gradient_rho computed, as evaluate_first_order_rho.cpp:
- * 113-209, with the precision switches of the _cu_host_ function. */
+ * 113-209, with the precision switches of the _cu_host_ function.  With FRPA_RHO1_INLINE_BIAS=1 it
+ * computes the v6 variant (see the _cu_host_ function) from this call's density_matrix and
+ * atom_valid_n_compute_c_batches, which are then uploaded too. */
 void evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_(
     int *n_my_batches_work_, int *j_atom_begin_, int *j_atom_end_, int *j_coord_, int *n_full_points_,
     int *n_basis_, int *n_atoms_, int *n_max_compute_ham_, int *n_centers_basis_I_, int *n_max_batch_size_,

@@ -71,3 +71,50 @@ void oracle_generate_first_order_rho_bias_part2(
     free(dm);
   }
 }
+
+/*
+ * Restatement of the inline bias of the v6 variant of evaluate_first_order_rho
+ * (task4_common_calculation_eliminated/version6_rho_with_bias_calculate/evaluate_first_order_rho.cpp:
+ * 179-231; its GPU copy evaluate_first_order_rho.cu.hpp:202-250): for every batch with n_compute_c > 0
+ * and every atom j of the tile (t = j - j_atom_begin), with V the compute rows of atom j,
+ *   work(i, p) = sum_k DM(i, k) phi(k, p), i in V                              (second dgemm, :189-201)
+ *   acc_bias(p, t) = sum_{i in V} -grad_{j_coord} phi(i, p) work(i, p)            (:205-219)
+ * which v6 adds to rho1 on top of first_order_rho_bias_part2 (:222).  Here it is ADDED into
+ * first_order_rho_bias (n_max_batch_size, n_batches, atom_tile_size) for the tile's perturbations;
+ * other entries are left as they are.  It is the second of the two terms of bias part 2 above, so
+ * inline(DM) + inline(DM^T) = part2(DM).
+ */
+void oracle_rho_bias_inline(int atom_tile_size, int n_my_batches_work, int j_atom_begin, int j_atom_end, int j_coord,
+                            int n_basis, int n_atoms, int n_centers_basis_I, int n_max_batch_size,
+                            const int *n_point_batches, const int *n_compute_c_batches, const int *i_basis_batches,
+                            const int *atom_valid_n_compute_c_batches, const int *i_batch_2_wave_offset,
+                            const double *wave_batches_compress, const double *gradient_wave_batches_compress,
+                            double *first_order_rho_bias, const double *density_matrix) {
+  (void)atom_tile_size;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int b = 0; b < n_my_batches_work; b++) {
+    const int np = n_point_batches[b], nc = n_compute_c_batches[b];
+    if (nc <= 0)
+      continue;
+    const int ld = (nc + 3) / 4 * 4;
+    const double *wave = wave_batches_compress + i_batch_2_wave_offset[b];
+    const double *gwave = gradient_wave_batches_compress + 3 * (size_t)i_batch_2_wave_offset[b];
+    const int *ib = i_basis_batches + (size_t)n_centers_basis_I * b;
+    for (int j = j_atom_begin; j <= j_atom_end; j++) {
+      const int t = j - j_atom_begin;
+      const int vs = atom_valid_n_compute_c_batches[(j - 1) + (size_t)(n_atoms + 1) * b];
+      const int ve = atom_valid_n_compute_c_batches[j + (size_t)(n_atoms + 1) * b];
+      double *out = first_order_rho_bias + (size_t)n_max_batch_size * (b + (size_t)n_my_batches_work * t);
+      for (int p = 0; p < np; p++) {
+        double acc = 0.0;
+        for (int i = vs; i < ve; i++) {
+          double w = 0.0;
+          for (int k = 0; k < nc; k++)
+            w += density_matrix[(ib[i] - 1) + (size_t)n_basis * (ib[k] - 1)] * wave[k + (size_t)ld * p];
+          acc += -gwave[i + (size_t)ld * (j_coord - 1) + (size_t)3 * ld * p] * w;
+        }
+        out[p] += acc;
+      }
+    }
+  }
+}

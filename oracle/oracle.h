@@ -25,6 +25,9 @@
  *                                    central differences of rho = phi^T DM phi under a displacement
  *                                    of atom j (its basis functions move with it), empty-V zeroing,
  *                                    n_compute_c = 0 batches untouched.
+ *   oracle_rho_bias_inline           central differences of sum_ik phi_i DM_ik phi_k with only the
+ *                                    first factor moving with atom j; inline(DM) + inline(DM^T)
+ *                                    equals bias part 2 of DM.
  *   oracle_evaluate_first_order_h    adjoint identity <v, rho1(P)> = <H1(v), P>, exact
  *                                    d rho1 / d P_ij = phi_i phi_j, GGA terms against an independent
  *                                    numpy restatement on tiny inputs.
@@ -99,6 +102,15 @@ void oracle_generate_first_order_rho_bias_part2(
     const int *i_batch_2_wave_offset, const double *wave_batches_compress,
     const double *gradient_wave_batches_compress, double *first_order_rho_bias_part2,
     const double *density_matrix);
+
+/* task4_common_calculation_eliminated/version6_rho_with_bias_calculate/evaluate_first_order_rho.cpp:179-231
+ * (the inline bias of the v6 variant); ADDS into first_order_rho_bias */
+void oracle_rho_bias_inline(int atom_tile_size, int n_my_batches_work, int j_atom_begin, int j_atom_end,
+                            int j_coord, int n_basis, int n_atoms, int n_centers_basis_I, int n_max_batch_size,
+                            const int *n_point_batches, const int *n_compute_c_batches, const int *i_basis_batches,
+                            const int *atom_valid_n_compute_c_batches, const int *i_batch_2_wave_offset,
+                            const double *wave_batches_compress, const double *gradient_wave_batches_compress,
+                            double *first_order_rho_bias, const double *density_matrix);
 
 /* Number of OpenMP threads used by the two grid loops (rho1 over batches, far field over points);
  * 1 reproduces the reference's single-threaded build.  Returns the previous value. */

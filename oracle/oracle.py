@@ -33,6 +33,7 @@ def lib():
         L.oracle_evaluate_first_order_h.argtypes = [i] * 11 + [p] * 22
         L.oracle_sum_up_near.argtypes = [i] * 9 + [p] * 15
         L.oracle_generate_first_order_rho_bias_part2.argtypes = [i] * 9 + [p] * 9
+        L.oracle_rho_bias_inline.argtypes = [i] * 9 + [p] * 9
         d = ctypes.c_double
         L.oracle_ylm.argtypes = [i, d, d, d, d, p]
         L.oracle_set_threads.argtypes = [i]
@@ -156,6 +157,22 @@ def bias2(prob, density_matrix, gwave, T, j_atom_begin, j_atom_end=None, j_coord
     nb, nmb = prob.n_batches, prob.n_max_batch_size
     out = np.zeros((nmb, nb, T), order="F") if bias is None else np.asfortranarray(bias).copy(order="F")
     lib().oracle_generate_first_order_rho_bias_part2(
+        T, nb, j_atom_begin, j_atom_end, j_coord, prob.n_basis, prob.n_atoms, prob.n_centers_basis_I, nmb,
+        _p(prob.n_point_batches, np.int32), _p(prob.n_compute_c_batches, np.int32),
+        _p(prob.dense_i_basis_batches(), np.int32), _p(prob.atom_valid_n_compute_c_batches(), np.int32),
+        _p(prob.wave_offset_i32(), np.int32), _p(prob.wave, np.float64), _p(gwave, np.float64), _p(out, np.float64),
+        _p(np.asfortranarray(density_matrix, dtype=np.float64), np.float64))
+    return out
+
+
+def bias_inline(prob, density_matrix, gwave, T, j_atom_begin, j_atom_end=None, j_coord=1, bias=None):
+    """The v6 variant's inline bias on the CPU, ADDED to `bias` (zeros by default) for the atoms
+    j_atom_begin .. j_atom_end; returns (n_max_batch_size, n_batches, T) F."""
+    if j_atom_end is None:
+        j_atom_end = j_atom_begin + T - 1
+    nb, nmb = prob.n_batches, prob.n_max_batch_size
+    out = np.zeros((nmb, nb, T), order="F") if bias is None else np.asfortranarray(bias).copy(order="F")
+    lib().oracle_rho_bias_inline(
         T, nb, j_atom_begin, j_atom_end, j_coord, prob.n_basis, prob.n_atoms, prob.n_centers_basis_I, nmb,
         _p(prob.n_point_batches, np.int32), _p(prob.n_compute_c_batches, np.int32),
         _p(prob.dense_i_basis_batches(), np.int32), _p(prob.atom_valid_n_compute_c_batches(), np.int32),

@@ -148,3 +148,117 @@ def test_bias2_reference_entry_point(cuda, oracle, monkeypatch):
     err = np.abs(bias - ref).max() / np.abs(ref).max()
     print(f"[bias2 reference entry point] rel {err:.2e}", flush=True)
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("T,n_tile,jb,jc,np_range,rows", [(8, 8, 1, 2, (20, 200), None),
+                                                          (3, 2, 5, 3, (150, 600), None),
+                                                          (4, 3, 9, 1, (20, 200), "0")])
+def test_bias_inline_against_oracle(cuda, oracle, monkeypatch, T, n_tile, jb, jc, np_range, rows):
+    """The v6 inline bias (one side of a nonsymmetric DM) ADDED to a random bias, within 1e-12 of
+    max|inline|; the table and the direct DM gathers (rows = 0) give the same bits; entries of other
+    perturbations, of the n_compute_c = 0 batch and past n_points are unchanged."""
+    prob, gw, A = _setup(T + jb + 100, np_range)
+    init = np.asfortranarray(np.random.default_rng(3).uniform(-1, 1, size=(prob.n_max_batch_size, prob.n_batches, T)))
+    ref = oracle.bias_inline(prob, A, gw, T, jb, j_atom_end=jb + n_tile - 1, j_coord=jc, bias=init)
+    plan = api.Rho1Plan.from_problem(prob)
+    args = (api.to_device(prob.wave), api.to_device(gw), api.to_device(np.asfortranarray(A)),
+            api.to_device(prob.atom_valid_n_compute_c_batches()), prob.n_atoms, jb)
+    got = plan.bias_inline(*args, api.to_device(init), n_tile=n_tile, j_coord=jc, T=T)
+    if rows is not None:
+        monkeypatch.setenv("FRPA_BIAS2_TABLE_ROWS", rows)
+        direct = plan.bias_inline(*args, api.to_device(init), n_tile=n_tile, j_coord=jc, T=T)
+        assert torch.equal(direct, got)
+    torch.cuda.synchronize()
+    got = got.cpu().numpy().reshape(init.shape, order="F")
+    err = np.abs(got - ref).max() / np.abs(ref - init).max()
+    print(f"[inline bias T={T} n_tile={n_tile} jb={jb} jc={jc} np_max={prob.n_point_batches.max()} rows={rows}] "
+          f"rel {err:.2e}", flush=True)
+    assert err <= 1e-12
+    assert np.array_equal(got[:, :, n_tile:], init[:, :, n_tile:])
+    assert np.array_equal(got[:, 3, :], init[:, 3, :])  # n_compute_c = 0
+    for b in range(prob.n_batches):
+        assert np.array_equal(got[prob.n_point_batches[b]:, b, :], init[prob.n_point_batches[b]:, b, :])
+
+
+def test_rho1_inline_bias_reference_entry_points(cuda, oracle, monkeypatch):
+    """FRPA_RHO1_INLINE_BIAS=1: the rho1 GPU entry point (density_matrix of init_dfpt_device_data) and
+    the CPU function's symbol (density_matrix of the call) compute the v6 variant, rho1 = P1 term +
+    inline bias + bias input, within 1e-10 at the default precision (gradient 1e-10); the host bias
+    input is not modified; without the switch the inline term is absent."""
+    monkeypatch.setenv("FRPA_ATOM_TILE_SIZE", "4")
+    for k in ("FRPA_RHO1_PRECISION", "FRPA_RHO1_GRADIENT", "FRPA_ALL_KERNELS_ON_GPU"):
+        monkeypatch.delenv(k, raising=False)
+    T, jb, je, jc = 4, 2, 4, 2
+    prob, gw, A = _setup(23)
+    DM = np.asfortranarray(A)
+    L = _lib.lib()
+    P = ctypes.c_void_p
+    keep = []
+
+    def ip(v):
+        c = ctypes.c_int(int(v))
+        keep.append(c)
+        return ctypes.byref(c)
+
+    def ptr(a, dt):
+        a = np.asfortranarray(np.asarray(a, dtype=dt))
+        keep.append(a)
+        return a.ctypes.data_as(P)
+
+    I = lambda a: ptr(a, np.int32)
+    D = lambda a: ptr(a, np.float64)
+    nb, nmb = prob.n_batches, prob.n_max_batch_size
+    rng = np.random.default_rng(29)
+    P1 = np.asfortranarray(rng.uniform(-1, 1, size=(prob.n_basis, prob.n_basis, T)))
+    bias_in = np.asfortranarray(rng.uniform(-1, 1, size=(nmb, nb, T)))
+    gbias = np.asfortranarray(rng.uniform(-1, 1, size=(3, nmb, nb, T)))
+    bias = bias_in.copy(order="F")
+    rho = np.zeros((prob.n_full_points, T), order="F")
+    grad = np.zeros((3, nmb, nb, T), order="F")
+    zi = np.zeros(max(prob.n_atoms, 8) + 8, np.int32)
+    zd = np.zeros(prob.n_full_points + 8)
+    ibd, avn = prob.dense_i_basis_batches(), prob.atom_valid_n_compute_c_batches()
+    L.init_dfpt_device_info()
+    L.init_dfpt_device_data(
+        ip(nb), ip(prob.n_full_points), ip(0), ip(prob.n_basis), ip(prob.n_atoms), ip(prob.n_max_compute_ham),
+        ip(prob.n_centers_basis_I), ip(nmb), ip(0), ip(1), ip(2), ip(0), ip(0), ip(1), ip(1), ip(1), ip(1),
+        I(prob.basis.atom + 1), I(prob.batch_sizes), I(prob.n_point_batches), I(prob.n_point_batches_prefix_sum),
+        I(prob.i_valid_point_2_i_full_points_map), I(prob.n_compute_c_batches), I(ibd), I(avn),
+        I(prob.wave_offset_i32()), D(prob.wave), D(gw), D(DM),
+        I(zi), I(zi), I(zi), D(prob.coord_points), I(zi), I(zi), I(zi), I(zi), I(zi), I(zi), I(zi), D(zd), D(zd),
+        D(zd), D(zd), D(zd), D(zd),
+        D(np.zeros((prob.n_basis, prob.n_basis, T))), D(np.zeros((prob.n_full_points, T))),
+        D(np.zeros((nmb, nb, T))), D(np.zeros((3, nmb, nb, T))), D(np.zeros((3, nmb, nb, T))),
+        I(zi), D(zd), D(zd), D(zd), D(zd), D(zd))
+
+    def call(fn):
+        fn(ip(nb), ip(jb), ip(je), ip(jc), ip(prob.n_full_points), ip(prob.n_basis), ip(prob.n_atoms),
+           ip(prob.n_max_compute_ham), ip(prob.n_centers_basis_I), ip(nmb), I(prob.batch_sizes),
+           I(prob.n_point_batches), I(prob.n_point_batches_prefix_sum), I(prob.i_valid_point_2_i_full_points_map),
+           I(prob.n_compute_c_batches), I(ibd), I(avn), I(prob.wave_offset_i32()), D(prob.wave), D(gw),
+           P1.ctypes.data_as(P), rho.ctypes.data_as(P), bias.ctypes.data_as(P), grad.ctypes.data_as(P),
+           gbias.ctypes.data_as(P), I(prob.basis.atom + 1), D(zd), D(DM))
+
+    bias_v6 = oracle.bias_inline(prob, DM, gw, T, jb, j_atom_end=je, j_coord=jc, bias=bias_in)
+    ref, gref = oracle.rho1(prob, P1, T=T, bias=bias_v6, gwave=gw, gbias=gbias, j_atom_begin=jb, j_atom_end=je)
+    ref_plain, _ = oracle.rho1(prob, P1, T=T, bias=bias_in, gwave=gw, gbias=gbias, j_atom_begin=jb, j_atom_end=je)
+    assert np.abs(ref - ref_plain).max() > 1e-2 * np.abs(ref).max()
+    try:
+        for name in ("evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_cu_host_",
+                     "evaluate_first_order_rho_reduce_memory_c_v3_batches_atoms_"):
+            for inline, r in (("1", ref), (None, ref_plain)):
+                if inline:
+                    monkeypatch.setenv("FRPA_RHO1_INLINE_BIAS", inline)
+                else:
+                    monkeypatch.delenv("FRPA_RHO1_INLINE_BIAS", raising=False)
+                rho[:] = -7.0
+                call(getattr(L, name))
+                rel = np.abs(rho - r).max() / np.abs(r).max()
+                grel = np.abs(grad - gref).max() / np.abs(gref).max()
+                print(f"[{name} inline bias {inline}] rel {rel:.3e}, gradient rel {grel:.3e}", flush=True)
+                assert rel <= 1e-10 and grel <= 1e-10
+                assert np.array_equal(bias, bias_in)
+                assert np.all(rho[:, je - jb + 1:] == 0.0)
+    finally:
+        L.free_dfpt_device_data()
+        L.free_dfpt_device_info()

@@ -86,7 +86,8 @@ def test_cxx_host_against_oracle(cuda, oracle, tmp_path):
     d = str(tmp_path)
     _write(d, arrays, scalars)
     env = dict(os.environ, FRPA_ATOM_TILE_SIZE=str(T))
-    for k in ("FRPA_RHO1_PRECISION", "FRPA_RHO1_GRADIENT", "FRPA_ALL_KERNELS_ON_GPU", "FRPA_FAR_VARIANT"):
+    for k in ("FRPA_RHO1_PRECISION", "FRPA_RHO1_GRADIENT", "FRPA_ALL_KERNELS_ON_GPU", "FRPA_FAR_VARIANT",
+              "FRPA_RHO1_INLINE_BIAS"):
         env.pop(k, None)
     print("[reference_host] running the C++ host program", flush=True)
     r = subprocess.run([BIN, d], env=env, capture_output=True, text=True, timeout=120)

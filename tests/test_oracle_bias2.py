@@ -104,3 +104,65 @@ def test_bias2_empty_atom_rows_give_zero(oracle):
                 assert np.all(got[:npt, b, t] != 0.0)
             assert np.all(got[npt:, b, t] == 3.0)
     assert n_empty > 0
+
+
+def _fd_bias_one_sided(prob, DM, j_atom, j_coord, h=1e-5):
+    """Central difference of sum_ik phi_i(R_j + h e_c) DM_ik phi_k(R) -- only the first factor moves."""
+    out = np.zeros((prob.n_max_batch_size, prob.n_batches))
+    for b in range(prob.n_batches):
+        nc, npt = int(prob.n_compute_c_batches[b]), int(prob.n_point_batches[b])
+        if nc == 0 or npt == 0:
+            continue
+        idx = prob.ibasis[prob.ibasis_ptr[b]:prob.ibasis_ptr[b + 1]] - 1
+        mp = prob.i_valid_point_2_i_full_points_map[prob.n_point_batches_prefix_sum[b]:prob.n_point_batches_prefix_sum[b + 1]]
+        pts = prob.coord_points[:, mp].T
+        phi0 = synth.eval_basis(prob.atom_xyz, prob.basis, idx, pts)
+        w = DM[np.ix_(idx, idx)] @ phi0
+        vals = []
+        for s in (1, -1):
+            xyz = prob.atom_xyz.copy()
+            xyz[j_atom - 1, j_coord - 1] += s * h
+            vals.append(np.sum(synth.eval_basis(xyz, prob.basis, idx, pts) * w, axis=0))
+        out[:npt, b] = (vals[0] - vals[1]) / (2 * h)
+    return out
+
+
+def test_inline_bias_is_the_one_sided_displacement_derivative(oracle):
+    """The v6 inline bias: d/dR_j of sum_ik phi_i DM_ik phi_k with only phi_i moving (nonsymmetric DM,
+    so the side matters); ADDED to the given bias; n_compute_c = 0 batches untouched."""
+    prob = synth.make_rho1_problem(8, 8.0, 16, 40, (20, 80), config_k=1, fill_wave=False, zero_nc=(5,))
+    wave, gwave = _true_wave_and_gradient(prob)
+    prob.wave = wave
+    DM = np.random.default_rng(12).standard_normal((prob.n_basis, prob.n_basis))
+    T, jb = 3, 2
+    for jc in (1, 3):
+        init = np.full((prob.n_max_batch_size, prob.n_batches, T), 7.0, order="F")
+        got = oracle.bias_inline(prob, DM, gwave, T, jb, j_coord=jc, bias=init) - 7.0
+        assert np.all(got[:, 5, :] == 0.0)
+        for t in range(T):
+            ref = _fd_bias_one_sided(prob, DM, jb + t, jc)
+            g = got[:, :, t].copy()
+            for b in range(prob.n_batches):
+                g[prob.n_point_batches[b]:, b] = 0.0
+            err = np.abs(g - ref).max() / np.abs(ref).max()
+            print(f"[inline bias FD j={jb + t} c={jc}] rel {err:.2e}")
+            assert err <= 1e-7
+        # the other side of DM is a different function (DM nonsymmetric)
+        other = oracle.bias_inline(prob, DM.T, gwave, T, jb, j_coord=jc)
+        assert np.abs(other[:, :, 0] - got[:, :, 0]).max() > 1e-3 * np.abs(got).max()
+
+
+def test_inline_bias_two_sides_make_bias_part2(oracle):
+    """inline(DM) + inline(DM^T) = bias part 2 of DM (the two terms of the symmetric derivative),
+    with random wave and gradient values and a short tile (j_atom_end < j_atom_begin + T - 1)."""
+    prob = synth.make_rho1_problem(10, 10.0, 20, (10, 50), (10, 60), config_k=2, zero_nc=(3,))
+    gw = synth.random_like_wave(prob, 3, 8)
+    DM = np.random.default_rng(9).standard_normal((prob.n_basis, prob.n_basis))
+    T, jb, je = 4, 3, 5
+    a = oracle.bias_inline(prob, DM, gw, T, jb, j_atom_end=je, j_coord=2)
+    both = oracle.bias_inline(prob, DM.T, gw, T, jb, j_atom_end=je, j_coord=2, bias=a)
+    ref = oracle.bias2(prob, DM, gw, T, jb, j_atom_end=je, j_coord=2)
+    err = np.abs(both - ref).max() / np.abs(ref).max()
+    print(f"[inline bias two sides vs part 2] rel {err:.2e}")
+    assert err <= 1e-13
+    assert np.all(both[:, :, je - jb + 1:] == 0.0)

@@ -89,6 +89,66 @@ void build_item_order(Rho1Plan &P, int pch) {
   P.items_pch = pch;
 }
 
+TileSched &build_tile_items(Rho1Plan &P, int pch, bool grad) {
+  TileSched &ts = P.tiles[grad ? 1 : 0];
+  if (ts.pch == pch)
+    return ts;
+  // row tiles grouped into contiguous ranges of about `work` K steps (a row tile I of nT costs
+  // nT - I steps, nT with the gradient, plus ~2 for staging its diagonal chunk and its epilogue)
+  int work = 16;
+  if (const char *e = std::getenv("FRPA_TILE_WORK"))
+    work = std::max(1, std::atoi(e));
+  const int C = grad ? 4 : 1;
+  std::vector<int> caller(P.n_valid);
+  std::iota(caller.begin(), caller.end(), 0);
+  std::stable_sort(caller.begin(), caller.end(), [&](int a, int b) { return P.h_desc[a].b < P.h_desc[b].b; });
+  std::vector<TileItem> seq;
+  std::vector<PassRec> recs;
+  std::vector<long long> qoff;
+  long long poff = 0;
+  for (int di : caller) {
+    const BatchDesc &d = P.h_desc[di];
+    const int nT = (d.nc + 63) / 64, passes = (d.np + pch - 1) / pch;
+    qoff.assign(passes, 0);
+    for (int q = 0; q < passes; q++) {
+      const int npc = std::min(pch, d.np - q * pch);
+      recs.push_back(PassRec{di, q, nT, npc, poff});
+      qoff[q] = poff;
+      poff += (long long)nT * C * npc;
+    }
+    // ranges of row tiles, each pass of a range an item: a batch's items are consecutive, the
+    // passes of one range adjacent (they read the same P' tiles at about the same time)
+    int I0 = 0, w = 0;
+    for (int I = 0; I < nT; I++) {
+      w += (grad ? nT : nT - I) + 2;
+      if (w >= work || I == nT - 1) {
+        for (int q = 0; q < passes; q++)
+          seq.push_back(TileItem{di, q, I0, I + 1, qoff[q]});
+        I0 = I + 1;
+        w = 0;
+      }
+    }
+  }
+  const int n = (int)seq.size();
+  std::vector<TileItem> items(std::max(n, 1), TileItem{0, 0, 0, 0, 0});
+  int start = 0;
+  for (int x = 0; x < N_XCD; x++) {
+    const int len = n / N_XCD + (x < n % N_XCD ? 1 : 0);
+    for (int k = 0; k < len; k++)
+      items[N_XCD * k + x] = seq[start + k];
+    start += len;
+  }
+  ts.d_items.upload(items);
+  if (recs.empty())
+    recs.push_back(PassRec{0, 0, 0, 0, 0});
+  ts.d_recs.upload(recs);
+  ts.n_items = n;
+  ts.n_recs = (int)recs.size();
+  ts.part_doubles = poff;
+  ts.pch = pch;
+  return ts;
+}
+
 void fill_f64(double *p, size_t n, double v, hipStream_t s) {
   if (n == 0)
     return;
@@ -439,8 +499,8 @@ int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int per
     FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
     FRPA_REQUIRE(precision == FRPA_PREC_FP64 || precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4,
                  FRPA_ERR_INVALID, "unknown precision %d", precision);
-    FRPA_REQUIRE(order == 0 || order == 1 || (order == 2 && precision != FRPA_PREC_FP64), FRPA_ERR_INVALID,
-                 "order must be 0 or 1 (or 2 for the INT8 split)");
+    FRPA_REQUIRE(order == 0 || order == 1 || ((order == 2 || order == 3) && precision != FRPA_PREC_FP64),
+                 FRPA_ERR_INVALID, "order must be 0 or 1 (or 2, 3 for the INT8 split)");
     FRPA_REQUIRE(persistent == 0 || persistent == 1 || persistent >= N_XCD, FRPA_ERR_INVALID,
                  "persistent must be 0, 1 or a grid of at least %d workgroups", N_XCD);
     const bool ok = points_per_wave == 0 ||
@@ -475,7 +535,8 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
     else
       npgs = {0, 2};
     std::vector<LaunchCfg> cands;
-    const int n_orders = precision == FRPA_PREC_FP64 ? 2 : 3;  // order 2: point passes as items (INT8)
+    // INT8: order 2, point passes as items; order 3, row-tile ranges of point passes as items
+    const int n_orders = precision == FRPA_PREC_FP64 ? 2 : 4;
     for (int order = 0; order < n_orders; order++)
       for (int pers = 0; pers < 2; pers++)
         for (int npg : npgs)

@@ -16,6 +16,29 @@ struct BatchDesc {
 };
 static_assert(sizeof(BatchDesc) == 32, "BatchDesc layout");
 
+// Order 3 of the INT8 forward: a work item is the row tiles [I0, I1) of one point pass q of a batch;
+// the items of a batch are claimed back to back on one XCD, so they share its phi digits and P'
+// tiles in that XCD's L2.  Each row tile leaves its per-point sums (C components: rho1, and the
+// three gradient components for the GGA launch) in a partial buffer at poff + (I C + c) npc + p, and
+// a finishing kernel adds them up per pass in row-tile order.
+struct TileItem {
+  int di;          // BatchDesc index
+  int q;           // point pass (points q pch .. + npc - 1)
+  int I0, I1;      // row tiles
+  long long poff;  // partial-buffer offset of the pass (doubles, per perturbation)
+};
+struct PassRec {
+  int di, q, nT, npc;
+  long long poff;
+};
+struct TileSched {
+  int pch = 0;                  // points per pass the lists were built for (0 = not built)
+  int n_items = 0, n_recs = 0;
+  long long part_doubles = 0;   // partial-buffer doubles per perturbation
+  DevBuf<TileItem> d_items;     // in launch-position order (XCD-interleaved)
+  DevBuf<PassRec> d_recs;
+};
+
 // Basis relabeling: new label pi[mu] = order of first appearance of mu over the batches in the
 // caller's order (a batch's functions in ascending index), so that the compute sets of a spatially
 // ordered batch list become long runs of consecutive labels.  Per batch, its compute rows sorted by
@@ -63,6 +86,8 @@ struct Rho1Plan {
   // its own, (BatchDesc index << 8) | pass; the passes of a batch are adjacent in one XCD's range
   DevBuf<int> d_order_items;
   int n_items = 0, items_pch = 0;
+  TileSched tiles[2];               // order 3 (INT8 forward): [0] rho1 only, [1] with the gradient
+  DevBuf<double> d_tile_part;       // order 3 partial sums (grow-only)
   DevBuf<int> d_counters;           // N_XCD work counters of the persistent mode
   int n_cu = 0;                     // compute units of the device the plan was made on
   bool wave_off_quads = false;      // every BatchDesc::wave_off a multiple of 4 doubles
@@ -101,6 +126,8 @@ struct LaunchArgs {
 LaunchArgs launch_args(Rho1Plan &P, const LaunchCfg &cfg, int T, int blocks_per_cu, hipStream_t s, int n_pos = -1);
 // order 2 position map for point passes of pch points (built on first use, rebuilt if pch changes)
 void build_item_order(Rho1Plan &P, int pch);
+// order 3 row-tile work items for point passes of pch points (gradient launch: every tile pair)
+TileSched &build_tile_items(Rho1Plan &P, int pch, bool grad);
 void fill_i32(int *p, size_t n, int v, hipStream_t s);
 
 // Persistent-mode work claiming: item k of XCD group x is launch position 8 (k % len) + x of

@@ -246,9 +246,39 @@ struct GradArgs {
   const int *srow_all;  // caller's local row of every sorted row (Relabel::d_srow)
 };
 
+// Row tiles [I0, I1) of the chunk.  Each row tile's per-point sums over its 64 rows (per component
+// c: rho1, and with GRAD the three gradient components) are combined across the four row groups in
+// LDS, ((g0 + g1) + g2) + g3, and then either added up over the row tiles in order by this block
+// (part == nullptr: the whole batch, final values written) or stored at part[(I C + c) npc + p] for
+// the finishing kernel, which adds them up in the same order -- so every launch order gives the
+// same bits.
+// Final value of component c at point p of batch d from its sum over the row tiles
+// (evaluate_first_order_rho.cpp:187-200): rho1 = 2 sum (symmetric form; GRAD: 1 sum) + bias,
+// grad rho1 = 2 sum - gradient bias.
+template <bool GRAD>
+__device__ __forceinline__ void ozaki_final(int c, double tot, const BatchDesc &d, int t, int p,
+                                            const int *__restrict__ map, const double *__restrict__ bias,
+                                            double *__restrict__ rho, int n_full, int nmb, int n_batches_all,
+                                            int n_tile, const GradArgs &ga) {
+  const size_t bi = (size_t)p + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
+  if (c == 0) {
+    double sum = (GRAD ? 1.0 : 2.0) * tot;
+    if (bias)
+      sum += bias[bi];
+    if (t < n_tile)
+      rho[(size_t)map[d.pstart + p] + (size_t)n_full * t] = sum;
+  } else {
+    double gv = 2.0 * tot;
+    if (ga.gbias)
+      gv -= ga.gbias[(c - 1) + 3 * bi];
+    ga.grad[(c - 1) + 3 * bi] = gv;
+  }
+}
+
 template <int S, int NPG, int NG, int H, bool GRAD>
 __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *red, const BatchDesc &d, int t,
-                                            int pc, int npc, int gp0, const int *__restrict__ sib,
+                                            int pc, int npc, int gp0, int I0, int I1, double *__restrict__ part,
+                                            const int *__restrict__ sib,
                                             const int *__restrict__ map, const unsigned long long *__restrict__ At,
                                             const double *__restrict__ ai,
                                             const signed char *__restrict__ Bg, const double *__restrict__ binv,
@@ -264,15 +294,31 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rg = w & 3;
   const int lr = lane & 15, lk = lane >> 4;
   const int nT = ncK / 64;
-  double racc[NG > 0 ? NG : 1];
-  double gacc[GRAD ? 3 : 1][NG > 0 ? NG : 1];
+  constexpr int C = GRAD ? 4 : 1;
+  // per-tile sums of row group rg, component c at redI[(4 c + rg) PCH + p]; GRAD keeps them in
+  // staging buffer 1, which no wave touches from the last K step of a tile until the second K
+  // step of the next one
+  double *redI = GRAD ? reinterpret_cast<double *>(Bs + BUF) : red;
+  // reducing threads: element e = tid + THREADS u is component e / PCH of point e % PCH
+  constexpr int RPT = (C * PCH + THREADS - 1) / THREADS;
+  double tot[RPT];
 #pragma unroll
-  for (int g = 0; g < NG; g++) {
-    racc[g] = 0.0;
+  for (int u = 0; u < RPT; u++)
+    tot[u] = 0.0;
+  auto reduce_tile = [&](int I) {
 #pragma unroll
-    for (int j = 0; j < (GRAD ? 3 : 1); j++)
-      gacc[j][g] = 0.0;
-  }
+    for (int u = 0; u < RPT; u++) {
+      const int e = tid + THREADS * u, rc = e / PCH, rp = e % PCH;
+      if (rc < C && rp < npc) {
+        const double *r = redI + 4 * rc * PCH + rp;
+        const double v = ((r[0] + r[PCH]) + r[2 * PCH]) + r[3 * PCH];
+        if (part)
+          part[((size_t)I * C + rc) * npc + rp] = v;
+        else
+          tot[u] += v;
+      }
+    }
+  };
   const int ncp = (nc + 3) & ~3;
   const double *GW = GRAD ? ga.gwave + 3 * d.wave_off : nullptr;
   const int *srow = GRAD ? ga.srow_all + d.ib_off : nullptr;
@@ -304,7 +350,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
     }
   };
 
-  for (int I = 0; I < nT; I++) {
+  for (int I = I0; I < I1; I++) {
     const int irow = 64 * I + 16 * rg + lr;
     const bool iok = irow < nc;
     const size_t arow = iok ? (size_t)sib[irow] : 0;
@@ -327,6 +373,8 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
       }
     };
     __syncthreads();  // the previous tile is done with all buffers
+    if (I > I0)
+      reduce_tile(I - 1);
     load_b(I);
     store_b(I, 2);
     __syncthreads();
@@ -413,51 +461,33 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
       }
       part += __shfl_xor(part, 16);
       part += __shfl_xor(part, 32);
-      racc[g] += part;
       if (GRAD) {
 #pragma unroll
         for (int j = 0; j < 3; j++) {
           gp[j] += __shfl_xor(gp[j], 16);
           gp[j] += __shfl_xor(gp[j], 32);
-          gacc[j][g] += gp[j];
+        }
+      }
+      if (lk == 0) {
+        redI[rg * PCH + p] = part;
+        if (GRAD) {
+#pragma unroll
+          for (int j = 0; j < 3; j++)
+            redI[(4 * (j + 1) + rg) * PCH + p] = gp[j];
         }
       }
     }
   }
 
-  // per-point sums over the four row groups; GRAD puts them (4 values per point and row group) in
-  // the staging buffers 0 and 1, which no wave reads after the last tile's K loop
-  double(*rd)[PCH] = reinterpret_cast<double(*)[PCH]>(GRAD ? reinterpret_cast<double *>(Bs) : red);
   __syncthreads();
-  if (lk == 0) {
+  if (I1 > I0)
+    reduce_tile(I1 - 1);
+  if (!part) {
 #pragma unroll
-    for (int g = 0; g < NG; g++) {
-      rd[rg][16 * (gp0 + g) + lr] = racc[g];
-      if (GRAD) {
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-          rd[4 * (j + 1) + rg][16 * (gp0 + g) + lr] = gacc[j][g];
-      }
-    }
-  }
-  __syncthreads();
-  for (int p = tid; p < npc; p += THREADS) {
-    const size_t bi = (size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
-    // symmetric form: 2 sum_i phi_i sum_{k >= i} P'(i, k) phi_k; GRAD: sum_i phi_i W_i
-    double sum = (GRAD ? 1.0 : 2.0) * (rd[0][p] + rd[1][p] + rd[2][p] + rd[3][p]);
-    if (bias)
-      sum += bias[bi];
-    if (t < n_tile)
-      rho[(size_t)map[d.pstart + pc + p] + (size_t)n_full * t] = sum;
-    if (GRAD) {
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        const int q = 4 * (j + 1);
-        double gv = 2.0 * (rd[q][p] + rd[q + 1][p] + rd[q + 2][p] + rd[q + 3][p]);
-        if (ga.gbias)
-          gv -= ga.gbias[j + 3 * bi];
-        ga.grad[j + 3 * bi] = gv;
-      }
+    for (int u = 0; u < RPT; u++) {
+      const int e = tid + THREADS * u, rc = e / PCH, rp = e % PCH;
+      if (rc < C && rp < npc)
+        ozaki_final<GRAD>(rc, tot[u], d, t, pc + rp, map, bias, rho, n_full, nmb, n_batches_all, n_tile, ga);
     }
   }
 }
@@ -465,12 +495,12 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
 template <int S, int NPG, int H, bool GRAD, int... Gs>
 __device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>, int ng, signed char *Bs, int *jbs,
                                                double *red, const BatchDesc &d, int t, int pc, int npc, int gp0,
-                                               const int *sib, const int *map, const unsigned long long *At,
+                                               int I0, int I1, double *part, const int *sib, const int *map, const unsigned long long *At,
                                                const double *ai, const signed char *Bg,
                                                const double *binv, int n_basis, const double *bias, double *rho,
                                                int n_full, int nmb, int n_batches_all, int n_tile,
                                                const GradArgs &ga) {
-  ((ng == Gs ? ozaki_chunk<S, NPG, Gs, H, GRAD>(Bs, jbs, red, d, t, pc, npc, gp0, sib, map, At, ai, Bg, binv, n_basis,
+  ((ng == Gs ? ozaki_chunk<S, NPG, Gs, H, GRAD>(Bs, jbs, red, d, t, pc, npc, gp0, I0, I1, part, sib, map, At, ai, Bg, binv, n_basis,
                                              bias, rho, n_full, nmb, n_batches_all, n_tile, ga)
              : (void)0),
    ...);
@@ -483,7 +513,7 @@ __global__ __launch_bounds__(256 * H, 2) void rho1_ozaki_kernel(
     const double *__restrict__ ainv, const signed char *__restrict__ bdig,
     const long long *__restrict__ boff, const double *__restrict__ binv, int n_basis,
     const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all, int n_tile,
-    int items, GradArgs ga) {
+    int items, const TileItem *__restrict__ titems, double *__restrict__ part, long long part_stride, GradArgs ga) {
   constexpr int PCH = 16 * H * NPG;
   __shared__ __attribute__((aligned(16))) signed char Bs[3 * S * PCH * BROW];
   __shared__ int jbs[192];
@@ -499,22 +529,52 @@ __global__ __launch_bounds__(256 * H, 2) void rho1_ozaki_kernel(
       break;
     }
     const int it = order ? order[pos] : pos;
-    const int di = items ? it >> 8 : it;  // items: (BatchDesc index << 8) | point pass
+    // items: (BatchDesc index << 8) | point pass; titems: row tiles of a point pass
+    const TileItem ti = titems ? titems[pos] : TileItem{0, 0, 0, 0, 0};
+    const int di = titems ? ti.di : items ? it >> 8 : it;
     const BatchDesc d = descs[di];
+    const int nT = (d.nc + 63) / 64;
+    const int I0 = titems ? ti.I0 : 0, I1 = titems ? ti.I1 : nT;
+    double *tpart = titems ? part + part_stride * t + ti.poff : nullptr;
     const int *sib = sib_all + d.ib_off;
     const size_t nn = (size_t)n_basis * n_basis * t;
     const double *ai = ainv + (size_t)n_basis * t;
     const signed char *Bg = bdig + boff[di];
-    const int pbeg = items ? PCH * (it & 255) : 0, pend = items ? min(d.np, pbeg + PCH) : d.np;
+    const int pbeg = titems ? PCH * ti.q : items ? PCH * (it & 255) : 0;
+    const int pend = (titems || items) ? min(d.np, pbeg + PCH) : d.np;
     for (int pc = pbeg; pc < pend; pc += PCH) {
       const int npc = min(PCH, d.np - pc);
       const int ngt = (npc + 15) >> 4;       // groups of 16 points in this chunk
       const int n0 = H == 2 ? (ngt + 1) >> 1 : ngt;  // the first half takes the extra group
       const int ng = half ? ngt - n0 : n0, gp0 = half ? n0 : 0;
       ozaki_dispatch<S, NPG, H, GRAD>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc,
-                                      gp0, sib, map, adig + nn, ai, Bg, binv, n_basis, bias, rho, n_full, nmb,
+                                      gp0, I0, I1, tpart, sib, map, adig + nn, ai, Bg, binv, n_basis, bias, rho, n_full, nmb,
                                       n_batches_all, n_tile, ga);
     }
+  }
+}
+
+// Order 3: per pass record, the row tiles' partial sums added up in row-tile order (as the block
+// adds them in the other orders) and the final values written.
+template <bool GRAD>
+__global__ __launch_bounds__(256) void rho1_ozaki_finish_kernel(const PassRec *__restrict__ recs,
+                                                                const BatchDesc *__restrict__ descs, int pch,
+                                                                const double *__restrict__ part,
+                                                                long long part_stride, const int *__restrict__ map,
+                                                                const double *__restrict__ bias,
+                                                                double *__restrict__ rho, int n_full, int nmb,
+                                                                int n_batches_all, int n_tile, GradArgs ga) {
+  constexpr int C = GRAD ? 4 : 1;
+  const PassRec r = recs[blockIdx.x];
+  const int t = blockIdx.y;
+  const BatchDesc d = descs[r.di];
+  const double *pp = part + part_stride * t + r.poff;
+  for (int e = threadIdx.x; e < C * r.npc; e += blockDim.x) {
+    const int c = e / r.npc, p = e % r.npc;
+    double tot = 0.0;
+    for (int I = 0; I < r.nT; I++)
+      tot += pp[((size_t)I * C + c) * r.npc + p];
+    ozaki_final<GRAD>(c, tot, d, t, pch * r.q + p, map, bias, rho, n_full, nmb, n_batches_all, n_tile, ga);
   }
 }
 
@@ -550,17 +610,31 @@ void launch(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s)
     FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_ozaki_kernel<S, NPG, H, GRAD>, 256 * H, 0));
     per_cu = std::max(per_cu, 1);
   }
+  constexpr int PCH = 16 * H * NPG;
   const bool items = cfg.order == 2;
+  TileSched *ts = cfg.order == 3 ? &build_tile_items(P, PCH, GRAD) : nullptr;
   if (items)
-    build_item_order(P, 16 * H * NPG);
-  const int n_pos = items ? P.n_items : P.n_valid;
-  const LaunchArgs la = launch_args(P, cfg, c.T, per_cu, s, n_pos);
+    build_item_order(P, PCH);
+  const int n_pos = ts ? ts->n_items : items ? P.n_items : P.n_valid;
+  if (ts)
+    P.d_tile_part.grow((size_t)ts->part_doubles * c.T);
+  LaunchCfg lc = cfg;
+  if (ts)
+    lc.order = 0;  // the item list is already in launch-position order
+  const LaunchArgs la = launch_args(P, lc, c.T, per_cu, s, n_pos);
   const GradArgs ga{c.gwave, c.gbias, c.grad, R.d_srow.ptr};
   hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG, H, GRAD>), la.grid, dim3(256 * H), 0, s, P.d_desc.ptr, la.order,
                      la.counters, n_pos, c.T, R.d_sib.ptr, P.d_map.ptr, st.d_adig.ptr, st.d_ainv.ptr,
                      st.d_bdig.ptr, st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full, P.nmb,
-                     P.n_batches, c.n_tile, (int)items, ga);
+                     P.n_batches, c.n_tile, (int)items, ts ? ts->d_items.ptr : nullptr,
+                     ts ? P.d_tile_part.ptr : nullptr, ts ? ts->part_doubles : 0LL, ga);
   check_launch("rho1_ozaki_kernel");
+  if (ts) {
+    hipLaunchKernelGGL((rho1_ozaki_finish_kernel<GRAD>), dim3(ts->n_recs, c.T), dim3(256), 0, s, ts->d_recs.ptr,
+                       P.d_desc.ptr, PCH, P.d_tile_part.ptr, ts->part_doubles, P.d_map.ptr, c.bias, c.rho, P.n_full,
+                       P.nmb, P.n_batches, c.n_tile, ga);
+    check_launch("rho1_ozaki_finish_kernel");
+  }
 }
 
 template <int S, bool GRAD>

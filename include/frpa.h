@@ -107,8 +107,9 @@ int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, c
                       double *first_order_H, void *stream);
 
 /* Autotuner of the rho1 forward launch: times every launch configuration valid for the plan and
- * precision (batch order: descending work, XCD-aware spatial, or spatial with point passes as
- * separate items; one workgroup per batch or a
+ * precision (batch order: descending work, XCD-aware spatial, spatial with point passes as
+ * separate items, or spatial with ranges of row tiles of point passes as items; one workgroup per
+ * batch or a
  * persistent grid with per-XCD work counters; points per pass) on the given inputs, `reps` calls
  * each after one warm-up, with HIP events on `stream`; keeps the fastest in the plan for every later
  * frpa_rho1_forward of that precision (the GGA gradient launch included) and writes a description
@@ -123,7 +124,11 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
 /* Sets the launch configuration of frpa_rho1_forward for one precision explicitly (what the
  * autotuner chooses): order 0 = descending work, 1 = XCD-aware spatial, 2 = (INT8 split only) the
  * spatial order with every point pass of a batch as its own work item, the passes of a batch
- * adjacent on one XCD so they share its P' gathers in L2; persistent 0 = one
+ * adjacent on one XCD so they share its P' gathers in L2; 3 = (INT8 split only) every range of
+ * 64-row tiles (about 16 K steps of work) of a point pass its own item, a batch's items adjacent on
+ * one XCD so they also share its phi digits in L2 (per-tile sums go through a device buffer of
+ * n_row_tiles x points doubles per batch, 4x with the gradient, times atom_tile_size, and a
+ * finishing kernel; environment variable FRPA_TILE_WORK sets the K steps per item); persistent 0 = one
  * workgroup per batch, 1 = a persistent grid sized from the occupancy, k >= 8 = a persistent grid of
  * k workgroups (rounded down to a multiple of 8); points_per_wave = groups of 16 points per wave and
  * pass (FP64: 4, 8, 13; INT8 split S = 5: 4; S = 4: 4, 5; 0 = default), and for the INT8 split 2 =

@@ -155,3 +155,29 @@ def test_rho1_512atom_shapes_all_precisions(cuda, oracle):
         print(f"[rho1 512 atoms {name}] max rel {mx:.3e}, median pointwise rel {med:.3e}, "
               f"max condition-normalised {cn:.3e}", flush=True)
         assert np.all(np.isfinite(got[valid])) and mx <= tol
+
+
+def test_rho1_1024atom_row_tile_items(shape1024):
+    """Launch order 3 (row-tile ranges of point passes as work items, up to 19 row tiles per batch,
+    per-tile sums finished by a second kernel) gives the bits of the default launch, with and
+    without the gradient, at the 1024-atom shapes."""
+    D = shape1024
+    prob, plan = D["prob"], D["plan"]
+    wave = D["wave"]
+    gw = api.to_device(synth.random_like_wave(prob, 3, 78))
+    gb = torch.zeros(3 * prob.n_max_batch_size * prob.n_batches, dtype=torch.float64, device="cuda")
+    w0 = wave[OFFSET_BASE:]
+    plan0 = api.Rho1Plan.from_problem(prob)
+    ref = plan0.forward(w0, D["P1"], precision=api.OZAKI).clone()
+    gref = [x.clone() for x in plan0.forward(w0, D["P1"], precision=api.OZAKI, gradient_wave=gw, grad_bias=gb)]
+    for persistent in (0, 1):
+        plan0.set_launch(api.OZAKI, 3, persistent, 2)
+        got = plan0.forward(w0, D["P1"], precision=api.OZAKI)
+        r2, g2 = plan0.forward(w0, D["P1"], precision=api.OZAKI, gradient_wave=gw, grad_bias=gb)
+        print(f"[rho1 1024 atoms order 3 persistent {persistent}] identical {torch.equal(got, ref)}, "
+              f"gradient identical {torch.equal(r2, gref[0]) and torch.equal(g2, gref[1])}", flush=True)
+        assert torch.equal(got, ref)
+        assert torch.equal(r2, gref[0]) and torch.equal(g2, gref[1])
+    valid = prob.valid_map()
+    err = np.abs(ref.cpu().numpy()[valid] - D["rho_ref"][valid]).max() / np.abs(D["rho_ref"]).max()
+    assert err <= 1e-10

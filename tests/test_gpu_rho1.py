@@ -168,11 +168,13 @@ def test_rho1_int8_split_nonsymmetric_multi_tile(cuda, oracle):
 
 
 @pytest.mark.parametrize("precision,npgs", [(api.FP64, (4, 8, 13)), (api.OZAKI, (4, 2)), (api.OZAKI4, (4, 5, 2))])
-def test_rho1_launch_configs_bit_identical(cuda, oracle, precision, npgs):
+def test_rho1_launch_configs_bit_identical(cuda, oracle, monkeypatch, precision, npgs):
     """Every launch configuration the autotuner can choose (batch order, point passes as separate
-    items, persistent grid with per-XCD work counters, points per pass, and for the INT8 split the
+    items, row-tile ranges of point passes as items -- here one row tile per item, FRPA_TILE_WORK = 3
+    -- persistent grid with per-XCD work counters, points per pass, and for the INT8 split the
     four-wave block over 64-point passes, npg = 2) gives the same bits; T = 2 exercises the persistent
     claim of (position, perturbation) items; the autotuner keeps one of them."""
+    monkeypatch.setenv("FRPA_TILE_WORK", "3")
     prob = synth.make_rho1_problem(14, 12.0, 101, (1, 230), (1, 150), config_k=13, zero_nc=(4,), zero_np=(8,))
     plan = api.Rho1Plan.from_problem(prob)
     wave = api.to_device(prob.wave)
@@ -189,7 +191,8 @@ def test_rho1_launch_configs_bit_identical(cuda, oracle, precision, npgs):
     gb = torch.zeros(3 * prob.n_max_batch_size * prob.n_batches * T, dtype=torch.float64, device="cuda")
     gref_rho, gref = (x.clone() for x in plan.forward(wave, P1, T=T, precision=precision, gradient_wave=gw,
                                                       grad_bias=gb))
-    orders = (0, 1) if precision == api.FP64 else (0, 1, 2)  # 2: point passes as separate items (INT8)
+    # INT8 only: 2, point passes as separate items; 3, row-tile ranges of point passes as items
+    orders = (0, 1) if precision == api.FP64 else (0, 1, 2, 3)
     for order in orders:
         for persistent in (0, 1, 16):
             for npg in npgs:

@@ -506,9 +506,10 @@ int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int per
     const bool ok = points_per_wave == 0 ||
                     (precision == FRPA_PREC_FP64 && (points_per_wave == 4 || points_per_wave == 8 ||
                                                      points_per_wave == 13)) ||
-                    (precision == FRPA_PREC_OZAKI && (points_per_wave == 4 || points_per_wave == 2)) ||
-                    (precision == FRPA_PREC_OZAKI4 &&
-                     (points_per_wave == 4 || points_per_wave == 5 || points_per_wave == 2));
+                    (precision == FRPA_PREC_OZAKI &&
+                     (points_per_wave == 4 || points_per_wave == 2 || points_per_wave == 1)) ||
+                    (precision == FRPA_PREC_OZAKI4 && (points_per_wave == 4 || points_per_wave == 5 ||
+                                                       points_per_wave == 2 || points_per_wave == 1));
     FRPA_REQUIRE(ok, FRPA_ERR_INVALID, "%d point groups per wave not available for precision %d", points_per_wave,
                  precision);
     reinterpret_cast<Rho1Plan *>(plan)->cfg[precision] = LaunchCfg{order, persistent, points_per_wave};
@@ -531,9 +532,9 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
     if (precision == FRPA_PREC_FP64)
       npgs = {4, 8, 13};
     else if (precision == FRPA_PREC_OZAKI4)
-      npgs = {4, 5, 2};
+      npgs = {4, 5, 2, 1};
     else
-      npgs = {0, 2};
+      npgs = {0, 2, 1};
     std::vector<LaunchCfg> cands;
     // INT8: order 2, point passes as items; order 3, row-tile ranges of point passes as items
     const int n_orders = precision == FRPA_PREC_FP64 ? 2 : 4;

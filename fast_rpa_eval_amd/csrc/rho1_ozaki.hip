@@ -28,6 +28,11 @@
 // digits are prepared once per wave (frpa_rho1_prepare_wave) as S int8 planes per batch, rows in the
 // batch's sorted order, (ncK, np) column-major with ncK = nc rounded up to 64 (zero rows), so a
 // lane's B fragment (16 consecutive k of one point) is one 16-byte load.
+//
+// Launch shapes: 8-wave blocks over 128-point passes or 4-wave blocks over 64-point passes with the
+// phi chunks staged through registers, or (ASYNC) 4-wave blocks whose phi chunks go straight from
+// global memory into LDS and whose P' gathers for the next K step are in flight during the current
+// step's MFMAs; work items are whole batches, point passes, or ranges of row tiles of a point pass.
 #include <algorithm>
 #include <utility>
 #include <vector>
@@ -223,6 +228,12 @@ __device__ __forceinline__ int gather_bytes(unsigned long long e0, unsigned long
 // through registers into the other LDS buffer and the P' digits of chunk J + 1 are gathered into
 // registers while chunk J's MFMAs run (one barrier per chunk).
 constexpr int BROW = 80;  // bytes per (digit, point) row of a staged B chunk: 64 k + 16 pad
+// ASYNC: the B chunks are staged with direct global -> LDS loads (global_load_lds_dwordx4: lane l of
+// a wave writes its 16 bytes at the wave's LDS base + 16 l), so a chunk is 1 KB blocks of 16
+// unpadded 64-byte rows and no registers hold it in flight; the P' gathers of chunk J + 1 are
+// issued before chunk J's MFMAs and both are waited for once, before the barrier that ends the step.
+constexpr int BROW_ASYNC = 64;
+typedef __attribute__((address_space(3))) void lds_void;
 
 template <int S>
 __device__ __forceinline__ void repack_a(const unsigned long long (&e)[16], v4i (&a)[S]) {
@@ -230,6 +241,32 @@ __device__ __forceinline__ void repack_a(const unsigned long long (&e)[16], v4i 
   for (int sl = 0; sl < S; sl++)
     a[sl] = v4i{gather_bytes(e[0], e[1], e[2], e[3], sl), gather_bytes(e[4], e[5], e[6], e[7], sl),
                 gather_bytes(e[8], e[9], e[10], e[11], sl), gather_bytes(e[12], e[13], e[14], e[15], sl)};
+}
+
+// ASYNC gather: P' digits of 16 elements as (lo, hi) dword pairs, loaded through a buffer resource
+// with 32-bit byte offsets (one add per element; every offset lies inside the perturbation's
+// n_basis^2 elements), and the digit planes rebuilt with byte permutes: for elements 4 h .. 4 h + 3,
+// digits 0-3 are a 4 x 4 byte transpose of the low dwords (8 v_perm) and digit 4 collects byte 0 of
+// the high dwords (3 v_perm).  v_perm_b32(x, y, sel): selector byte 0-3 takes byte of y, 4-7 of x.
+typedef unsigned u2 __attribute__((ext_vector_type(2)));
+
+template <int S>
+__device__ __forceinline__ void repack_a_perm(const u2 (&el)[16], v4i (&a)[S]) {
+#pragma unroll
+  for (int h = 0; h < 4; h++) {
+    const unsigned L0 = el[4 * h].x, L1 = el[4 * h + 1].x, L2 = el[4 * h + 2].x, L3 = el[4 * h + 3].x;
+    const unsigned t01lo = __builtin_amdgcn_perm(L1, L0, 0x05010400u), t01hi = __builtin_amdgcn_perm(L1, L0, 0x07030602u);
+    const unsigned t23lo = __builtin_amdgcn_perm(L3, L2, 0x05010400u), t23hi = __builtin_amdgcn_perm(L3, L2, 0x07030602u);
+    a[0][h] = (int)__builtin_amdgcn_perm(t23lo, t01lo, 0x05040100u);
+    a[1][h] = (int)__builtin_amdgcn_perm(t23lo, t01lo, 0x07060302u);
+    a[2][h] = (int)__builtin_amdgcn_perm(t23hi, t01hi, 0x05040100u);
+    a[3][h] = (int)__builtin_amdgcn_perm(t23hi, t01hi, 0x07060302u);
+    if constexpr (S == 5) {
+      const unsigned u = __builtin_amdgcn_perm(el[4 * h + 1].y, el[4 * h].y, 0x05010400u);
+      const unsigned v = __builtin_amdgcn_perm(el[4 * h + 3].y, el[4 * h + 2].y, 0x05010400u);
+      a[4][h] = (int)__builtin_amdgcn_perm(v, u, 0x05040100u);
+    }
+  }
 }
 
 // One point chunk; this wave multiplies NG (exact, compile-time) groups of 16 points starting at
@@ -275,7 +312,7 @@ __device__ __forceinline__ void ozaki_final(int c, double tot, const BatchDesc &
   }
 }
 
-template <int S, int NPG, int NG, int H, bool GRAD>
+template <int S, int NPG, int NG, int H, bool GRAD, bool ASYNC>
 __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *red, const BatchDesc &d, int t,
                                             int pc, int npc, int gp0, int I0, int I1, double *__restrict__ part,
                                             const int *__restrict__ sib,
@@ -286,7 +323,8 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
                                             int n_full, int nmb, int n_batches_all, int n_tile, const GradArgs &ga) {
   constexpr int THREADS = 256 * H;
   constexpr int PCH = 16 * H * NPG;             // point slots per chunk
-  constexpr int BUF = S * PCH * BROW;           // bytes per LDS buffer
+  constexpr int RB = ASYNC ? BROW_ASYNC : BROW;  // bytes per staged (digit, point) row
+  constexpr int BUF = S * PCH * RB;             // bytes per LDS buffer
   constexpr int NQ = S * PCH * 4;               // 16-byte pieces of one staged chunk
   constexpr int QPT = (NQ + THREADS - 1) / THREADS;
   const int nc = d.nc, ncK = ceil64(nc);
@@ -295,10 +333,10 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
   const int lr = lane & 15, lk = lane >> 4;
   const int nT = ncK / 64;
   constexpr int C = GRAD ? 4 : 1;
-  // per-tile sums of row group rg, component c at redI[(4 c + rg) PCH + p]; GRAD keeps them in
-  // staging buffer 1, which no wave touches from the last K step of a tile until the second K
-  // step of the next one
-  double *redI = GRAD ? reinterpret_cast<double *>(Bs + BUF) : red;
+  // per-tile sums of row group rg, component c at redI[(4 c + rg) PCH + p]; the register-staged
+  // GRAD launch keeps them in staging buffer 1, which no wave touches from the last K step of a tile
+  // until the second K step of the next one (the ASYNC launch gives them an array of their own)
+  double *redI = (GRAD && !ASYNC) ? reinterpret_cast<double *>(Bs + BUF) : red;
   // reducing threads: element e = tid + THREADS u is component e / PCH of point e % PCH
   constexpr int RPT = (C * PCH + THREADS - 1) / THREADS;
   double tot[RPT];
@@ -349,6 +387,23 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
       jbs[64 * buf + tid] = j < nc ? sib[j] : -1;
     }
   };
+  // ASYNC: chunk c's digits straight into buffer buf, 1 KB block k of 64 pieces by wave k mod (4 H);
+  // points past npc read the last point's digits (their sums are never used)
+  auto stage_b = [&](int c, int buf) {
+    constexpr int NBLK = S * PCH / 16, NW = 4 * H;
+    signed char *B = Bs + buf * BUF;
+#pragma unroll
+    for (int u = 0; u < (NBLK + NW - 1) / NW; u++) {
+      const int blk = NW * u + w;
+      if (blk < NBLK) {
+        const int e = 64 * blk + lane;
+        const int q = e & 3, p = min((e >> 2) % PCH, npc - 1), sl = (e >> 2) / PCH;
+        __builtin_amdgcn_global_load_lds(
+            static_cast<const void *>(Bg + sl * plane + (size_t)ncK * (pc + p) + 64 * c + 16 * q),
+            (lds_void *)(B + 1024 * blk), 16, 0, 0);
+      }
+    }
+  };
 
   for (int I = I0; I < I1; I++) {
     const int irow = 64 * I + 16 * rg + lr;
@@ -375,40 +430,118 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
     __syncthreads();  // the previous tile is done with all buffers
     if (I > I0)
       reduce_tile(I - 1);
-    load_b(I);
-    store_b(I, 2);
-    __syncthreads();
     v4i a[S];
-    load_a(I, 2);
-    repack_a<S>(e, a);
-    // step st visits column tile c = I + st (GRAD: (I + st) mod nT, every tile)
-    const int nsteps = GRAD ? nT : nT - I;
-    for (int st = 0; st < nsteps; st++) {
-      const int cn = GRAD ? (I + st + 1) % nT : I + st + 1;  // the next step's tile
-      const int cur = st == 0 ? 2 : (st - 1) & 1, nxt = st & 1;
-      const bool more = st + 1 < nsteps;
-      if (more)
-        load_b(cn);
-      const signed char *B = Bs + cur * BUF;
+    if constexpr (ASYNC) {
+      // symmetric launch: column tiles c = I .. nT - 1; GRAD: every tile, c = (I + st) mod nT; the
+      // chunk of step st + 1 goes to buffer st & 1.  jbs holds byte
+      // offsets of the chunk's columns in At (0 past nc: those k meet zero phi digits); rows past nc
+      // read row 0 (their sums are multiplied by zero scales); only the symmetric launch's diagonal
+      // chunk is masked.
+      unsigned *jco = reinterpret_cast<unsigned *>(jbs);
+      const unsigned nb8 = 8u * (unsigned)n_basis;
+      const __amdgpu_buffer_rsrc_t rsrc =
+          __builtin_amdgcn_make_buffer_rsrc((void *)At, 0, (int)(nb8 * (unsigned)n_basis), 0x00020000);
+      const unsigned arowB = 8u * (unsigned)arow;
+      u2 el[16];
+      auto gather = [&](int buf) {
+        const uint4 *cp = reinterpret_cast<const uint4 *>(jco + 64 * buf + 16 * lk);
 #pragma unroll
-      for (int g = 0; g < NG; g++) {
-        v4i b[S];
-#pragma unroll
-        for (int sl = 0; sl < S; sl++)
-          b[sl] = *reinterpret_cast<const v4i *>(&B[(sl * PCH + 16 * (gp0 + g) + lr) * BROW + 16 * lk]);
-#pragma unroll
-        for (int sa = 0; sa < S; sa++)
-#pragma unroll
-          for (int sb = 0; sb < S - sa; sb++)
-            acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
+        for (int h = 0; h < 4; h++) {
+          const uint4 c4 = cp[h];
+          el[4 * h] = __builtin_amdgcn_raw_buffer_load_b64(rsrc, arowB + c4.x, 0, 0);
+          el[4 * h + 1] = __builtin_amdgcn_raw_buffer_load_b64(rsrc, arowB + c4.y, 0, 0);
+          el[4 * h + 2] = __builtin_amdgcn_raw_buffer_load_b64(rsrc, arowB + c4.z, 0, 0);
+          el[4 * h + 3] = __builtin_amdgcn_raw_buffer_load_b64(rsrc, arowB + c4.w, 0, 0);
+        }
+      };
+      auto col_off = [&](int j) { return j < nc ? nb8 * (unsigned)sib[j] : 0u; };
+      const int nsteps = GRAD ? nT : nT - I;
+      auto tile_of = [&](int st) { return GRAD ? (I + st) % nT : I + st; };
+      stage_b(I, 2);
+      if (nsteps > 1)
+        stage_b(tile_of(1), 0);
+      if (tid < 64) {
+        jco[128 + tid] = col_off(64 * I + tid);
+        if (nsteps > 1)
+          jco[tid] = col_off(64 * tile_of(1) + tid);
       }
-      if (more)
-        store_b(cn, nxt);
+      __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
-      if (more) {
-        load_a(cn, nxt);
-        repack_a<S>(e, a);
+      gather(2);
+      if (!GRAD) {
+#pragma unroll
+        for (int q = 0; q < 16; q++)  // P'(i, k) = 0 below the diagonal (k < i)
+          if (64 * I + 16 * lk + q < irow)
+            el[q] = u2{0u, 0u};
       }
+      repack_a_perm<S>(el, a);
+      for (int st = 0; st < nsteps; st++) {
+        const bool hn = st + 1 < nsteps, hnn = st + 2 < nsteps;
+        const int cur = st == 0 ? 2 : (st - 1) & 1, bn = st & 1, bnn = (st + 1) & 1;
+        if (hn)
+          gather(bn);  // in flight during this step's MFMAs
+        unsigned jr = 0;
+        if (hnn && tid < 64)
+          jr = col_off(64 * tile_of(st + 2) + tid);
+        __builtin_amdgcn_sched_barrier(0);
+        const signed char *B = Bs + cur * BUF;
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+          v4i b[S];
+#pragma unroll
+          for (int sl = 0; sl < S; sl++)
+            b[sl] = *reinterpret_cast<const v4i *>(&B[(sl * PCH + 16 * (gp0 + g) + lr) * RB + 16 * lk]);
+#pragma unroll
+          for (int sa = 0; sa < S; sa++)
+#pragma unroll
+            for (int sb = 0; sb < S - sa; sb++)
+              acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
+        }
+        if (hnn && tid < 64)
+          jco[64 * bnn + tid] = jr;
+        __builtin_amdgcn_s_waitcnt(0);  // chunk c + 1 staged, its P' gathers arrived
+        __syncthreads();
+        if (hnn)
+          stage_b(tile_of(st + 2), bnn);  // into the buffer this step's chunk was read from
+        if (hn)
+          repack_a_perm<S>(el, a);
+      }
+    } else {
+      load_b(I);
+      store_b(I, 2);
+      __syncthreads();
+      load_a(I, 2);
+      repack_a<S>(e, a);
+      // step st visits column tile c = I + st (GRAD: (I + st) mod nT, every tile)
+      const int nsteps = GRAD ? nT : nT - I;
+      for (int st = 0; st < nsteps; st++) {
+        const int cn = GRAD ? (I + st + 1) % nT : I + st + 1;  // the next step's tile
+        const int cur = st == 0 ? 2 : (st - 1) & 1, nxt = st & 1;
+        const bool more = st + 1 < nsteps;
+        if (more)
+          load_b(cn);
+        const signed char *B = Bs + cur * BUF;
+  #pragma unroll
+        for (int g = 0; g < NG; g++) {
+          v4i b[S];
+  #pragma unroll
+          for (int sl = 0; sl < S; sl++)
+            b[sl] = *reinterpret_cast<const v4i *>(&B[(sl * PCH + 16 * (gp0 + g) + lr) * RB + 16 * lk]);
+  #pragma unroll
+          for (int sa = 0; sa < S; sa++)
+  #pragma unroll
+            for (int sb = 0; sb < S - sa; sb++)
+              acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
+        }
+        if (more)
+          store_b(cn, nxt);
+        __syncthreads();
+        if (more) {
+          load_a(cn, nxt);
+          repack_a<S>(e, a);
+        }
+      }
+
     }
 
     // epilogue: v_mfma_i32_16x16x64_i8 leaves D(4 (lane / 16) + r, lane % 16) in acc[r], so
@@ -433,7 +566,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
         unsigned dw[S];
 #pragma unroll
         for (int sl = 0; sl < S; sl++)
-          dw[sl] = *reinterpret_cast<const unsigned *>(&BD[(sl * PCH + p) * BROW + 16 * rg + 4 * lk]);
+          dw[sl] = *reinterpret_cast<const unsigned *>(&BD[(sl * PCH + p) * RB + 16 * rg + 4 * lk]);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           double wv = (double)acc[S - 1][g][r];
@@ -492,7 +625,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
   }
 }
 
-template <int S, int NPG, int H, bool GRAD, int... Gs>
+template <int S, int NPG, int H, bool GRAD, bool ASYNC, int... Gs>
 __device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>, int ng, signed char *Bs, int *jbs,
                                                double *red, const BatchDesc &d, int t, int pc, int npc, int gp0,
                                                int I0, int I1, double *part, const int *sib, const int *map, const unsigned long long *At,
@@ -500,13 +633,13 @@ __device__ __forceinline__ void ozaki_dispatch(std::integer_sequence<int, Gs...>
                                                const double *binv, int n_basis, const double *bias, double *rho,
                                                int n_full, int nmb, int n_batches_all, int n_tile,
                                                const GradArgs &ga) {
-  ((ng == Gs ? ozaki_chunk<S, NPG, Gs, H, GRAD>(Bs, jbs, red, d, t, pc, npc, gp0, I0, I1, part, sib, map, At, ai, Bg, binv, n_basis,
+  ((ng == Gs ? ozaki_chunk<S, NPG, Gs, H, GRAD, ASYNC>(Bs, jbs, red, d, t, pc, npc, gp0, I0, I1, part, sib, map, At, ai, Bg, binv, n_basis,
                                              bias, rho, n_full, nmb, n_batches_all, n_tile, ga)
              : (void)0),
    ...);
 }
 
-template <int S, int NPG, int H, bool GRAD>
+template <int S, int NPG, int H, bool GRAD, bool ASYNC>
 __global__ __launch_bounds__(256 * H, 2) void rho1_ozaki_kernel(
     const BatchDesc *__restrict__ descs, const int *__restrict__ order, int *__restrict__ counters, int n_pos, int T,
     const int *__restrict__ sib_all, const int *__restrict__ map, const unsigned long long *__restrict__ adig,
@@ -515,9 +648,9 @@ __global__ __launch_bounds__(256 * H, 2) void rho1_ozaki_kernel(
     const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all, int n_tile,
     int items, const TileItem *__restrict__ titems, double *__restrict__ part, long long part_stride, GradArgs ga) {
   constexpr int PCH = 16 * H * NPG;
-  __shared__ __attribute__((aligned(16))) signed char Bs[3 * S * PCH * BROW];
+  __shared__ __attribute__((aligned(16))) signed char Bs[3 * S * PCH * (ASYNC ? BROW_ASYNC : BROW)];
   __shared__ int jbs[192];
-  __shared__ double red[4 * PCH];
+  __shared__ double red[(GRAD && ASYNC ? 16 : 4) * PCH];
   __shared__ int s_item;
   const int half = H == 2 ? (int)(threadIdx.x >> 8) : 0;
   int pos = blockIdx.x, t = blockIdx.y;
@@ -547,7 +680,7 @@ __global__ __launch_bounds__(256 * H, 2) void rho1_ozaki_kernel(
       const int ngt = (npc + 15) >> 4;       // groups of 16 points in this chunk
       const int n0 = H == 2 ? (ngt + 1) >> 1 : ngt;  // the first half takes the extra group
       const int ng = half ? ngt - n0 : n0, gp0 = half ? n0 : 0;
-      ozaki_dispatch<S, NPG, H, GRAD>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc,
+      ozaki_dispatch<S, NPG, H, GRAD, ASYNC>(std::make_integer_sequence<int, NPG + 1>{}, ng, Bs, jbs, red, d, t, pc, npc,
                                       gp0, I0, I1, tpart, sib, map, adig + nn, ai, Bg, binv, n_basis, bias, rho, n_full, nmb,
                                       n_batches_all, n_tile, ga);
     }
@@ -601,13 +734,14 @@ void prepare(Rho1Plan &P, OzakiState &st, const double *wave, hipStream_t s) {
   st.bytes = off[P.n_valid] + (long long)nvp * 8;
 }
 
-template <int S, int NPG, int H, bool GRAD>
+template <int S, int NPG, int H, bool GRAD, bool ASYNC = false>
 void launch(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s) {
   OzakiState &st = *P.ozaki;
   const Relabel &R = get_relabel(P);
   static int per_cu = -1;
   if (per_cu < 0) {
-    FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_ozaki_kernel<S, NPG, H, GRAD>, 256 * H, 0));
+    FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_ozaki_kernel<S, NPG, H, GRAD, ASYNC>,
+                                                          256 * H, 0));
     per_cu = std::max(per_cu, 1);
   }
   constexpr int PCH = 16 * H * NPG;
@@ -623,7 +757,7 @@ void launch(Rho1Plan &P, const Rho1Call &c, const LaunchCfg &cfg, hipStream_t s)
     lc.order = 0;  // the item list is already in launch-position order
   const LaunchArgs la = launch_args(P, lc, c.T, per_cu, s, n_pos);
   const GradArgs ga{c.gwave, c.gbias, c.grad, R.d_srow.ptr};
-  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG, H, GRAD>), la.grid, dim3(256 * H), 0, s, P.d_desc.ptr, la.order,
+  hipLaunchKernelGGL((rho1_ozaki_kernel<S, NPG, H, GRAD, ASYNC>), la.grid, dim3(256 * H), 0, s, P.d_desc.ptr, la.order,
                      la.counters, n_pos, c.T, R.d_sib.ptr, P.d_map.ptr, st.d_adig.ptr, st.d_ainv.ptr,
                      st.d_bdig.ptr, st.d_boff.ptr, st.d_binv.ptr, P.n_basis, c.bias, c.rho, P.n_full, P.nmb,
                      P.n_batches, c.n_tile, (int)items, ts ? ts->d_items.ptr : nullptr,
@@ -663,9 +797,13 @@ void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   // point-chunk width 16 H NPG: NPG at most the widest whose accumulators, prefetched digits and
   // staged B fit 256 registers per lane (two waves per SIMD) without spilling: 4 groups for 5 digits,
   // 5 for 4 (the default when some batch has more than 128 points).  cfg.npg = 2 selects the
-  // four-wave block (H = 1, 64-point passes, two blocks per CU).
+  // four-wave block (H = 1, 64-point passes, two blocks per CU); cfg.npg = 1 the same block with its
+  // phi chunks staged by direct global -> LDS loads and the P' gathers one step ahead.
   const LaunchCfg &cfg = P.cfg[S == 5 ? FRPA_PREC_OZAKI : FRPA_PREC_OZAKI4];
-  if (cfg.npg == 2) {
+  // (the buffer-resource gather needs the n_basis^2 elements of one perturbation under 4 GiB)
+  if (cfg.npg == 1 && (unsigned long long)n * n * 8 < (1ull << 32)) {
+    launch<S, 4, 1, GRAD, true>(P, c, cfg, s);
+  } else if (cfg.npg == 2 || cfg.npg == 1) {
     launch<S, 4, 1, GRAD>(P, c, cfg, s);
   } else if constexpr (S == 5) {
     launch<S, 4, 2, GRAD>(P, c, cfg, s);

@@ -133,7 +133,9 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
  * k workgroups (rounded down to a multiple of 8); points_per_wave = groups of 16 points per wave and
  * pass (FP64: 4, 8, 13; INT8 split S = 5: 4; S = 4: 4, 5; 0 = default), and for the INT8 split 2 =
  * four-wave workgroups over 64-point passes (two per CU, where the 8-wave workgroups over 128-point
- * passes fill a CU's LDS alone).  Results do not depend on it.  No counterpart in the reference
+ * passes fill a CU's LDS alone), 1 = those workgroups with the phi digits staged by direct
+ * global -> LDS loads and the P1 digit gathers issued one K step ahead through a buffer resource
+ * (n_basis^2 * 8 < 2^32; larger bases use 2).  Results do not depend on it.  No counterpart in the reference
  * (compile-time tiling). */
 int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave);
 

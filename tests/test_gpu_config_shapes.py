@@ -170,11 +170,11 @@ def test_rho1_1024atom_row_tile_items(shape1024):
     plan0 = api.Rho1Plan.from_problem(prob)
     ref = plan0.forward(w0, D["P1"], precision=api.OZAKI).clone()
     gref = [x.clone() for x in plan0.forward(w0, D["P1"], precision=api.OZAKI, gradient_wave=gw, grad_bias=gb)]
-    for persistent in (0, 1):
-        plan0.set_launch(api.OZAKI, 3, persistent, 2)
+    for persistent, npg in ((0, 2), (1, 2), (1, 1)):
+        plan0.set_launch(api.OZAKI, 3, persistent, npg)
         got = plan0.forward(w0, D["P1"], precision=api.OZAKI)
         r2, g2 = plan0.forward(w0, D["P1"], precision=api.OZAKI, gradient_wave=gw, grad_bias=gb)
-        print(f"[rho1 1024 atoms order 3 persistent {persistent}] identical {torch.equal(got, ref)}, "
+        print(f"[rho1 1024 atoms order 3 persistent {persistent} npg {npg}] identical {torch.equal(got, ref)}, "
               f"gradient identical {torch.equal(r2, gref[0]) and torch.equal(g2, gref[1])}", flush=True)
         assert torch.equal(got, ref)
         assert torch.equal(r2, gref[0]) and torch.equal(g2, gref[1])

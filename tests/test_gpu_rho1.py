@@ -167,12 +167,14 @@ def test_rho1_int8_split_nonsymmetric_multi_tile(cuda, oracle):
     _check(got, ref, rel=1e-10, what="rho1 int8 split, nonsymmetric P1, T = 3")
 
 
-@pytest.mark.parametrize("precision,npgs", [(api.FP64, (4, 8, 13)), (api.OZAKI, (4, 2)), (api.OZAKI4, (4, 5, 2))])
+@pytest.mark.parametrize("precision,npgs", [(api.FP64, (4, 8, 13)), (api.OZAKI, (4, 2, 1)),
+                                            (api.OZAKI4, (4, 5, 2, 1))])
 def test_rho1_launch_configs_bit_identical(cuda, oracle, monkeypatch, precision, npgs):
     """Every launch configuration the autotuner can choose (batch order, point passes as separate
     items, row-tile ranges of point passes as items -- here one row tile per item, FRPA_TILE_WORK = 3
     -- persistent grid with per-XCD work counters, points per pass, and for the INT8 split the
-    four-wave block over 64-point passes, npg = 2) gives the same bits; T = 2 exercises the persistent
+    four-wave block over 64-point passes, npg = 2, and that block with direct global -> LDS staging,
+    npg = 1) gives the same bits; T = 2 exercises the persistent
     claim of (position, perturbation) items; the autotuner keeps one of them."""
     monkeypatch.setenv("FRPA_TILE_WORK", "3")
     prob = synth.make_rho1_problem(14, 12.0, 101, (1, 230), (1, 150), config_k=13, zero_nc=(4,), zero_np=(8,))

@@ -639,8 +639,10 @@ def rooflines(r, path, D, w, tuned, workload):
         prof = PROFILED_KERNEL[(name, path)]
         if name == "rho1" and path == "fast":
             # the INT8 kernel's template arguments follow the chosen launch: npg 2 = four-wave workgroups
+            # with register-staged phi chunks, npg 1 = four-wave workgroups with direct-to-LDS staging
             m = re.search(r"npg=(\d+)", tuned.get(path) or "")
-            prof += "<5,4,1" if (m and int(m.group(1)) == 2) else "<5,4,2"
+            npg = int(m.group(1)) if m else 0
+            prof += {1: "<5,4,1,false,true>", 2: "<5,4,1,false,false>"}.get(npg, "<5,4,2,false,false>")
         tr = next((v for k, v in traffic.items() if k.startswith(prof)), None)
         if name == "far":
             from fast_rpa_eval_amd.multipole_tables import horner_fma_count

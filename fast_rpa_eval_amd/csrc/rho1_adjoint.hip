@@ -54,6 +54,10 @@ struct AdjState {
   DevBuf<AdjTask> d_tasks;
   DevBuf<double> d_hs;       // push mode: upper triangle of the relabelled H, (n_basis, n_basis, T)
   DevBuf<double> d_gga_cp;   // full first-order H: per point, C then phi of the batch's sorted rows
+  // full first-order H work items: (BatchDesc index, first row tile, end row tile), the items of a
+  // batch adjacent on one XCD (launch position 8 k + x takes item k of range x)
+  int n_gga_items = 0;
+  DevBuf<int4> d_gga_items;
 };
 
 void adj_release(AdjState *st) { delete st; }
@@ -352,6 +356,9 @@ void build_tiles(Rho1Plan &P, AdjState &st) {
 // with the push kernel's structure: tile pairs I <= J of the label-sorted rows, the points as the
 // contraction index in chunks of KCH (prefactors per chunk in LDS, C built while staging), both
 // products on FP64 MFMA into one accumulator, the upper triangle added with FP64 atomics into Hs.
+// A work item is a range of row tiles of one batch (its pairs I <= J for I in the range), the items
+// of a batch adjacent on one XCD; the operands of the next (pair, chunk) step are loaded into
+// registers while the current step's MFMAs run.
 struct GgaIn {
   const double *gwave, *hwave;                            // (ncp, 3, np) / (ncp, 6, np) per batch
   const int *avn;                                         // (n_atoms + 1, n_batches)
@@ -430,11 +437,13 @@ __global__ __launch_bounds__(256) void h1_gga_prep_kernel(const BatchDesc *__res
 // rows, points as the contraction index in chunks of KCH (both products in one accumulator), the
 // upper triangle added with FP64 atomics into the relabelled scratch of perturbation t.
 __global__ __launch_bounds__(256) void h1_gga_kernel(const BatchDesc *__restrict__ descs,
+                                                     const int4 *__restrict__ items,
                                                      const int *__restrict__ sib_all, const double *__restrict__ CP,
                                                      int n_basis, double *__restrict__ Ht) {
   constexpr int LDT = TILE + 1;
   __shared__ double Ci[KCH * LDT], Pi[KCH * LDT], Cj[KCH * LDT], Pj[KCH * LDT];
-  const BatchDesc d = descs[blockIdx.x];
+  const int4 item = items[blockIdx.x];  // rows tiles [item.y, item.z) of batch item.x
+  const BatchDesc d = descs[item.x];
   const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
   const double *CPb = CP + 2 * d.wave_off;
   const int *sib = sib_all + d.ib_off;
@@ -442,58 +451,89 @@ __global__ __launch_bounds__(256) void h1_gga_kernel(const BatchDesc *__restrict
   const int lr = lane & 15, lk = lane >> 4;
   const int wm = 32 * (w >> 1), wn = 32 * (w & 1);
   const int nT = (nc + TILE - 1) / TILE;
+  const int nck = (np + KCH - 1) / KCH;
   const int m = tid % TILE;
-  for (int I = 0; I < nT; I++) {
-    const int i_st = TILE * I + m;
-    for (int J = I; J < nT; J++) {
-      const int j_st = TILE * J + m;
-      v4d acc[2][2];
+  // steps (I, J, chunk) over the tile pairs I <= J and the point chunks; the operands of the next
+  // step are loaded into registers while this step's MFMAs run
+  constexpr int QN = TILE * KCH / 256;  // staged elements per thread and array
+  double nx[QN][4];
+  auto load_step = [&](int I, int J, int c) {
+    const int i_st = TILE * I + m, j_st = TILE * J + m;
+#pragma unroll
+    for (int u = 0; u < QN; u++) {
+      const int k = (tid + 256 * u) / TILE, p = KCH * c + k;
+      double ci = 0.0, fi = 0.0, cj = 0.0, fj = 0.0;
+      if (p < np) {
+        const double *col = CPb + (size_t)2 * ld * p;
+        if (i_st < nc) {
+          ci = col[i_st];
+          fi = col[ld + i_st];
+        }
+        if (j_st < nc) {
+          cj = col[j_st];
+          fj = col[ld + j_st];
+        }
+      }
+      nx[u][0] = ci;
+      nx[u][1] = fi;
+      nx[u][2] = cj;
+      nx[u][3] = fj;
+    }
+  };
+  auto advance = [&](int &I, int &J, int &c) {
+    if (++c == nck) {
+      c = 0;
+      if (++J == nT) {
+        ++I;
+        J = I;
+      }
+    }
+  };
+  v4d acc[2][2];
+  const int I1 = min(item.z, nT);
+  int I = item.y, J = item.y, c = 0;
+  if (I < I1)
+    load_step(I, J, c);
+  while (I < I1) {
+    __syncthreads();  // the previous step's MFMAs are done with the staged operands
+#pragma unroll
+    for (int u = 0; u < QN; u++) {
+      const int k = (tid + 256 * u) / TILE;
+      Ci[k * LDT + m] = nx[u][0];
+      Pi[k * LDT + m] = nx[u][1];
+      Cj[k * LDT + m] = nx[u][2];
+      Pj[k * LDT + m] = nx[u][3];
+    }
+    __syncthreads();
+    int nI = I, nJ = J, ncx = c;
+    advance(nI, nJ, ncx);
+    if (nI < I1)
+      load_step(nI, nJ, ncx);  // in flight during the MFMAs below
+    if (c == 0) {
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
           acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-      for (int p0 = 0; p0 < np; p0 += KCH) {
-        __syncthreads();  // the previous chunk's MFMAs are done with the staged operands
+    }
 #pragma unroll
-        for (int q = tid; q < TILE * KCH; q += 256) {
-          const int k = q / TILE, p = p0 + k;
-          double ci = 0.0, fi = 0.0, cj = 0.0, fj = 0.0;
-          if (p < np) {
-            const double *col = CPb + (size_t)2 * ld * p;
-            if (i_st < nc) {
-              ci = col[i_st];
-              fi = col[ld + i_st];
-            }
-            if (j_st < nc) {
-              cj = col[j_st];
-              fj = col[ld + j_st];
-            }
-          }
-          Ci[k * LDT + m] = ci;
-          Pi[k * LDT + m] = fi;
-          Cj[k * LDT + m] = cj;
-          Pj[k * LDT + m] = fj;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < KCH; kk += 4) {
-          const int o = (kk + lk) * LDT;
-          const double ca0 = Ci[o + wm + lr], ca1 = Ci[o + wm + 16 + lr];
-          const double pa0 = Pi[o + wm + lr], pa1 = Pi[o + wm + 16 + lr];
-          const double cb0 = Cj[o + wn + lr], cb1 = Cj[o + wn + 16 + lr];
-          const double pb0 = Pj[o + wn + lr], pb1 = Pj[o + wn + 16 + lr];
-          // D = (J rows) x (I rows): C(i) phi(j) + phi(i) C(j)
-          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb0, ca0, acc[0][0], 0, 0, 0);
-          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb0, pa0, acc[0][0], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb1, ca0, acc[0][1], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb1, pa0, acc[0][1], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb0, ca1, acc[1][0], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb0, pa1, acc[1][0], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb1, ca1, acc[1][1], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb1, pa1, acc[1][1], 0, 0, 0);
-        }
-      }
+    for (int kk = 0; kk < KCH; kk += 4) {
+      const int o = (kk + lk) * LDT;
+      const double ca0 = Ci[o + wm + lr], ca1 = Ci[o + wm + 16 + lr];
+      const double pa0 = Pi[o + wm + lr], pa1 = Pi[o + wm + 16 + lr];
+      const double cb0 = Cj[o + wn + lr], cb1 = Cj[o + wn + 16 + lr];
+      const double pb0 = Pj[o + wn + lr], pb1 = Pj[o + wn + 16 + lr];
+      // D = (J rows) x (I rows): C(i) phi(j) + phi(i) C(j)
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb0, ca0, acc[0][0], 0, 0, 0);
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb0, pa0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb1, ca0, acc[0][1], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb1, pa0, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb0, ca1, acc[1][0], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb0, pa1, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(pb1, ca1, acc[1][1], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cb1, pa1, acc[1][1], 0, 0, 0);
+    }
+    if (c == nck - 1) {
       // acc[a][b][r] = Hloc(i = 64 I + wm + 16 a + lr, j = 64 J + wn + 16 b + 4 r + lk)
 #pragma unroll
       for (int a = 0; a < 2; a++) {
@@ -509,6 +549,9 @@ __global__ __launch_bounds__(256) void h1_gga_kernel(const BatchDesc *__restrict
           }
       }
     }
+    I = nI;
+    J = nJ;
+    c = ncx;
   }
 }
 
@@ -566,6 +609,43 @@ void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const d
   check_launch("h1_tile_kernel");
 }
 
+// Work items of the full first-order H: per batch (in the caller's order, so that neighbouring
+// batches follow each other), ranges of row tiles of about `work` tile pairs (tile I has nT - I); the
+// items of a batch are adjacent in one XCD's range, so they read its staged operands from that L2.
+void build_gga_items(Rho1Plan &P, AdjState &st) {
+  int work = 8;
+  if (const char *e = std::getenv("FRPA_GGA_TILE_WORK"))
+    work = std::max(1, std::atoi(e));
+  std::vector<int> caller(P.n_valid);
+  for (int i = 0; i < P.n_valid; i++)
+    caller[i] = i;
+  std::stable_sort(caller.begin(), caller.end(), [&](int a, int b) { return P.h_desc[a].b < P.h_desc[b].b; });
+  std::vector<int4> seq;
+  for (int di : caller) {
+    const int nT = (P.h_desc[di].nc + TILE - 1) / TILE;
+    int I0 = 0, w = 0;
+    for (int I = 0; I < nT; I++) {
+      w += nT - I;
+      if (w >= work || I == nT - 1) {
+        seq.push_back(make_int4(di, I0, I + 1, 0));
+        I0 = I + 1;
+        w = 0;
+      }
+    }
+  }
+  const int n = (int)seq.size();
+  std::vector<int4> items(std::max(n, 1), make_int4(0, 0, 0, 0));
+  int start = 0;
+  for (int x = 0; x < N_XCD; x++) {
+    const int len = n / N_XCD + (x < n % N_XCD ? 1 : 0);
+    for (int k = 0; k < len; k++)
+      items[N_XCD * k + x] = seq[start + k];
+    start += len;
+  }
+  st.d_gga_items.upload(items);
+  st.n_gga_items = n;
+}
+
 void rho1_h1_gga(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,
                  const double *wave, const double *gwave, const double *hwave, const double *vsigma,
                  const double *gradient_rho, const double *pref1, const double *grho1, const double *vhg,
@@ -591,12 +671,14 @@ void rho1_h1_gga(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, 
   GgaIn in{gwave, hwave, avn, vsigma, v2rho2, v2rhosigma, v2sigma2, gradient_rho, pref1, grho1, vhg, rho1,
            partition_tab, n_atoms, j_atom_begin, j_coord, P.nmb, P.n_batches};
   st.d_gga_cp.grow((size_t)(2 * P.wave_len));
+  if (st.n_gga_items == 0)
+    build_gga_items(P, st);
   for (int t = 0; t < n_tile; t++) {
     hipLaunchKernelGGL(h1_gga_prep_kernel, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, R.d_srow.ptr, P.d_map.ptr,
                        wave, in, t, P.n_full, st.d_gga_cp.ptr);
     check_launch("h1_gga_prep_kernel");
-    hipLaunchKernelGGL(h1_gga_kernel, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, R.d_sib.ptr, st.d_gga_cp.ptr, n,
-                       st.d_hs.ptr + (size_t)n * n * t);
+    hipLaunchKernelGGL(h1_gga_kernel, dim3(st.n_gga_items), dim3(256), 0, s, P.d_desc.ptr, st.d_gga_items.ptr,
+                       R.d_sib.ptr, st.d_gga_cp.ptr, n, st.d_hs.ptr + (size_t)n * n * t);
     check_launch("h1_gga_kernel");
   }
   hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, n_tile), dim3(256), 0, s, st.d_hs.ptr,

@@ -300,6 +300,12 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
   }
   P->d_ibasis0.upload(P->h_ibasis0);
   P->d_map.upload(map, (size_t)n_valid_points);
+  // default launches: what the autotuner picks on the 60k-batch, 1024-atom workload (FP64: XCD-aware
+  // spatial order, persistent, 13 point groups per wave; INT8 split: row-tile ranges of point passes
+  // as work items, persistent, four-wave workgroups with direct global -> LDS staging)
+  P->cfg[FRPA_PREC_FP64] = LaunchCfg{1, 1, 13};
+  P->cfg[FRPA_PREC_OZAKI] = LaunchCfg{3, 1, 1};
+  P->cfg[FRPA_PREC_OZAKI4] = LaunchCfg{3, 1, 1};
   return P.release();
 }
 

@@ -135,7 +135,8 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
  * four-wave workgroups over 64-point passes (two per CU, where the 8-wave workgroups over 128-point
  * passes fill a CU's LDS alone), 1 = those workgroups with the phi digits staged by direct
  * global -> LDS loads and the P1 digit gathers issued one K step ahead through a buffer resource
- * (n_basis^2 * 8 < 2^32; larger bases use 2).  Results do not depend on it.  No counterpart in the reference
+ * (n_basis^2 * 8 < 2^32; larger bases use 2).  Results do not depend on it.  A new plan starts with
+ * FP64 (1, 1, 13) and the INT8 split (3, 1, 1), the autotuner's choices on a 60k-batch workload.  No counterpart in the reference
  * (compile-time tiling). */
 int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave);
 

@@ -291,6 +291,46 @@ def bench_h1_gga(steps, warmup, no_cpu, threads, jb=1, jc=1):
         res["cpu_baseline"] = {"value": float(sub.n_point_batches.sum()) / cpu_s, "unit": "points/s",
                                "cores": threads, "kind": "port",
                                "sample": f"{len(pick)} random batches ({int(sub.n_point_batches.sum())} points)"}
+
+    # bias part 2 of the first-order H (evaluate_first_order_H_bias_part2) for the same atom: random
+    # H_times_psi and vrho; HBM-bound (H_times_psi, phi and dphi read once, the atom's rows of dphi, the
+    # point data, two FP64 atomics per (row, atom row) element)
+    hpsi, vrho = U(wl), U(nmb * nb)
+
+    def run_b2():
+        plan.h1_bias_part2(wave, gwave, hpsi, vrho, xc["vsigma"], xc["gradient_rho"], xc["partition_tab"], avn,
+                           prob.n_atoms, jb, j_coord=jc, H=H)
+    for _ in range(max(warmup, 1)):
+        run_b2()
+    torch.cuda.synchronize()
+    ev = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run_b2()
+        e1.record()
+        ev.append((e0, e1))
+    torch.cuda.synchronize()
+    ms2 = float(np.mean([x.elapsed_time(y) for x, y in ev]))
+    b2_bytes = float(np.sum((5 * 8 * nc * npt + 40 * npt + 32 * nc * nv)[nv > 0]))
+    b2_flops = float(np.sum((2 * nc * nv * npt + 12 * nc * npt)[nv > 0]))
+    res["bias_part2"] = {"ms": ms2, "algorithmic_bytes": b2_bytes, "algorithmic_flops": b2_flops,
+                         "roofline": {"kernel": "frpa::h1_bias2_kernel", "bound": "hbm",
+                                      "achieved": b2_bytes / (ms2 / 1e3) / 1e9, "peak": PEAK_HBM, "unit": "GB/s",
+                                      "frac": b2_bytes / (ms2 / 1e3) / 1e9 / PEAK_HBM}}
+    if not no_cpu:
+        hp_s = blocks(hpsi, 1)
+        vr_s = per_batch(vrho, 1).reshape(nmb2, len(pick), order="F")
+        vs_s = per_batch(xc["vsigma"], 3).reshape(3, nmb2, len(pick), order="F")
+        ref2 = oracle.h1_bias2(sub, 1, jb, jb, jc, np.zeros((prob.n_basis, prob.n_basis, 1), order="F"), vr_s, vs_s,
+                               arr["gradient_rho"], hp_s, arr["gwave"], arr["partition_tab"])
+        Hb = splan.h1_bias_part2(api.to_device(sub.wave), api.to_device(arr["gwave"]), api.to_device(hp_s),
+                                 api.to_device(vr_s), api.to_device(vs_s), api.to_device(arr["gradient_rho"]),
+                                 api.to_device(arr["partition_tab"]), api.to_device(sub.atom_valid_n_compute_c_batches()),
+                                 prob.n_atoms, jb, j_coord=jc)
+        got2 = Hb.cpu().numpy().reshape(prob.n_basis, prob.n_basis, order="F")
+        res["bias_part2"]["vs_oracle_rel"] = float(np.abs(got2 - ref2[:, :, 0]).max() / max(np.abs(ref2).max(), 1e-300))
+    log(f"H1 bias part 2: {res['bias_part2']}")
     return res
 
 

@@ -50,6 +50,7 @@ def lib():
         L.frpa_far_plan_set_near.argtypes = [p, i, p, p, p, p]
         L.frpa_rho1_bias_part2.argtypes = [p, i, i, i, i, i, p, p, p, p, p, p]
         L.frpa_rho1_bias_inline.argtypes = [p, i, i, i, i, i, p, p, p, p, p, p]
+        L.frpa_rho1_h1_bias_part2.argtypes = [p, i, i, i, i, i] + [p] * 10
         L.frpa_rho1_h1.argtypes = [p, i, i, i, i, i] + [p] * 16
         L.frpa_near_forward.argtypes = [p, i, i, i, i, i, p, p, p, p, p, p, p]
         L.frpa_far_forward_noalloc.argtypes = [p, i, i, i, i, i, p, p, p, p, p, p, p, p]

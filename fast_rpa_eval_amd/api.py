@@ -175,6 +175,21 @@ class Rho1Plan:
             _dp(gradient_wave), _dp(density_matrix), _dp(bias), _stream(stream)), "frpa_rho1_bias_inline")
         return bias
 
+    def h1_bias_part2(self, wave, gradient_wave, H_times_psi, vrho, vsigma, gradient_rho, partition_tab,
+                      atom_valid_n_compute_c_batches, n_atoms, j_atom_begin, n_tile=1, j_coord=1, T=None, H=None,
+                      stream=None):
+        """Bias part 2 of the first-order H (evaluate_first_order_H_bias_part2.cpp:13-218), added into H
+        (n_basis^2 * T, zeros if None); device tensors in the reference's layouts (see
+        frpa_rho1_h1_bias_part2)."""
+        T = n_tile if T is None else T
+        if H is None:
+            H = torch.zeros(self.n_basis * self.n_basis * T, dtype=torch.float64, device="cuda")
+        check(lib().frpa_rho1_h1_bias_part2(
+            self.handle, T, j_atom_begin, n_tile, j_coord, n_atoms, _dp(atom_valid_n_compute_c_batches), _dp(wave),
+            _dp(gradient_wave), _dp(H_times_psi), _dp(vrho), _dp(vsigma), _dp(gradient_rho), _dp(partition_tab), _dp(H),
+            _stream(stream)), "frpa_rho1_h1_bias_part2")
+        return H
+
     def h1(self, wave, gradient_wave, hessian_wave, xc, atom_valid_n_compute_c_batches, n_atoms, j_atom_begin,
            n_tile=1, j_coord=1, T=None, H=None, stream=None):
         """Full first-order H (LDA + GGA terms of evaluate_first_order_H), accumulated into H

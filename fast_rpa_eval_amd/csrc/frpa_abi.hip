@@ -749,6 +749,19 @@ int frpa_rho1_bias_inline(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_b
   });
 }
 
+int frpa_rho1_h1_bias_part2(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord,
+                            int n_atoms, const int *atom_valid_n_compute_c_batches, const double *wave,
+                            const double *gradient_wave, const double *H_times_psi, const double *vrho_batches,
+                            const double *vsigma_batches, const double *gradient_rho, const double *partition_tab,
+                            double *first_order_H_tile, void *stream) {
+  return guarded([&] {
+    FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
+    rho1_h1_bias_part2(*reinterpret_cast<Rho1Plan *>(plan), atom_tile_size, j_atom_begin, n_tile, j_coord, n_atoms,
+                       atom_valid_n_compute_c_batches, wave, gradient_wave, H_times_psi, vrho_batches, vsigma_batches,
+                       gradient_rho, partition_tab, first_order_H_tile, as_stream(stream));
+  });
+}
+
 int frpa_rho1_h1(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord, int n_atoms,
                  const int *atom_valid_n_compute_c_batches, const double *wave, const double *gradient_wave,
                  const double *hessian_wave, const double *vsigma_batches, const double *gradient_rho,

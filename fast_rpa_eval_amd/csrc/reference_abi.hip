@@ -41,7 +41,7 @@ struct RefState {
   DevBuf<double> dm0;       // density_matrix of init_dfpt_device_data (device_data.cu:328)
   DevBuf<double> bias_inl;  // bias input plus the v6 inline bias
   // per-call arrays of the first-order H entry point
-  DevBuf<double> h_hw, h_vs, h_v2rr, h_v2rs, h_v2ss, h_grho, h_pref1, h_grho1, h_vhg, h_rho1, h_w, h_H;
+  DevBuf<double> h_hw, h_vs, h_v2rr, h_v2rs, h_v2ss, h_grho, h_pref1, h_grho1, h_vhg, h_rho1, h_w, h_H, h_vr;
   DevBuf<int> avn;  // atom_valid_n_compute_c_batches (n_atoms + 1, n_batches)
   DevBuf<double> coord_points;
   // far per-call buffers
@@ -356,7 +356,8 @@ void free_device_data() {
   for (DevBuf<double> *b : {&S.f_spl, &S.wave, &S.gwave, &S.P1, &S.bias, &S.gbias, &S.dm, &S.dm0, &S.bias_inl,
                             &S.h_hw, &S.h_vs,
                             &S.h_v2rr, &S.h_v2rs, &S.h_v2ss, &S.h_grho, &S.h_pref1, &S.h_grho1, &S.h_vhg, &S.h_rho1,
-                            &S.h_w, &S.h_H, &S.rho, &S.grad, &S.coord_points, &S.f_cp, &S.f_mrsq, &S.f_opr, &S.f_M,
+                            &S.h_w, &S.h_H, &S.h_vr, &S.rho, &S.grad, &S.coord_points, &S.f_cp, &S.f_mrsq, &S.f_opr,
+                            &S.f_M,
                             &S.f_dv})
     b->release();
   S.avn.release();
@@ -661,6 +662,49 @@ void evaluate_first_order_h_reduce_memory_c_v2_(
                                 *j_coord_, S.n_atoms, S.avn.ptr, S.wave.ptr, S.gwave.ptr, S.h_hw.ptr, S.h_vs.ptr,
                                 S.h_grho.ptr, S.h_pref1.ptr, S.h_grho1.ptr, S.h_vhg.ptr, S.h_rho1.ptr, S.h_v2rr.ptr,
                                 S.h_v2rs.ptr, S.h_v2ss.ptr, S.h_w.ptr, S.h_H.ptr, s);
+    FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
+    FRPA_HIP(hipMemcpyAsync(first_order_H_tile_ptr, S.h_H.ptr, nbas * nbas * T * 8, hipMemcpyDeviceToHost, s));
+    FRPA_HIP(hipStreamSynchronize(s));
+  });
+}
+
+void evaluate_first_order_h_bias_part2_(
+    int *j_atom_begin_, int *j_atom_end_, int *n_full_points_, int *j_coord_, int *n_basis_, int *n_atoms_,
+    int *n_max_compute_ham_, int *n_max_batch_size_, int *n_my_batches_work_, int *n_centers_basis_I_,
+    int *n_point_batches_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *n_point_batches_prefix_sum_ptr,
+    int *i_valid_point_2_i_full_points_map_ptr, double *vrho_batches_ptr, double *vsigma_batches_ptr,
+    double *gradient_rho_ptr, double *H_times_psi_batches_compress_ptr, int *i_batch_2_wave_offset_ptr,
+    double *wave_batches_compress_ptr, double *gradient_wave_batches_compress_ptr, double *first_order_H_tile_ptr,
+    double *H_prefactor1_batches_tile_ptr, double *first_order_gradient_rho_batches_tile_ptr,
+    double *v_hartree_gradient_ptr, double *first_order_rho_ptr, double *v2rho2_batches_ptr,
+    double *v2rhosigma_batches_ptr, double *v2sigma2_batches_ptr, double *partition_tab_ptr, int *not_use) {
+  run_or_die("evaluate_first_order_h_bias_part2_", [&] {
+    RefState &S = ref();
+    FRPA_REQUIRE(S.data && S.rho1, FRPA_ERR_STATE,
+                 "init_dfpt_device_data must be called first (the wave arrays are used from the device)");
+    FRPA_REQUIRE(*n_my_batches_work_ == S.n_batches && *n_full_points_ == S.n_full && *n_basis_ == S.n_basis &&
+                     *n_max_batch_size_ == S.nmb && *n_atoms_ == S.n_atoms,
+                 FRPA_ERR_INVALID, "sizes differ from init_dfpt_device_data");
+    const int n_tile = *j_atom_end_ - *j_atom_begin_ + 1;
+    FRPA_REQUIRE(n_tile >= 1 && n_tile <= S.T, FRPA_ERR_INVALID,
+                 "j_atom_end - j_atom_begin + 1 = %d outside [1, ATOM_TILE_SIZE = %d]", n_tile, S.T);
+    hipStream_t s = S.stream;
+    const size_t nb = S.n_batches, nmb = S.nmb, nf = S.n_full, T = S.T, nbas = S.n_basis;
+    auto up = [&](DevBuf<double> &b, const double *h, size_t n) {
+      b.grow(n);
+      if (n)
+        FRPA_HIP(hipMemcpyAsync(b.ptr, h, n * 8, hipMemcpyHostToDevice, s));
+    };
+    up(S.h_hw, H_times_psi_batches_compress_ptr, S.wave.n);  // the hessian-wave buffer holds H_times_psi here
+    up(S.h_vr, vrho_batches_ptr, nmb * nb);
+    up(S.h_vs, vsigma_batches_ptr, 3 * nmb * nb);
+    up(S.h_grho, gradient_rho_ptr, 3 * nf);
+    up(S.h_w, partition_tab_ptr, nf);
+    up(S.h_H, first_order_H_tile_ptr, nbas * nbas * T);
+    const int rc = frpa_rho1_h1_bias_part2(reinterpret_cast<frpa_rho1_plan *>(S.rho1.get()), S.T, *j_atom_begin_,
+                                           n_tile, *j_coord_, S.n_atoms, S.avn.ptr, S.wave.ptr, S.gwave.ptr,
+                                           S.h_hw.ptr, S.h_vr.ptr, S.h_vs.ptr, S.h_grho.ptr, S.h_w.ptr, S.h_H.ptr, s);
     FRPA_REQUIRE(rc == FRPA_OK, rc, "%s", frpa_last_error());
     FRPA_HIP(hipMemcpyAsync(first_order_H_tile_ptr, S.h_H.ptr, nbas * nbas * T * 8, hipMemcpyDeviceToHost, s));
     FRPA_HIP(hipStreamSynchronize(s));

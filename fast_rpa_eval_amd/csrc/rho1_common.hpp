@@ -182,6 +182,13 @@ void rho1_bias_part2(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coo
                      const double *wave, const double *gwave, const double *DM, double *bias, hipStream_t s,
                      int sides = 2, bool accumulate = false);
 
+// Bias part 2 of the first-order H (evaluate_first_order_H_bias_part2.cpp:13-218) for the atoms
+// j_atom_begin .. + n_tile - 1, added into H (n_basis, n_basis, T) (rho1_h1_bias2.hip).
+void rho1_h1_bias_part2(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,
+                        const double *wave, const double *gwave, const double *hpsi, const double *vrho,
+                        const double *vsigma, const double *gradient_rho, const double *partition_tab, double *H,
+                        hipStream_t s);
+
 // Full first-order H (LDA + GGA terms of evaluate_first_order_H.cpp:13-222) for the perturbations
 // t < n_tile (atoms j_atom_begin + t), accumulated into H (n_basis, n_basis, T) (rho1_adjoint.hip).
 void rho1_h1_gga(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,

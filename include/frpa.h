@@ -223,6 +223,26 @@ int frpa_rho1_bias_inline(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_b
                           const double *gradient_wave, const double *density_matrix, double *first_order_rho_bias,
                           void *stream);
 
+/*
+ * Bias part 2 of the first-order H (fast_rap_src_dfpt_gpu_eval/evaluate_first_order_H_bias_part2.cpp:
+ * 13-218), ADDED to first_order_H_tile for the atoms j = j_atom_begin .. + n_tile - 1 (t = j -
+ * j_atom_begin): per batch, with V the compute rows of atom j, w = partition_tab(q),
+ *   C(i, p) = w H_times_psi(i, p) + w vrho(p) phi(i, p) + sum_c 2 w vsigma(0, p) gradient_rho(c, q) d_c phi(i, p)
+ *   L(i, k) = sum_p C(i, p) (-d_{j_coord} phi(k, p)),  k in V
+ *   first_order_H(ib_i, ib_k, t) += L(i, k),  first_order_H(ib_k, ib_i, t) += L(i, k).
+ * Device arrays: atom_valid_n_compute_c_batches (n_atoms + 1, n_batches) int; wave and gradient_wave
+ * as for frpa_rho1_forward; H_times_psi in the wave's layout; vrho_batches (n_max_batch_size,
+ * n_batches); vsigma_batches (3, n_max_batch_size, n_batches) (component 0 is read); gradient_rho
+ * (3, n_full_points); partition_tab (n_full_points); first_order_H_tile (n_basis, n_basis,
+ * atom_tile_size).  n_tile <= min(atom_tile_size, 8), j_coord 1..3.  FP64; the scatter is FP64
+ * atomics.
+ */
+int frpa_rho1_h1_bias_part2(frpa_rho1_plan *plan, int atom_tile_size, int j_atom_begin, int n_tile, int j_coord,
+                            int n_atoms, const int *atom_valid_n_compute_c_batches, const double *wave,
+                            const double *gradient_wave, const double *H_times_psi, const double *vrho_batches,
+                            const double *vsigma_batches, const double *gradient_rho, const double *partition_tab,
+                            double *first_order_H_tile, void *stream);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Far-field Hartree multipole sum                                                            */
 /* ------------------------------------------------------------------------------------------ */
@@ -487,6 +507,23 @@ void evaluate_first_order_h_reduce_memory_c_v2_(
     double *first_order_gradient_rho_batches_tile_ptr, double *v_hartree_gradient_ptr, double *first_order_rho_ptr,
     double *v2rho2_batches_ptr, double *v2rhosigma_batches_ptr, double *v2sigma2_batches_ptr,
     double *partition_tab_ptr, int *not_use);
+
+/* fast_rap_src_dfpt_gpu_eval/evaluate_first_order_H_bias_part2.cpp:13-52 (a CPU function there; same
+ * name, C linkage, 33 pointer arguments).  Runs frpa_rho1_h1_bias_part2 on the GPU with the wave and
+ * gradient wave that init_dfpt_device_data put on the device (which must come first); uploads
+ * H_times_psi, vrho, vsigma, gradient_rho, partition_tab and first_order_H_tile, accumulates and
+ * copies first_order_H_tile back.  The arguments the reference does not read are not read here. */
+void evaluate_first_order_h_bias_part2_(
+    int *j_atom_begin_, int *j_atom_end_, int *n_full_points_, int *j_coord_, int *n_basis_, int *n_atoms_,
+    int *n_max_compute_ham_, int *n_max_batch_size_, int *n_my_batches_work_, int *n_centers_basis_I_,
+    int *n_point_batches_ptr, int *n_compute_c_batches_ptr, int *i_basis_batches_ptr,
+    int *atom_valid_n_compute_c_batches_ptr, int *n_point_batches_prefix_sum_ptr,
+    int *i_valid_point_2_i_full_points_map_ptr, double *vrho_batches_ptr, double *vsigma_batches_ptr,
+    double *gradient_rho_ptr, double *H_times_psi_batches_compress_ptr, int *i_batch_2_wave_offset_ptr,
+    double *wave_batches_compress_ptr, double *gradient_wave_batches_compress_ptr, double *first_order_H_tile_ptr,
+    double *H_prefactor1_batches_tile_ptr, double *first_order_gradient_rho_batches_tile_ptr,
+    double *v_hartree_gradient_ptr, double *first_order_rho_ptr, double *v2rho2_batches_ptr,
+    double *v2rhosigma_batches_ptr, double *v2sigma2_batches_ptr, double *partition_tab_ptr, int *not_use);
 
 /* fast_rap_src_dfpt_gpu_eval/sum_up_whole_potential.h:50-95.
  * Both branches, as the reference (sum_up_whole_potential.cu:641-769): uploads the per-call arrays

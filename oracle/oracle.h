@@ -28,6 +28,11 @@
  *   oracle_rho_bias_inline           central differences of sum_ik phi_i DM_ik phi_k with only the
  *                                    first factor moving with atom j; inline(DM) + inline(DM^T)
  *                                    equals bias part 2 of DM.
+ *   oracle_evaluate_first_order_h_bias_part2
+ *                                    central differences of M(i, k) = sum_p C(i, p) phi_k(p) under a
+ *                                    displacement of atom j (only its functions phi_k move); the
+ *                                    vrho / vsigma prefactors equal the derivative of a model
+ *                                    functional f(rho, sigma) along phi_i (central differences in eps).
  *   oracle_evaluate_first_order_h    adjoint identity <v, rho1(P)> = <H1(v), P>, exact
  *                                    d rho1 / d P_ij = phi_i phi_j, GGA terms against an independent
  *                                    numpy restatement on tiny inputs.
@@ -111,6 +116,17 @@ void oracle_rho_bias_inline(int atom_tile_size, int n_my_batches_work, int j_ato
                             const int *atom_valid_n_compute_c_batches, const int *i_batch_2_wave_offset,
                             const double *wave_batches_compress, const double *gradient_wave_batches_compress,
                             double *first_order_rho_bias, const double *density_matrix);
+
+/* fast_rap_src_dfpt_gpu_eval/evaluate_first_order_H_bias_part2.cpp:13-218 (ADDS into
+ * first_order_H_tile; the arguments the reference does not read are omitted) */
+void oracle_evaluate_first_order_h_bias_part2(
+    int atom_tile_size, int j_atom_begin, int j_atom_end, int n_full_points, int j_coord, int n_basis, int n_atoms,
+    int n_max_batch_size, int n_my_batches_work, int n_centers_basis_I, const int *n_point_batches,
+    const int *n_compute_c_batches, const int *i_basis_batches, const int *atom_valid_n_compute_c_batches,
+    const int *n_point_batches_prefix_sum, const int *i_valid_point_2_i_full_points_map, const double *vrho_batches,
+    const double *vsigma_batches, const double *gradient_rho, const double *H_times_psi_batches_compress,
+    const int *i_batch_2_wave_offset, const double *wave_batches_compress,
+    const double *gradient_wave_batches_compress, double *first_order_H_tile, const double *partition_tab);
 
 /* Number of OpenMP threads used by the two grid loops (rho1 over batches, far field over points);
  * 1 reproduces the reference's single-threaded build.  Returns the previous value. */

@@ -34,6 +34,7 @@ def lib():
         L.oracle_sum_up_near.argtypes = [i] * 9 + [p] * 15
         L.oracle_generate_first_order_rho_bias_part2.argtypes = [i] * 9 + [p] * 9
         L.oracle_rho_bias_inline.argtypes = [i] * 9 + [p] * 9
+        L.oracle_evaluate_first_order_h_bias_part2.argtypes = [i] * 10 + [p] * 15
         d = ctypes.c_double
         L.oracle_ylm.argtypes = [i, d, d, d, d, p]
         L.oracle_set_threads.argtypes = [i]
@@ -200,4 +201,20 @@ def h1(prob, T, j_atom_begin, j_atom_end, j_coord, H, arrays):
         _p(a["grad_rho1"], np.float64), _p(a["v_hartree_gradient"], np.float64),
         _p(a["first_order_rho"], np.float64), _p(a["v2rho2"], np.float64), _p(a["v2rhosigma"], np.float64),
         _p(a["v2sigma2"], np.float64), _p(a["partition_tab"], np.float64))
+    return H
+
+
+def h1_bias2(prob, T, j_atom_begin, j_atom_end, j_coord, H, vrho, vsigma, gradient_rho, hpsi, gwave, partition_tab):
+    """evaluate_first_order_h_bias_part2 on the CPU, ADDED to H (n_basis, n_basis, T) F; vrho (nmb, nb),
+    vsigma (3, nmb, nb), gradient_rho (3, n_full), hpsi like the wave, gwave 3x the wave."""
+    lib().oracle_evaluate_first_order_h_bias_part2(
+        T, j_atom_begin, j_atom_end, prob.n_full_points, j_coord, prob.n_basis, prob.n_atoms,
+        prob.n_max_batch_size, prob.n_batches, prob.n_centers_basis_I,
+        _p(prob.n_point_batches, np.int32), _p(prob.n_compute_c_batches, np.int32),
+        _p(prob.dense_i_basis_batches(), np.int32), _p(prob.atom_valid_n_compute_c_batches(), np.int32),
+        _p(prob.n_point_batches_prefix_sum, np.int32), _p(prob.i_valid_point_2_i_full_points_map, np.int32),
+        _p(np.asfortranarray(vrho), np.float64), _p(np.asfortranarray(vsigma), np.float64),
+        _p(np.asfortranarray(gradient_rho), np.float64), _p(hpsi, np.float64), _p(prob.wave_offset_i32(), np.int32),
+        _p(prob.wave, np.float64), _p(gwave, np.float64), _p(H, np.float64),
+        _p(np.asfortranarray(partition_tab), np.float64))
     return H

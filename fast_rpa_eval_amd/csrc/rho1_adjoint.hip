@@ -166,9 +166,12 @@ __global__ __launch_bounds__(256) void h1_tile_kernel(const AdjTask *__restrict_
       }
 }
 
-// Default mode: one workgroup per batch; local tile pairs (I <= J) of its sorted compute rows;
-// atomics into the upper triangle of the relabelled scratch Hs.
+// Default mode: a work item is a range of row tiles of one batch (its local tile pairs I <= J of the
+// sorted compute rows, I in the range), the items of a batch adjacent on one XCD; atomics into the
+// upper triangle of the relabelled scratch Hs.  The operands of the next (pair, point chunk) step
+// are loaded into registers while the current step's MFMAs run.
 __global__ __launch_bounds__(256) void h1_push_kernel(const BatchDesc *__restrict__ descs,
+                                                      const int4 *__restrict__ items,
                                                       const int *__restrict__ sib_all, const int *__restrict__ srow_all,
                                                       const int *__restrict__ map, const double *__restrict__ wave,
                                                       const double *__restrict__ g, int n_full, int n_basis,
@@ -176,8 +179,8 @@ __global__ __launch_bounds__(256) void h1_push_kernel(const BatchDesc *__restric
   constexpr int LDT = TILE + 1;
   __shared__ double As[KCH * LDT];  // rows of tile I times g
   __shared__ double Bs[KCH * LDT];  // rows of tile J
-  __shared__ double gk[KCH];
-  const BatchDesc d = descs[blockIdx.x];
+  const int4 item = items[blockIdx.x];  // row tiles [item.y, item.z) of batch item.x
+  const BatchDesc d = descs[item.x];
   const int t = blockIdx.y;
   const int nc = d.nc, np = d.np, ld = (nc + 3) & ~3;
   const double *W = wave + d.wave_off;
@@ -189,47 +192,74 @@ __global__ __launch_bounds__(256) void h1_push_kernel(const BatchDesc *__restric
   const int lr = lane & 15, lk = lane >> 4;
   const int wm = 32 * (w >> 1), wn = 32 * (w & 1);
   const int nT = (nc + TILE - 1) / TILE;
-  // this thread's staged rows (fixed over the K loop): m = tid % 64
-  const int m = tid % TILE;
-  for (int I = 0; I < nT; I++) {
-    const int i_st = TILE * I + m;
-    const int ri = i_st < nc ? srow[i_st] : -1;
-    for (int J = I; J < nT; J++) {
-      const int j_st = TILE * J + m;
-      const int rj = j_st < nc ? srow[j_st] : -1;
-      v4d acc[2][2];
+  const int nck = (np + KCH - 1) / KCH;
+  const int m = tid % TILE;  // this thread's staged row within a tile
+  constexpr int QN = TILE * KCH / 256;
+  double nx[QN][2];
+  auto load_step = [&](int I, int J, int c) {
+    const int i_st = TILE * I + m, j_st = TILE * J + m;
+    const int ri = i_st < nc ? srow[i_st] : -1, rj = j_st < nc ? srow[j_st] : -1;
+#pragma unroll
+    for (int u = 0; u < QN; u++) {
+      const int k = (tid + 256 * u) / TILE, p = KCH * c + k;
+      double a = 0.0, b = 0.0;
+      if (p < np) {
+        if (ri >= 0)
+          a = W[ri + (size_t)ld * p] * gt[map[d.pstart + p]];
+        if (rj >= 0)
+          b = W[rj + (size_t)ld * p];
+      }
+      nx[u][0] = a;
+      nx[u][1] = b;
+    }
+  };
+  auto advance = [&](int &I, int &J, int &c) {
+    if (++c == nck) {
+      c = 0;
+      if (++J == nT) {
+        ++I;
+        J = I;
+      }
+    }
+  };
+  v4d acc[2][2];
+  const int I1 = min(item.z, nT);
+  int I = item.y, J = item.y, c = 0;
+  if (I < I1)
+    load_step(I, J, c);
+  while (I < I1) {
+    __syncthreads();  // the previous step's MFMAs are done with the staged operands
+#pragma unroll
+    for (int u = 0; u < QN; u++) {
+      const int k = (tid + 256 * u) / TILE;
+      As[k * LDT + m] = nx[u][0];
+      Bs[k * LDT + m] = nx[u][1];
+    }
+    __syncthreads();
+    int nI = I, nJ = J, ncx = c;
+    advance(nI, nJ, ncx);
+    if (nI < I1)
+      load_step(nI, nJ, ncx);  // in flight during the MFMAs below
+    if (c == 0) {
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
           acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-      for (int p0 = 0; p0 < np; p0 += KCH) {
-        __syncthreads();
-        if (tid < KCH) {
-          const int p = p0 + tid;
-          gk[tid] = p < np ? gt[map[d.pstart + p]] : 0.0;
-        }
-        __syncthreads();
+    }
 #pragma unroll
-        for (int q = tid; q < TILE * KCH; q += 256) {
-          const int k = q / TILE, p = p0 + k;
-          As[k * LDT + m] = (ri >= 0 && p < np) ? W[ri + (size_t)ld * p] * gk[k] : 0.0;
-          Bs[k * LDT + m] = (rj >= 0 && p < np) ? W[rj + (size_t)ld * p] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < KCH; kk += 4) {
-          const double a0 = As[(kk + lk) * LDT + wm + lr];
-          const double a1 = As[(kk + lk) * LDT + wm + 16 + lr];
-          const double b0 = Bs[(kk + lk) * LDT + wn + lr];
-          const double b1 = Bs[(kk + lk) * LDT + wn + 16 + lr];
-          // D = (J rows) x (I rows): the I row runs along lane % 16 in the result
-          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
-        }
-      }
+    for (int kk = 0; kk < KCH; kk += 4) {
+      const double a0 = As[(kk + lk) * LDT + wm + lr];
+      const double a1 = As[(kk + lk) * LDT + wm + 16 + lr];
+      const double b0 = Bs[(kk + lk) * LDT + wn + lr];
+      const double b1 = Bs[(kk + lk) * LDT + wn + 16 + lr];
+      // D = (J rows) x (I rows): the I row runs along lane % 16 in the result
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+    }
+    if (c == nck - 1) {
       // acc[a][b][r] = Hloc(i = 64 I + wm + 16 a + lr, j = 64 J + wn + 16 b + 4 r + lk)
 #pragma unroll
       for (int a = 0; a < 2; a++) {
@@ -245,6 +275,9 @@ __global__ __launch_bounds__(256) void h1_push_kernel(const BatchDesc *__restric
           }
       }
     }
+    I = nI;
+    J = nJ;
+    c = ncx;
   }
 }
 
@@ -557,59 +590,7 @@ __global__ __launch_bounds__(256) void h1_gga_kernel(const BatchDesc *__restrict
 
 } // namespace
 
-void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const double *g, double *H, hipStream_t s) {
-  FRPA_REQUIRE(wave || P.n_valid == 0, FRPA_ERR_INVALID, "null wave");
-  if (P.n_valid == 0)
-    return;
-  FRPA_REQUIRE(precision == FRPA_PREC_FP64 || precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4,
-               FRPA_ERR_INVALID, "unknown precision %d", precision);
-  if (!P.adj)
-    P.adj = new AdjState();
-  AdjState &st = *P.adj;
-  const Relabel &R = get_relabel(P);
-  const int n = P.n_basis;
-  const char *mode = getenv("FRPA_ADJOINT");
-  if (precision != FRPA_PREC_FP64 || !(mode && strcmp(mode, "tiles") == 0)) {
-    const size_t nn = (size_t)n * n * T;
-    st.d_hs.grow(nn);
-    fill_f64(st.d_hs.ptr, nn, 0.0, s);  // +0.0 and int64 0 have the same bits
-    const double *bound = nullptr;      // non-null: Hs holds int64 fixed point (INT8 split)
-    if (precision == FRPA_PREC_FP64) {
-      hipLaunchKernelGGL(h1_push_kernel, dim3(P.n_valid, T), dim3(256), 0, s, P.d_desc.ptr, R.d_sib.ptr,
-                         R.d_srow.ptr, P.d_map.ptr, wave, g, P.n_full, n, st.d_hs.ptr);
-      check_launch("h1_push_kernel");
-    } else {
-      bound = h1_ozaki_push(P, T, precision == FRPA_PREC_OZAKI ? 5 : 4, wave, g,
-                            reinterpret_cast<unsigned long long *>(st.d_hs.ptr), s);
-      // a slot wait that gave up (here or in a graph replay since the last check) means an
-      // incomplete H: report it instead of adding it (not while the stream is being captured;
-      // frpa_rho1_adjoint_status covers graph replays)
-      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-      FRPA_HIP(hipStreamIsCapturing(s, &cap));
-      if (cap == hipStreamCaptureStatusNone) {
-        const int fails = adjoz_take_wait_failures(P, s);
-        FRPA_REQUIRE(fails == 0, FRPA_ERR_HIP,
-                     "INT8 adjoint: %d slot waits gave up (in this call or a graph replay since the last check); "
-                     "first_order_H was not updated by this call",
-                     fails);
-      }
-    }
-    hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, T), dim3(256), 0, s, st.d_hs.ptr,
-                       R.d_pinv.ptr, n, bound, H);
-    check_launch("h1_unrelabel_kernel");
-    return;
-  }
-  if (!st.tiles_built)
-    build_tiles(P, st);
-  if (st.n_tasks == 0)
-    return;
-  hipLaunchKernelGGL(h1_tile_kernel, dim3(st.n_tasks, T), dim3(256), 0, s, st.d_tasks.ptr, st.d_entries.ptr,
-                     P.d_desc.ptr, P.d_ibasis0.ptr, R.d_srow.ptr, R.d_pi.ptr, R.d_pinv.ptr, P.d_map.ptr, wave, g,
-                     P.n_full, n, H);
-  check_launch("h1_tile_kernel");
-}
-
-// Work items of the full first-order H: per batch (in the caller's order, so that neighbouring
+// Work items of the FP64 adjoint and the full first-order H: per batch (in the caller's order, so that neighbouring
 // batches follow each other), ranges of row tiles of about `work` tile pairs (tile I has nT - I); the
 // items of a batch are adjacent in one XCD's range, so they read its staged operands from that L2.
 void build_gga_items(Rho1Plan &P, AdjState &st) {
@@ -644,6 +625,60 @@ void build_gga_items(Rho1Plan &P, AdjState &st) {
   }
   st.d_gga_items.upload(items);
   st.n_gga_items = n;
+}
+
+void rho1_adjoint(Rho1Plan &P, int T, int precision, const double *wave, const double *g, double *H, hipStream_t s) {
+  FRPA_REQUIRE(wave || P.n_valid == 0, FRPA_ERR_INVALID, "null wave");
+  if (P.n_valid == 0)
+    return;
+  FRPA_REQUIRE(precision == FRPA_PREC_FP64 || precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4,
+               FRPA_ERR_INVALID, "unknown precision %d", precision);
+  if (!P.adj)
+    P.adj = new AdjState();
+  AdjState &st = *P.adj;
+  const Relabel &R = get_relabel(P);
+  const int n = P.n_basis;
+  const char *mode = getenv("FRPA_ADJOINT");
+  if (precision != FRPA_PREC_FP64 || !(mode && strcmp(mode, "tiles") == 0)) {
+    const size_t nn = (size_t)n * n * T;
+    st.d_hs.grow(nn);
+    fill_f64(st.d_hs.ptr, nn, 0.0, s);  // +0.0 and int64 0 have the same bits
+    const double *bound = nullptr;      // non-null: Hs holds int64 fixed point (INT8 split)
+    if (precision == FRPA_PREC_FP64) {
+      if (st.n_gga_items == 0)
+        build_gga_items(P, st);
+      hipLaunchKernelGGL(h1_push_kernel, dim3(st.n_gga_items, T), dim3(256), 0, s, P.d_desc.ptr, st.d_gga_items.ptr,
+                         R.d_sib.ptr, R.d_srow.ptr, P.d_map.ptr, wave, g, P.n_full, n, st.d_hs.ptr);
+      check_launch("h1_push_kernel");
+    } else {
+      bound = h1_ozaki_push(P, T, precision == FRPA_PREC_OZAKI ? 5 : 4, wave, g,
+                            reinterpret_cast<unsigned long long *>(st.d_hs.ptr), s);
+      // a slot wait that gave up (here or in a graph replay since the last check) means an
+      // incomplete H: report it instead of adding it (not while the stream is being captured;
+      // frpa_rho1_adjoint_status covers graph replays)
+      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+      FRPA_HIP(hipStreamIsCapturing(s, &cap));
+      if (cap == hipStreamCaptureStatusNone) {
+        const int fails = adjoz_take_wait_failures(P, s);
+        FRPA_REQUIRE(fails == 0, FRPA_ERR_HIP,
+                     "INT8 adjoint: %d slot waits gave up (in this call or a graph replay since the last check); "
+                     "first_order_H was not updated by this call",
+                     fails);
+      }
+    }
+    hipLaunchKernelGGL(h1_unrelabel_kernel, dim3((n + 63) / 64, (n + 63) / 64, T), dim3(256), 0, s, st.d_hs.ptr,
+                       R.d_pinv.ptr, n, bound, H);
+    check_launch("h1_unrelabel_kernel");
+    return;
+  }
+  if (!st.tiles_built)
+    build_tiles(P, st);
+  if (st.n_tasks == 0)
+    return;
+  hipLaunchKernelGGL(h1_tile_kernel, dim3(st.n_tasks, T), dim3(256), 0, s, st.d_tasks.ptr, st.d_entries.ptr,
+                     P.d_desc.ptr, P.d_ibasis0.ptr, R.d_srow.ptr, R.d_pi.ptr, R.d_pinv.ptr, P.d_map.ptr, wave, g,
+                     P.n_full, n, H);
+  check_launch("h1_tile_kernel");
 }
 
 void rho1_h1_gga(Rho1Plan &P, int T, int j_atom_begin, int n_tile, int j_coord, int n_atoms, const int *avn,

@@ -283,7 +283,7 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
   for (int pp = 0; pp < npk; pp += PMAX) {
     const int npp = min(PMAX, npk - pp);  // points of this pass (multiple of 64)
     const int nck = npp / 64;
-    __syncthreads();
+    block_sync();
     for (int j = tid; j < npp; j += AZ_THREADS)
       gs[j] = pp + j < np ? gt[map[d.pstart + pp + j]] : 0.0;
     for (int I = 0; I < nT; I++) {
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
       // 16-point groups w and w + 8 of the pass.
       const int a = tid & 63;
       const int ia = 64 * I + a;
-      __syncthreads();  // gs written; the previous tile's Y, scales and slots are no longer read
+      block_sync();  // gs written; the previous tile's Y, scales and slots are no longer read
       if (PC && tid < PC_CW)
         slot_full[tid] = 0;  // read again only after the barrier that ends the staging
       double z[2][16];
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
         }
       }
       red[w][a] = mx;
-      __syncthreads();
+      block_sync();
       if (w == 0) {
         double m = red[0][a];
 #pragma unroll
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
         red[0][a] = m > 0.0 ? 127.0 / m : 1.0;
         ysc_s[a] = ia < nc && m > 0.0 ? m / 127.0 * fi[ia] : 0.0;
       }
-      __syncthreads();
+      block_sync();
       const double sc = red[0][a];
 #pragma unroll
       for (int h = 0; h < 2; h++) {
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
                 v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
         }
       }
-      __syncthreads();
+      block_sync();
       double ysc[4];
 #pragma unroll
       for (int gi = 0; gi < 4; gi++)

@@ -130,15 +130,25 @@ void build_item_order(Rho1Plan &P, int pch);
 TileSched &build_tile_items(Rho1Plan &P, int pch, bool grad);
 void fill_i32(int *p, size_t n, int v, hipStream_t s);
 
+// Workgroup barrier that first waits until this wave's LDS operations have completed.
+// __syncthreads() alone lets the compiler drop that wait on paths where it takes LDS operations as
+// ordered across waves; on gfx950 a wave that reached such a barrier with LDS writes still
+// outstanding (its partial sums of a block reduction) has had them read stale by another wave right
+// after the barrier.  Used wherever one wave reads what other waves wrote to LDS.
+__device__ __forceinline__ void block_sync() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt at their maxima (no wait)
+  __syncthreads();
+}
+
 // Persistent-mode work claiming: item k of XCD group x is launch position 8 (k % len) + x of
 // perturbation t = k / len, where len = number of positions congruent to x.
 __device__ __forceinline__ bool claim_item(int *counters, int n_pos, int T, int &pos, int &t, int *s_item) {
   const int x = blockIdx.x % N_XCD;
   const int len = n_pos / N_XCD + (x < n_pos % N_XCD ? 1 : 0);
-  __syncthreads();  // every thread has read the previous claim
+  block_sync();  // every thread has read the previous claim
   if (threadIdx.x == 0)
     *s_item = atomicAdd(&counters[x], 1);
-  __syncthreads();
+  block_sync();
   const int k = *s_item;
   if (len == 0 || k >= len * T)
     return false;

@@ -427,7 +427,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
         e[q] = ok ? At[arow + (size_t)n_basis * jb] : 0ull;
       }
     };
-    __syncthreads();  // the previous tile is done with all buffers
+    block_sync();  // the previous tile is done with all buffers, its row-group sums are in LDS
     if (I > I0)
       reduce_tile(I - 1);
     v4i a[S];
@@ -612,7 +612,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
     }
   }
 
-  __syncthreads();
+  block_sync();
   if (I1 > I0)
     reduce_tile(I1 - 1);
   if (!part) {

@@ -478,11 +478,13 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
       for (int st = 0; st < nsteps; st++) {
         const bool hn = st + 1 < nsteps, hnn = st + 2 < nsteps;
         const int cur = st == 0 ? 2 : (st - 1) & 1, bn = st & 1, bnn = (st + 1) & 1;
+        // labels of chunk st + 2 (index clamped into the batch; used only if hnn), issued before
+        // the gathers and consumed after the MFMAs: waiting for them leaves the gathers in flight
+        const int jn = 64 * tile_of(st + 2) + lane;
+        const int sj = sib[min(jn, nc - 1)];
+        __builtin_amdgcn_sched_barrier(0);
         if (hn)
           gather(bn);  // in flight during this step's MFMAs
-        unsigned jr = 0;
-        if (hnn && tid < 64)
-          jr = col_off(64 * tile_of(st + 2) + tid);
         __builtin_amdgcn_sched_barrier(0);
         const signed char *B = Bs + cur * BUF;
 #pragma unroll
@@ -498,7 +500,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
               acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
         }
         if (hnn && tid < 64)
-          jco[64 * bnn + tid] = jr;
+          jco[64 * bnn + tid] = jn < nc ? nb8 * (unsigned)sj : 0u;
         __builtin_amdgcn_s_waitcnt(0);  // chunk c + 1 staged, its P' gathers arrived
         __syncthreads();
         if (hnn)

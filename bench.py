@@ -1125,14 +1125,15 @@ def bench_config3(steps, warmup, no_cpu, threads):
         return e0.elapsed_time(e1) / steps
 
     tuned = {}
-    for prec in (api.FP64, api.OZAKI):
+    for prec in (api.FP64, api.OZAKI, api.FP32, api.BF16X3):
         tuned[prec] = plan.autotune(wave, P1, precision=prec, reps=2).split(";")[0]
     ref = plan.forward(wave, P1, precision=api.FP64)[valid].clone()
     cond = plan.forward(wave.abs(), P1.abs(), precision=api.FP64)[valid].clone()
     big = ref.abs() > 1e-3 * ref.abs().max()
     res = {"workload": f"{C3_WORKLOAD}: {prob.n_batches} batches, {nb} basis functions, n_compute 80-800, "
                        f"{pts} points", "points": pts,
-           "launch": {"fp64": tuned[api.FP64], "int8_split_s5": tuned[api.OZAKI]}, "precisions": {}}
+           "launch": {"fp64": tuned[api.FP64], "int8_split_s5": tuned[api.OZAKI], "fp32_mfma": tuned[api.FP32],
+                      "bf16x3": tuned[api.BF16X3]}, "precisions": {}}
     for name, prec in (("fp64", api.FP64), ("int8_split_s5", api.OZAKI), ("int8_split_s4", api.OZAKI4),
                        ("fp32_mfma", api.FP32), ("bf16x3", api.BF16X3)):
         try:

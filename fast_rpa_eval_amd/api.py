@@ -69,7 +69,9 @@ class Rho1Plan:
         k = self._keep
         self.n_batches = len(k[0])
         self.n_basis, self.n_full_points, self.n_max_batch_size = n_basis, n_full_points, n_max_batch_size
-        self._wave_seen = None  # (weakref, version, address) of the wave the INT8 digits were made from
+        # per converted-wave family (INT8 digits; FP32 / BF16x3 planes): (weakref, version, address) of
+        # the wave they were made from
+        self._wave_seen = {}
         self.handle = ctypes.c_void_p()
         check(lib().frpa_rho1_plan_create(
             ctypes.byref(self.handle), self.n_batches, n_basis, n_full_points, n_max_batch_size,
@@ -91,15 +93,19 @@ class Rho1Plan:
             self.handle = None
 
     def _follow_wave(self, wave, precision, stream=None):
-        """The library keeps INT8 digits of a wave by its address; a different tensor at the same
-        address (a freed one's memory reused) or an in-place change (a new version) needs them
-        rebuilt, which frpa_rho1_prepare_wave does for the forward and the adjoint."""
-        if precision not in (OZAKI, OZAKI4):
+        """The library keeps INT8 digits (and the FP32 / BF16x3 planes) of a wave by its address; a
+        different tensor at the same address (a freed one's memory reused) or an in-place change (a new
+        version) needs them rebuilt, which frpa_rho1_prepare_wave does for the forward and the adjoint."""
+        if precision in (OZAKI, OZAKI4):
+            fam = "int8"
+        elif precision in (FP32, BF16X3):
+            fam = "lowp"
+        else:
             return
-        seen = self._wave_seen
+        seen = self._wave_seen.get(fam)
         if seen is not None and (seen[0]() is not wave or seen[1] != wave._version or seen[2] != wave.data_ptr()):
             self.prepare_wave(wave, precision, stream)
-        self._wave_seen = (weakref.ref(wave), wave._version, wave.data_ptr())
+        self._wave_seen[fam] = (weakref.ref(wave), wave._version, wave.data_ptr())
 
     def forward(self, wave, density_matrix, T=1, n_tile=None, precision=FP64, rho_bias=None,
                 gradient_wave=None, grad_bias=None, out=None, grad_out=None, stream=None):

@@ -164,6 +164,8 @@ Rho1Plan::~Rho1Plan() {
     adj_release(adj);
   if (adjoz)
     adjoz_release(adjoz);
+  if (lowp)
+    lowp_release(lowp);
   delete relabel;
 }
 
@@ -302,10 +304,13 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
   P->d_map.upload(map, (size_t)n_valid_points);
   // default launches: what the autotuner picks on the 60k-batch, 1024-atom workload (FP64: XCD-aware
   // spatial order, persistent, 13 point groups per wave; INT8 split: row-tile ranges of point passes
-  // as work items, persistent, four-wave workgroups with direct global -> LDS staging)
+  // as work items, persistent, four-wave workgroups with direct global -> LDS staging; the FP32 /
+  // BF16x3 comparison paths share that launch)
   P->cfg[FRPA_PREC_FP64] = LaunchCfg{1, 1, 13};
   P->cfg[FRPA_PREC_OZAKI] = LaunchCfg{3, 1, 1};
   P->cfg[FRPA_PREC_OZAKI4] = LaunchCfg{3, 1, 1};
+  P->cfg[FRPA_PREC_FP32] = LaunchCfg{3, 1, 1};
+  P->cfg[FRPA_PREC_BF16X3] = LaunchCfg{3, 1, 1};
   return P.release();
 }
 
@@ -503,10 +508,10 @@ int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, c
 int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave) {
   return guarded([&] {
     FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
-    FRPA_REQUIRE(precision == FRPA_PREC_FP64 || precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4,
-                 FRPA_ERR_INVALID, "unknown precision %d", precision);
+    FRPA_REQUIRE(precision >= FRPA_PREC_FP64 && precision <= FRPA_PREC_BF16X3, FRPA_ERR_INVALID,
+                 "unknown precision %d", precision);
     FRPA_REQUIRE(order == 0 || order == 1 || ((order == 2 || order == 3) && precision != FRPA_PREC_FP64),
-                 FRPA_ERR_INVALID, "order must be 0 or 1 (or 2, 3 for the INT8 split)");
+                 FRPA_ERR_INVALID, "order must be 0 or 1 (or 2, 3 for the INT8 split, FP32 and BF16x3)");
     FRPA_REQUIRE(persistent == 0 || persistent == 1 || persistent >= N_XCD, FRPA_ERR_INVALID,
                  "persistent must be 0, 1 or a grid of at least %d workgroups", N_XCD);
     const bool ok = points_per_wave == 0 ||
@@ -515,7 +520,8 @@ int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int per
                     (precision == FRPA_PREC_OZAKI &&
                      (points_per_wave == 4 || points_per_wave == 2 || points_per_wave == 1)) ||
                     (precision == FRPA_PREC_OZAKI4 && (points_per_wave == 4 || points_per_wave == 5 ||
-                                                       points_per_wave == 2 || points_per_wave == 1));
+                                                       points_per_wave == 2 || points_per_wave == 1)) ||
+                    ((precision == FRPA_PREC_FP32 || precision == FRPA_PREC_BF16X3) && points_per_wave == 1);
     FRPA_REQUIRE(ok, FRPA_ERR_INVALID, "%d point groups per wave not available for precision %d", points_per_wave,
                  precision);
     reinterpret_cast<Rho1Plan *>(plan)->cfg[precision] = LaunchCfg{order, persistent, points_per_wave};
@@ -528,8 +534,8 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
   return guarded([&] {
     FRPA_REQUIRE(plan && density_matrix && first_order_rho && atom_tile_size >= 1 && reps >= 1, FRPA_ERR_INVALID,
                  "bad autotune arguments");
-    FRPA_REQUIRE(precision == FRPA_PREC_FP64 || precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4,
-                 FRPA_ERR_INVALID, "unknown precision %d", precision);
+    FRPA_REQUIRE(precision >= FRPA_PREC_FP64 && precision <= FRPA_PREC_BF16X3, FRPA_ERR_INVALID,
+                 "unknown precision %d", precision);
     Rho1Plan &P = *reinterpret_cast<Rho1Plan *>(plan);
     if (P.n_valid == 0)
       return;
@@ -539,10 +545,12 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
       npgs = {4, 8, 13};
     else if (precision == FRPA_PREC_OZAKI4)
       npgs = {4, 5, 2, 1};
-    else
+    else if (precision == FRPA_PREC_OZAKI)
       npgs = {0, 2, 1};
+    else
+      npgs = {1};  // FP32 / BF16x3: one block shape
     std::vector<LaunchCfg> cands;
-    // INT8: order 2, point passes as items; order 3, row-tile ranges of point passes as items
+    // INT8, FP32, BF16x3: order 2, point passes as items; order 3, row-tile ranges of point passes as items
     const int n_orders = precision == FRPA_PREC_FP64 ? 2 : 4;
     for (int order = 0; order < n_orders; order++)
       for (int pers = 0; pers < 2; pers++)
@@ -598,9 +606,15 @@ int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wa
                            void *stream) {
   return guarded([&] {
     FRPA_REQUIRE(plan && wave, FRPA_ERR_INVALID, "bad arguments");
-    FRPA_REQUIRE(precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4, FRPA_ERR_INVALID,
-                 "prepare_wave applies to the INT8-split precisions");
+    FRPA_REQUIRE(precision != FRPA_PREC_FP64 && precision >= FRPA_PREC_OZAKI && precision <= FRPA_PREC_BF16X3,
+                 FRPA_ERR_INVALID, "prepare_wave applies to the INT8-split, FP32 and BF16x3 precisions");
     Rho1Plan &P = *reinterpret_cast<Rho1Plan *>(plan);
+    if (precision == FRPA_PREC_FP32 || precision == FRPA_PREC_BF16X3) {
+      const long long b = lowp_prepare_wave(P, wave, precision, as_stream(stream));
+      if (bytes)
+        *bytes = b;
+      return;
+    }
     const int S = precision == FRPA_PREC_OZAKI ? 5 : 4;
     long long b = ozaki_prepare_wave(P, wave, S, as_stream(stream));
     // the adjoint keeps digits of its own: rebuild them too once the adjoint has made them

@@ -67,6 +67,7 @@ constexpr int N_XCD = 8;
 struct OzakiState;  // INT8 split data (rho1_ozaki.hip)
 struct AdjState;    // basis-tile work lists of the adjoint (rho1_adjoint.hip)
 struct AdjOzState;  // digits of the INT8-split adjoint (rho1_adjoint_ozaki.hip)
+struct LowpState;   // converted operands of the FP32 / BF16x3 forward (rho1_lowp.hip)
 
 struct Rho1Plan {
   int n_batches = 0, n_basis = 0, n_full = 0, nmb = 0;
@@ -80,6 +81,7 @@ struct Rho1Plan {
   OzakiState *ozaki = nullptr;
   AdjState *adj = nullptr;
   AdjOzState *adjoz = nullptr;
+  LowpState *lowp = nullptr;
   Relabel *relabel = nullptr;       // built on first use (get_relabel)
   DevBuf<int> d_order_spatial;      // launch position -> BatchDesc index, order 1
   // order 2 (INT8 forward): the spatial order with every point pass of PCH points a work item of
@@ -96,7 +98,7 @@ struct Rho1Plan {
   DevBuf<int> d_b2_rows;            // 1 + n_basis: [0] = number of rows R, then their basis indices
   DevBuf<double> d_b2_tab;          // (b2_cap, n_basis): column c holds the R rows at c
   int b2_cap = 0;
-  LaunchCfg cfg[3];                 // per precision (FP64, INT8 split S = 5, S = 4)
+  LaunchCfg cfg[5];                 // per precision (FP64, INT8 split S = 5, S = 4, FP32, BF16x3)
   int adj_spin_limit = 0;           // polls per slot wait of the INT8 adjoint (0 = its default)
   ~Rho1Plan();
 };
@@ -159,6 +161,9 @@ __device__ __forceinline__ bool claim_item(int *counters, int n_pos, int T, int 
 void ozaki_release(OzakiState *st);
 void adj_release(AdjState *st);
 void adjoz_release(AdjOzState *st);
+void lowp_release(LowpState *st);
+// FP32 / BF16x3 forward: (re)converts the wave's phi for that precision; returns the bytes held
+long long lowp_prepare_wave(Rho1Plan &P, const double *wave, int precision, hipStream_t s);
 // INT8-split adjoint of one call: int64 fixed-point atomics into the zeroed relabelled upper-triangle
 // scratch Hs; returns the device array of the per-perturbation bounds B that fix the scales
 const double *h1_ozaki_push(Rho1Plan &P, int T, int S, const double *wave, const double *g, unsigned long long *Hs,

@@ -1,14 +1,30 @@
-// rho1 forward with the GEMM W = P1loc * phi in reduced precision, for comparison with the FP64 and
-// INT8-split paths (their error against FP64 is what the comparison states):
-//   FRPA_PREC_FP32  : operands rounded to FP32, v_mfma_f32_16x16x4f32;
+// rho1 forward with the GEMM W = P1loc * phi in reduced precision, the comparison paths of the
+// INT8-split and FP64 forwards (their error against FP64 is what the comparison states):
+//   FRPA_PREC_FP32  : operands rounded to FP32, v_mfma_f32_16x16x4_f32;
 //   FRPA_PREC_BF16X3: each operand split x = hi + lo with hi = bf16(x), lo = bf16(x - hi), three
 //                     products hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16.
-// Both accumulate each K-chunk of 32 compute rows in FP32 MFMA accumulators and add the chunk into
-// FP64 registers (compensation at chunk granularity); the ddot rho(p) = sum_i phi(i, p) W(i, p), the
-// bias and the scatter stay FP64.  Data path as the FP64 kernel: P1 gathered straight from the
-// caller's matrix, phi staged through LDS (4 waves of 16 rows x 64 points per pass).
+// The FP32 MFMA accumulators hold one K step (64 compute rows) and are added into FP64 registers
+// after every step (compensation at K-step granularity); the ddot rho(p) = sum_i phi(i, p) W(i, p),
+// the bias and the scatter are FP64, with phi(i, p) the converted operand (FP32, or hi + lo).
 // Reference: evaluate_first_order_rho.cpp:149-207 (the same per-batch GEMM + ddot).
+//
+// Data path of the INT8 split's direct-to-LDS launch (rho1_ozaki.hip), with its symmetric form
+// rho(p) = 2 sum_i phi_i(p) sum_{k >= i} P'(i, k) phi_k(p), P' the symmetric part of P1 with the
+// diagonal halved, tile pairs J >= I of the batch's label-sorted compute rows:
+//   * phi is converted once per wave into 4 bytes per element -- FP32, or a bf16 hi plane and a bf16
+//     lo plane -- per batch (nc rounded up to 64) rows in sorted order x np points, rows contiguous;
+//   * P' is converted per call into one 4-byte word per element (FP32, or bf16 hi | lo << 16) in
+//     the relabelled (n_basis, n_basis, T) layout and gathered as the MFMA A operand through a
+//     buffer resource, the next K step's gathers in flight during the current step's MFMAs;
+//   * a K step's phi chunk (64 rows x 64 points, 16 KB) goes straight from global memory into LDS
+//     (global_load_lds_dwordx4) as four 64-byte sub-rows per point: FP32 rows 16 q .. 16 q + 15
+//     (sub-row q), BF16x3 hi rows 32 h .. + 31 (sub-row h) and lo rows (sub-row 2 + h); a lane
+//     (m = lane % 16, kq = lane / 16) of MFMA q (FP32) holds k = 16 kq + q, of MFMA h (BF16x3) the
+//     eight k = 16 kq + 8 h .. + 7 -- the same K order for both operands, so the sums are those of
+//     the GEMM.
+#include <algorithm>
 #include <utility>
+#include <vector>
 
 #include "rho1_common.hpp"
 
@@ -16,152 +32,488 @@ namespace frpa {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef double v4d __attribute__((ext_vector_type(4)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) void lds_void;
+
+struct LowpState {
+  int prec = -1;
+  const double *wave = nullptr;  // the wave the phi planes were made from
+  long long bytes = 0;
+  DevBuf<long long> d_boff;      // per BatchDesc: byte offset of its phi planes
+  DevBuf<unsigned char> d_bdat;  // phi planes (4 bytes per element)
+  DevBuf<unsigned> d_a;          // per call: P' words in the new labels, (n, n, T)
+};
+
+void lowp_release(LowpState *st) { delete st; }
 
 namespace {
 
-constexpr int LKC = 32;   // K chunk
-constexpr int LNP = 64;   // points per pass
-constexpr int LPLD = LNP + 1;
+constexpr int LP_PCH = 64;              // points per pass
+constexpr int LP_BUF = 4 * LP_PCH * 64;  // bytes of one staged K step: 4 sub-rows of 64 bytes per point
 
-template <int MODE>
-__global__ __launch_bounds__(256) void rho1_lowp_kernel(const BatchDesc *__restrict__ descs,
-                                                        const int *__restrict__ ibasis0, const int *__restrict__ map,
-                                                        const double *__restrict__ wave, const double *__restrict__ P1,
-                                                        int n_basis, const double *__restrict__ bias,
-                                                        double *__restrict__ rho, int n_full, int nmb,
-                                                        int n_batches_all, int n_tile) {
-  __shared__ double Bs[LKC * LPLD];
-  __shared__ int jbs[LKC];
-  __shared__ double red[4][LNP];
+__device__ __forceinline__ int ceil64(int x) { return (x + 63) & ~63; }
+
+__device__ __forceinline__ unsigned short bf16_bits(__bf16 h) { return __builtin_bit_cast(unsigned short, h); }
+
+// x = hi + lo, hi = bf16(x) and lo = bf16(x - hi) (round to nearest), as (hi bits, lo bits)
+__device__ __forceinline__ void bf16_split(double x, unsigned short &hi, unsigned short &lo) {
+  const __bf16 h = (__bf16)(float)x;
+  const __bf16 l = (__bf16)(float)(x - (double)(float)h);
+  hi = bf16_bits(h);
+  lo = bf16_bits(l);
+}
+
+// ---- phi planes (once per wave) ---------------------------------------------------------------
+// One wave per point, lanes down the batch's sorted rows (row k is the caller's local row srow[k]).
+template <int PREC>
+__global__ __launch_bounds__(256) void phi_lowp_kernel(const BatchDesc *__restrict__ descs,
+                                                       const long long *__restrict__ boff,
+                                                       const int *__restrict__ srow_all,
+                                                       const double *__restrict__ wave,
+                                                       unsigned char *__restrict__ bdat) {
   const BatchDesc d = descs[blockIdx.x];
-  const int t = blockIdx.y;
-  const int nc = d.nc, ld = (nc + 3) & ~3;
+  const int ld = (d.nc + 3) & ~3, ncK = ceil64(d.nc);
   const double *W = wave + d.wave_off;
-  const int *ib = ibasis0 + d.ib_off;
-  const double *Pt = P1 + (size_t)n_basis * n_basis * t;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
-  for (int pc = 0; pc < d.np; pc += LNP) {
-    const int np = min(LNP, d.np - pc);
-    const double *Wp = W + (size_t)ld * pc;
-    double racc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i0 = 0; i0 < nc; i0 += 64) {
-      const int irow = i0 + 16 * w + lr;
-      const bool iok = irow < nc;
-      const size_t arow = iok ? (size_t)ib[irow] : 0;
-      v4d acc[4];
-#pragma unroll
-      for (int g = 0; g < 4; g++)
-        acc[g] = v4d{0.0, 0.0, 0.0, 0.0};
-      for (int k0 = 0; k0 < nc; k0 += LKC) {
-        __syncthreads();
-        for (int e = tid; e < LKC * LNP; e += 256) {
-          const int k = e % LKC, p = e / LKC, j = k0 + k;
-          Bs[k * LPLD + p] = (j < nc && p < np) ? Wp[j + (size_t)ld * p] : 0.0;
-        }
-        if (tid < LKC)
-          jbs[tid] = k0 + tid < nc ? ib[k0 + tid] : -1;
-        __syncthreads();
-        v4f cacc[4];
-#pragma unroll
-        for (int g = 0; g < 4; g++)
-          cacc[g] = v4f{0.f, 0.f, 0.f, 0.f};
-        if constexpr (MODE == FRPA_PREC_FP32) {
-          // 16x16x4: A(m = lr, k = lk), B(k = lk, n = lr); 8 steps of 4 k
-#pragma unroll
-          for (int s = 0; s < LKC / 4; s++) {
-            const int jb = jbs[4 * s + lk];
-            const float a = (iok && jb >= 0) ? (float)Pt[arow + (size_t)n_basis * jb] : 0.f;
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-              const float b = (float)Bs[(4 * s + lk) * LPLD + 16 * g + lr];
-              cacc[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, cacc[g], 0, 0, 0);
-            }
-          }
-        } else {
-          // 16x16x32 bf16: lane holds k = 8 lk .. 8 lk + 7 of row m = lr (A) / column n = lr (B)
-          v8bf ah, al;
-#pragma unroll
-          for (int q = 0; q < 8; q++) {
-            const int jb = jbs[8 * lk + q];
-            const float x = (iok && jb >= 0) ? (float)Pt[arow + (size_t)n_basis * jb] : 0.f;
-            const __bf16 h = (__bf16)x;
-            ah[q] = h;
-            al[q] = (__bf16)(x - (float)h);
-          }
-#pragma unroll
-          for (int g = 0; g < 4; g++) {
-            v8bf bh, bl;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-              const float x = (float)Bs[(8 * lk + q) * LPLD + 16 * g + lr];
-              const __bf16 h = (__bf16)x;
-              bh[q] = h;
-              bl[q] = (__bf16)(x - (float)h);
-            }
-            cacc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, cacc[g], 0, 0, 0);
-            cacc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, cacc[g], 0, 0, 0);
-            cacc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, cacc[g], 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int g = 0; g < 4; g++)
-#pragma unroll
-          for (int r = 0; r < 4; r++)
-            acc[g][r] += (double)cacc[g][r];
+  const int *srow = srow_all + d.ib_off;
+  unsigned char *out = bdat + boff[blockIdx.x];
+  const size_t plane = (size_t)ncK * d.np;
+  const int lane = threadIdx.x & 63;
+  for (int p = threadIdx.x >> 6; p < d.np; p += 4) {
+    const double *Wp = W + (size_t)ld * p;
+    for (int k = lane; k < ncK; k += 64) {
+      const double x = k < d.nc ? Wp[srow[k]] : 0.0;
+      if constexpr (PREC == FRPA_PREC_FP32) {
+        reinterpret_cast<float *>(out)[(size_t)ncK * p + k] = (float)x;
+      } else {
+        unsigned short hi, lo;
+        bf16_split(x, hi, lo);
+        unsigned short *o = reinterpret_cast<unsigned short *>(out);
+        o[(size_t)ncK * p + k] = hi;
+        o[plane + (size_t)ncK * p + k] = lo;
       }
-      // both MFMA shapes leave D(4 (lane / 16) + r, lane % 16) in acc[r]:
-      // acc[g][r] = W(i0 + 16 w + 4 lk + r, pc + 16 g + lr)
+    }
+  }
+}
+
+// ---- P' words (per call) ------------------------------------------------------------------------
+// P'(pinv[x], pinv[y]) at x + n y: one 64 x 64 tile of the new labels per block, the transposed
+// tile P1(pinv[y], pinv[x]) through LDS so both reads run down columns.
+template <int PREC>
+__global__ __launch_bounds__(256) void p1_lowp_kernel(const double *__restrict__ P1, const int *__restrict__ pinv,
+                                                      int n_basis, unsigned *__restrict__ out) {
+  __shared__ double tr[64][65];
+  const int x0 = 64 * blockIdx.x, y0 = 64 * blockIdx.y, t = blockIdx.z;
+  const size_t nn = (size_t)n_basis * n_basis;
+  const double *Pt = P1 + nn * t;
+  const int a = threadIdx.x & 63, b0 = threadIdx.x >> 6;
+  {
+    const int y = y0 + a;
+    const int cy = y < n_basis ? pinv[y] : 0;
+#pragma unroll 4
+    for (int u = 0; u < 16; u++) {
+      const int i = b0 + 4 * u, x = x0 + i;
+      tr[i][a] = (x < n_basis && y < n_basis) ? Pt[cy + (size_t)n_basis * pinv[x]] : 0.0;
+    }
+  }
+  __syncthreads();
+  const int x = x0 + a;
+  if (x >= n_basis)
+    return;
+  const int rx = pinv[x];
+#pragma unroll 4
+  for (int u = 0; u < 16; u++) {
+    const int j = b0 + 4 * u, y = y0 + j;
+    if (y >= n_basis)
+      break;
+    double v = 0.5 * (Pt[rx + (size_t)n_basis * pinv[y]] + tr[a][j]);
+    if (x == y)
+      v *= 0.5;
+    unsigned wv;
+    if constexpr (PREC == FRPA_PREC_FP32) {
+      wv = __float_as_uint((float)v);
+    } else {
+      unsigned short hi, lo;
+      bf16_split(v, hi, lo);
+      wv = (unsigned)hi | ((unsigned)lo << 16);
+    }
+    out[x + (size_t)n_basis * y + nn * t] = wv;
+  }
+}
+
+// ---- fused forward ------------------------------------------------------------------------------
+// Block: 4 waves, wave rg owns rows 16 rg .. + 15 of each 64-row tile I, all 64 points of the pass
+// (4 groups of 16).  Three LDS buffers as in the INT8 split: buffer 2 holds the diagonal chunk J = I
+// for the whole row tile (its values also give phi(i, p) for the ddot), buffers 0 and 1 alternate for
+// J > I.  Row tiles [I0, I1) of one pass; per-tile sums combined across the row groups in LDS,
+// ((g0 + g1) + g2) + g3, then added up over the row tiles in order here (part == nullptr) or by the
+// finishing kernel (order 3), so every launch order gives the same bits.
+template <int PREC>
+__device__ __forceinline__ void lowp_chunk(unsigned char *Bs, unsigned *jco, double *red, const BatchDesc &d, int t,
+                                           int pc, int npc, int I0, int I1, double *__restrict__ part,
+                                           const int *__restrict__ sib, const int *__restrict__ map,
+                                           const unsigned *__restrict__ At, const unsigned char *__restrict__ Bg,
+                                           int n_basis, const double *__restrict__ bias, double *__restrict__ rho,
+                                           int n_full, int nmb, int n_batches_all, int n_tile) {
+  constexpr bool FP32 = PREC == FRPA_PREC_FP32;
+  const int nc = d.nc, ncK = ceil64(nc);
+  const size_t plane2 = (size_t)2 * ncK * d.np;  // bytes of the BF16x3 hi plane
+  const int tid = threadIdx.x, lane = tid & 63, rg = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nT = ncK / 64;
+  double tot = 0.0;  // the point of reducing thread tid (tid < npc)
+  auto reduce_tile = [&](int I) {
+    if (tid < npc) {
+      const double *r = red + tid;
+      const double v = ((r[0] + r[LP_PCH]) + r[2 * LP_PCH]) + r[3 * LP_PCH];
+      if (part)
+        part[(size_t)I * npc + tid] = v;
+      else
+        tot += v;
+    }
+  };
+  // chunk c (rows 64 c .. + 63) straight into buffer buf: 1 KB block k = 64 pieces of 16 bytes by
+  // wave k mod 4; piece e: 16-byte part q = e & 3 of sub-row sl = e / (4 PCH) of point p; points past
+  // npc read the last point's values (their sums are never used)
+  auto stage_b = [&](int c, int buf) {
+    unsigned char *B = Bs + buf * LP_BUF;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int blk = 4 * u + rg;
+      const int e = 64 * blk + lane;
+      const int q = e & 3, p = min((e >> 2) % LP_PCH, npc - 1), sl = (e >> 2) / LP_PCH;
+      const size_t col = (size_t)ncK * (pc + p) + 64 * c;
+      const unsigned char *src = FP32 ? Bg + 4 * (col + 16 * sl) + 16 * q
+                                      : Bg + (sl >> 1) * plane2 + 2 * (col + 32 * (sl & 1)) + 16 * q;
+      __builtin_amdgcn_global_load_lds(static_cast<const void *>(src), (lds_void *)(B + 1024 * blk), 16, 0, 0);
+    }
+  };
+  const unsigned nb4 = 4u * (unsigned)n_basis;
+  auto col_off = [&](int j) { return j < nc ? nb4 * (unsigned)sib[j] : 0u; };
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)At, 0, (int)(nb4 * (unsigned)n_basis), 0x00020000);
+
+  for (int I = I0; I < I1; I++) {
+    const int irow = 64 * I + 16 * rg + lr;
+    const bool iok = irow < nc;
+    const unsigned arowB = 4u * (unsigned)(iok ? sib[irow] : 0);
+    double accd[4][4];
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        accd[g][r] = 0.0;
+    unsigned el[16];
+    // P' words of row irow, columns k = 64 c + 16 lk + q (labels from jco, buffer buf); columns past
+    // nc read column 0 and meet zero phi rows; rows past nc read row 0 and are never summed
+    auto gather = [&](int buf) {
+      const uint4 *cp = reinterpret_cast<const uint4 *>(jco + 64 * buf + 16 * lk);
+#pragma unroll
+      for (int h = 0; h < 4; h++) {
+        const uint4 c4 = cp[h];
+        el[4 * h] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, arowB + c4.x, 0, 0);
+        el[4 * h + 1] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, arowB + c4.y, 0, 0);
+        el[4 * h + 2] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, arowB + c4.z, 0, 0);
+        el[4 * h + 3] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, arowB + c4.w, 0, 0);
+      }
+    };
+    // A operands: FP32 a32[q] = element q; BF16x3 MFMA h takes elements 8 h .. 8 h + 7 (hi and lo)
+    float a32[16];
+    v8bf ahi[2], alo[2];
+    auto repack = [&]() {
+      if constexpr (FP32) {
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+          a32[q] = __uint_as_float(el[q]);
+      } else {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          v4u hw, lw;
+#pragma unroll
+          for (int m = 0; m < 4; m++) {
+            const unsigned x0 = el[8 * h + 2 * m], x1 = el[8 * h + 2 * m + 1];
+            hw[m] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);  // low halves: hi of both
+            lw[m] = __builtin_amdgcn_perm(x1, x0, 0x07060302u);  // high halves: lo of both
+          }
+          ahi[h] = __builtin_bit_cast(v8bf, hw);
+          alo[h] = __builtin_bit_cast(v8bf, lw);
+        }
+      }
+    };
+    block_sync();  // the previous tile is done with all buffers, its row-group sums are in LDS
+    if (I > I0)
+      reduce_tile(I - 1);
+    const int nsteps = nT - I;
+    stage_b(I, 2);
+    if (nsteps > 1)
+      stage_b(I + 1, 0);
+    if (tid < 64) {
+      jco[128 + tid] = col_off(64 * I + tid);
+      if (nsteps > 1)
+        jco[tid] = col_off(64 * (I + 1) + tid);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    gather(2);
+#pragma unroll
+    for (int q = 0; q < 16; q++)  // P'(i, k) = 0 below the diagonal (k < i)
+      if (64 * I + 16 * lk + q < irow)
+        el[q] = 0u;
+    repack();
+    for (int st = 0; st < nsteps; st++) {
+      const bool hn = st + 1 < nsteps, hnn = st + 2 < nsteps;
+      const int cur = st == 0 ? 2 : (st - 1) & 1, bn = st & 1, bnn = (st + 1) & 1;
+      // labels of chunk st + 2 (index clamped into the batch; used only if hnn), loaded before the
+      // gathers and consumed after the MFMAs
+      const int jn = 64 * (I + st + 2) + lane;
+      const int sj = sib[min(jn, nc - 1)];
+      __builtin_amdgcn_sched_barrier(0);
+      if (hn)
+        gather(bn);  // in flight during this step's MFMAs
+      __builtin_amdgcn_sched_barrier(0);
+      const unsigned char *B = Bs + cur * LP_BUF;
 #pragma unroll
       for (int g = 0; g < 4; g++) {
         const int p = 16 * g + lr;
-        double part = 0.0;
-        if (p < np) {
+        v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+        if constexpr (FP32) {
+          // sub-row lk of point p: rows 16 lk .. + 15
+          const v4u *row = reinterpret_cast<const v4u *>(B + (lk * LP_PCH + p) * 64);
 #pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int i = i0 + 16 * w + 4 * lk + r;
-            if (i < nc)
-              part += acc[g][r] * Wp[i + (size_t)ld * p];
+          for (int h = 0; h < 4; h++) {
+            const v4u b4 = row[h];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a32[4 * h + u], __uint_as_float(b4[u]), acc, 0, 0, 0);
+          }
+        } else {
+          // rows 16 lk + 8 h .. + 7: hi in sub-row lk / 2, lo in sub-row 2 + lk / 2, at byte
+          // 32 (lk % 2) + 16 h of the point's 64-byte row
+          const unsigned char *rh = B + ((lk >> 1) * LP_PCH + p) * 64 + 32 * (lk & 1);
+          const unsigned char *rl = B + ((2 + (lk >> 1)) * LP_PCH + p) * 64 + 32 * (lk & 1);
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            const v8bf bh = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u *>(rh + 16 * h));
+            const v8bf bl = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4u *>(rl + 16 * h));
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[h], bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[h], bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[h], bh, acc, 0, 0, 0);
           }
         }
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
-        racc[g] += part;
-      }
-    }
-    __syncthreads();
-    if (lk == 0) {
 #pragma unroll
-      for (int g = 0; g < 4; g++)
-        red[w][16 * g + lr] = racc[g];
+        for (int r = 0; r < 4; r++)
+          accd[g][r] += (double)acc[r];
+      }
+      if (hnn && tid < 64)
+        jco[64 * bnn + tid] = jn < nc ? nb4 * (unsigned)sj : 0u;
+      __builtin_amdgcn_s_waitcnt(0);  // chunk st + 1 staged, its P' gathers arrived
+      __syncthreads();
+      if (hnn)
+        stage_b(I + st + 2, bnn);  // into the buffer this step's chunk was read from
+      if (hn)
+        repack();
     }
-    __syncthreads();
-    for (int p = tid; p < np; p += 256) {
-      const size_t bi = (size_t)(pc + p) + (size_t)nmb * (d.b + (size_t)n_batches_all * t);
-      double sum = red[0][p] + red[1][p] + red[2][p] + red[3][p];
-      if (bias)
-        sum += bias[bi];
-      if (t < n_tile)
-        rho[(size_t)map[d.pstart + pc + p] + (size_t)n_full * t] = sum;
+
+    // epilogue: the MFMAs leave D(4 (lane / 16) + r, lane % 16) in acc[r], so
+    // accd[g][r] = W(i = 64 I + 16 rg + 4 lk + r, p = pc + 16 g + lr); phi(i, p) from the diagonal
+    // chunk (local row 16 rg + 4 lk + r)
+    const unsigned char *BD = Bs + 2 * LP_BUF;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const int p = 16 * g + lr;
+      double part_p = 0.0;
+      if (p < npc) {
+        double ph[4];
+        if constexpr (FP32) {
+          const v4u f = *reinterpret_cast<const v4u *>(BD + (rg * LP_PCH + p) * 64 + 16 * lk);
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            ph[r] = (double)__uint_as_float(f[r]);
+        } else {
+          const int off = 32 * (rg & 1) + 8 * lk;
+          const v2u hv = *reinterpret_cast<const v2u *>(BD + ((rg >> 1) * LP_PCH + p) * 64 + off);
+          const v2u lv = *reinterpret_cast<const v2u *>(BD + ((2 + (rg >> 1)) * LP_PCH + p) * 64 + off);
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const unsigned sh = 16 * (r & 1);
+            const float hi = __uint_as_float(((hv[r >> 1] >> sh) & 0xffffu) << 16);
+            const float lo = __uint_as_float(((lv[r >> 1] >> sh) & 0xffffu) << 16);
+            ph[r] = (double)hi + (double)lo;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (64 * I + 16 * rg + 4 * lk + r < nc)
+            part_p += accd[g][r] * ph[r];
+      }
+      part_p += __shfl_xor(part_p, 16);
+      part_p += __shfl_xor(part_p, 32);
+      if (lk == 0)
+        red[rg * LP_PCH + p] = part_p;
     }
+  }
+  block_sync();
+  if (I1 > I0)
+    reduce_tile(I1 - 1);
+  if (!part && tid < npc) {
+    // rho1 = 2 sum (symmetric form) + bias (evaluate_first_order_rho.cpp:187-207)
+    const int p = pc + tid;
+    double sum = 2.0 * tot;
+    if (bias)
+      sum += bias[(size_t)p + (size_t)nmb * (d.b + (size_t)n_batches_all * t)];
+    if (t < n_tile)
+      rho[(size_t)map[d.pstart + p] + (size_t)n_full * t] = sum;
+  }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256, 2) void rho1_lowp_kernel(
+    const BatchDesc *__restrict__ descs, const int *__restrict__ order, int *__restrict__ counters, int n_pos, int T,
+    const int *__restrict__ sib_all, const int *__restrict__ map, const unsigned *__restrict__ adat,
+    const unsigned char *__restrict__ bdat, const long long *__restrict__ boff, int n_basis,
+    const double *__restrict__ bias, double *__restrict__ rho, int n_full, int nmb, int n_batches_all, int n_tile,
+    int items, const TileItem *__restrict__ titems, double *__restrict__ part, long long part_stride) {
+  __shared__ __attribute__((aligned(16))) unsigned char Bs[3 * LP_BUF];
+  __shared__ __attribute__((aligned(16))) unsigned jco[192];
+  __shared__ double red[4 * LP_PCH];
+  __shared__ int s_item;
+  int pos = blockIdx.x, t = blockIdx.y;
+  for (bool first = true;; first = false) {
+    if (counters) {
+      if (!claim_item(counters, n_pos, T, pos, t, &s_item))
+        break;
+    } else if (!first) {
+      break;
+    }
+    const int it = order ? order[pos] : pos;
+    // items: (BatchDesc index << 8) | point pass; titems: row tiles of a point pass
+    const TileItem ti = titems ? titems[pos] : TileItem{0, 0, 0, 0, 0};
+    const int di = titems ? ti.di : items ? it >> 8 : it;
+    const BatchDesc d = descs[di];
+    const int nT = (d.nc + 63) / 64;
+    const int I0 = titems ? ti.I0 : 0, I1 = titems ? ti.I1 : nT;
+    double *tpart = titems ? part + part_stride * t + ti.poff : nullptr;
+    const int pbeg = titems ? LP_PCH * ti.q : items ? LP_PCH * (it & 255) : 0;
+    const int pend = (titems || items) ? min(d.np, pbeg + LP_PCH) : d.np;
+    for (int pc = pbeg; pc < pend; pc += LP_PCH)
+      lowp_chunk<PREC>(Bs, jco, red, d, t, pc, min(LP_PCH, d.np - pc), I0, I1, tpart, sib_all + d.ib_off, map,
+                       adat + (size_t)n_basis * n_basis * t, bdat + boff[di], n_basis, bias, rho, n_full, nmb,
+                       n_batches_all, n_tile);
+  }
+}
+
+// Order 3: per pass record, the row tiles' partial sums added up in row-tile order and the final
+// values written (rho1 = 2 sum + bias).
+__global__ __launch_bounds__(256) void rho1_lowp_finish_kernel(const PassRec *__restrict__ recs,
+                                                               const BatchDesc *__restrict__ descs,
+                                                               const double *__restrict__ part, long long part_stride,
+                                                               const int *__restrict__ map,
+                                                               const double *__restrict__ bias,
+                                                               double *__restrict__ rho, int n_full, int nmb,
+                                                               int n_batches_all, int n_tile) {
+  const PassRec r = recs[blockIdx.x];
+  const int t = blockIdx.y;
+  const BatchDesc d = descs[r.di];
+  const double *pp = part + part_stride * t + r.poff;
+  for (int e = threadIdx.x; e < r.npc; e += blockDim.x) {
+    double tot = 0.0;
+    for (int I = 0; I < r.nT; I++)
+      tot += pp[(size_t)I * r.npc + e];
+    const int p = LP_PCH * r.q + e;
+    double sum = 2.0 * tot;
+    if (bias)
+      sum += bias[(size_t)p + (size_t)nmb * (d.b + (size_t)n_batches_all * t)];
+    if (t < n_tile)
+      rho[(size_t)map[d.pstart + p] + (size_t)n_full * t] = sum;
+  }
+}
+
+template <int PREC>
+void prepare(Rho1Plan &P, LowpState &st, const double *wave, hipStream_t s) {
+  std::vector<long long> off(P.n_valid + 1, 0);
+  for (int i = 0; i < P.n_valid; i++) {
+    const BatchDesc &d = P.h_desc[i];
+    off[i + 1] = off[i] + 4LL * ((d.nc + 63) / 64 * 64) * d.np;
+  }
+  st.d_boff.upload(off);
+  st.d_bdat.alloc(std::max<long long>(off[P.n_valid], 16));
+  const Relabel &R = get_relabel(P);
+  hipLaunchKernelGGL(phi_lowp_kernel<PREC>, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, st.d_boff.ptr,
+                     R.d_srow.ptr, wave, st.d_bdat.ptr);
+  check_launch("phi_lowp_kernel");
+  st.prec = PREC;
+  st.wave = wave;
+  st.bytes = off[P.n_valid];
+}
+
+template <int PREC>
+void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
+  LowpState &st = *P.lowp;
+  const Relabel &R = get_relabel(P);
+  const int n = P.n_basis;
+  st.d_a.grow((size_t)n * n * c.T);
+  hipLaunchKernelGGL(p1_lowp_kernel<PREC>, dim3((n + 63) / 64, (n + 63) / 64, c.T), dim3(256), 0, s, c.P1,
+                     R.d_pinv.ptr, n, st.d_a.ptr);
+  check_launch("p1_lowp_kernel");
+  static int per_cu = -1;
+  if (per_cu < 0) {
+    FRPA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rho1_lowp_kernel<PREC>, 256, 0));
+    per_cu = std::max(per_cu, 1);
+  }
+  const LaunchCfg &cfg = P.cfg[PREC];
+  const bool items = cfg.order == 2;
+  TileSched *ts = cfg.order == 3 ? &build_tile_items(P, LP_PCH, false) : nullptr;
+  if (items)
+    build_item_order(P, LP_PCH);
+  const int n_pos = ts ? ts->n_items : items ? P.n_items : P.n_valid;
+  if (ts)
+    P.d_tile_part.grow((size_t)ts->part_doubles * c.T);
+  LaunchCfg lc = cfg;
+  if (ts)
+    lc.order = 0;  // the item list is already in launch-position order
+  const LaunchArgs la = launch_args(P, lc, c.T, per_cu, s, n_pos);
+  hipLaunchKernelGGL(rho1_lowp_kernel<PREC>, la.grid, dim3(256), 0, s, P.d_desc.ptr, la.order, la.counters, n_pos,
+                     c.T, R.d_sib.ptr, P.d_map.ptr, st.d_a.ptr, st.d_bdat.ptr, st.d_boff.ptr, n, c.bias, c.rho,
+                     P.n_full, P.nmb, P.n_batches, c.n_tile, (int)items, ts ? ts->d_items.ptr : nullptr,
+                     ts ? P.d_tile_part.ptr : nullptr, ts ? ts->part_doubles : 0LL);
+  check_launch("rho1_lowp_kernel");
+  if (ts) {
+    hipLaunchKernelGGL(rho1_lowp_finish_kernel, dim3(ts->n_recs, c.T), dim3(256), 0, s, ts->d_recs.ptr,
+                       P.d_desc.ptr, P.d_tile_part.ptr, ts->part_doubles, P.d_map.ptr, c.bias, c.rho, P.n_full,
+                       P.nmb, P.n_batches, c.n_tile);
+    check_launch("rho1_lowp_finish_kernel");
   }
 }
 
 } // namespace
 
+long long lowp_prepare_wave(Rho1Plan &P, const double *wave, int precision, hipStream_t s) {
+  FRPA_REQUIRE(precision == FRPA_PREC_FP32 || precision == FRPA_PREC_BF16X3, FRPA_ERR_INVALID,
+               "not a reduced-precision rho1 path: %d", precision);
+  if (!P.lowp)
+    P.lowp = new LowpState();
+  if (P.n_valid == 0)
+    return 0;
+  if (precision == FRPA_PREC_FP32)
+    prepare<FRPA_PREC_FP32>(P, *P.lowp, wave, s);
+  else
+    prepare<FRPA_PREC_BF16X3>(P, *P.lowp, wave, s);
+  return P.lowp->bytes;
+}
+
 void rho1_forward_lowp(Rho1Plan &P, const Rho1Call &c, int precision, hipStream_t s) {
   FRPA_REQUIRE(!c.grad, FRPA_ERR_UNSUPPORTED, "the FP32 / BF16x3 comparison paths compute rho1 only");
   if (P.n_valid == 0)
     return;
-  dim3 grid(P.n_valid, c.T);
+  // the P' gathers address one perturbation's n_basis^2 words through a buffer resource (32-bit offsets)
+  FRPA_REQUIRE((unsigned long long)P.n_basis * P.n_basis * 4 < (1ull << 32), FRPA_ERR_UNSUPPORTED,
+               "FP32 / BF16x3 rho1: n_basis %d too large (n_basis^2 * 4 bytes must stay under 4 GiB)", P.n_basis);
+  if (!P.lowp || P.lowp->wave != c.wave || P.lowp->prec != precision)
+    lowp_prepare_wave(P, c.wave, precision, s);
   if (precision == FRPA_PREC_FP32)
-    hipLaunchKernelGGL(rho1_lowp_kernel<FRPA_PREC_FP32>, grid, dim3(256), 0, s, P.d_desc.ptr, P.d_ibasis0.ptr,
-                       P.d_map.ptr, c.wave, c.P1, P.n_basis, c.bias, c.rho, P.n_full, P.nmb, P.n_batches, c.n_tile);
+    forward<FRPA_PREC_FP32>(P, c, s);
   else
-    hipLaunchKernelGGL(rho1_lowp_kernel<FRPA_PREC_BF16X3>, grid, dim3(256), 0, s, P.d_desc.ptr, P.d_ibasis0.ptr,
-                       P.d_map.ptr, c.wave, c.P1, P.n_basis, c.bias, c.rho, P.n_full, P.nmb, P.n_batches, c.n_tile);
-  check_launch("rho1_lowp_kernel");
+    forward<FRPA_PREC_BF16X3>(P, c, s);
 }
 
 } // namespace frpa

@@ -51,8 +51,10 @@ typedef struct frpa_rho1_plan frpa_rho1_plan;
 #define FRPA_PREC_OZAKI 1  /* INT8-MFMA split, 5 radix-254 digits per operand, 15 exact INT8     */
                            /* GEMMs, FP64 recombination (max|d rho1|/max|rho1| ~ 1e-12)          */
 #define FRPA_PREC_OZAKI4 2 /* INT8-MFMA split with 4 digits, 10 INT8 GEMMs (~ 5e-10)            */
-#define FRPA_PREC_FP32 3   /* rho1 forward only, for comparison: FP32 MFMA, FP32 chunk sums      */
-#define FRPA_PREC_BF16X3 4 /* rho1 forward only, for comparison: bf16 hi/lo split, 3 products    */
+#define FRPA_PREC_FP32 3   /* rho1 forward only, for comparison: FP32 MFMA, FP32 sums per 64-row */
+                           /* K step added in FP64 (~1e-7)                                       */
+#define FRPA_PREC_BF16X3 4 /* rho1 forward only, for comparison: bf16 hi/lo split, 3 products,   */
+                           /* FP32 sums per K step added in FP64 (~5e-6)                         */
 
 /*
  * Batch metadata (host arrays, copied into the plan):
@@ -122,9 +124,9 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
                        int choice_len, void *stream);
 
 /* Sets the launch configuration of frpa_rho1_forward for one precision explicitly (what the
- * autotuner chooses): order 0 = descending work, 1 = XCD-aware spatial, 2 = (INT8 split only) the
+ * autotuner chooses): order 0 = descending work, 1 = XCD-aware spatial, 2 = (not FP64) the
  * spatial order with every point pass of a batch as its own work item, the passes of a batch
- * adjacent on one XCD so they share its P' gathers in L2; 3 = (INT8 split only) every range of
+ * adjacent on one XCD so they share its P' gathers in L2; 3 = (not FP64) every range of
  * 64-row tiles (about 16 K steps of work) of a point pass its own item, a batch's items adjacent on
  * one XCD so they also share its phi digits in L2 (per-tile sums go through a device buffer of
  * n_row_tiles x points doubles per batch, 4x with the gradient, times atom_tile_size, and a
@@ -135,18 +137,22 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
  * four-wave workgroups over 64-point passes (two per CU, where the 8-wave workgroups over 128-point
  * passes fill a CU's LDS alone), 1 = those workgroups with the phi digits staged by direct
  * global -> LDS loads and the P1 digit gathers issued one K step ahead through a buffer resource
- * (n_basis^2 * 8 < 2^32; larger bases use 2).  Results do not depend on it.  A new plan starts with
- * FP64 (1, 1, 13) and the INT8 split (3, 1, 1), the autotuner's choices on a 60k-batch workload.  No counterpart in the reference
+ * (n_basis^2 * 8 < 2^32; larger bases use 2); FP32 and BF16x3 have that one block shape (1, direct
+ * global -> LDS staging, 64-point passes; n_basis^2 * 4 < 2^32).  Results do not depend on it.  A
+ * new plan starts with FP64 (1, 1, 13) and the INT8 split, FP32 and BF16x3 (3, 1, 1), the
+ * autotuner's choices on a 60k-batch workload.  No counterpart in the reference
  * (compile-time tiling). */
 int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave);
 
 /* One-off conversion of wave_batches_compress into the INT8 digits used by the INT8-split
- * precisions.  The forward and the adjoint each keep digits of their own in the plan, made on their
- * first INT8-split call and reused while the same wave pointer is passed: after changing the wave
- * in place (or passing new contents at the same address), call this function, which rebuilds the
- * forward's digits and, once the adjoint has made its own, the adjoint's.  Optional otherwise.
- * Returns in *bytes (nullable) the device bytes the digits it made hold (the forward's, plus the
- * adjoint's when it rebuilt them). */
+ * precisions (FRPA_PREC_FP32 / FRPA_PREC_BF16X3: into the FP32 values / bf16 hi and lo planes of
+ * those forwards, 4 bytes per element).  The forward and the adjoint each keep digits of their own
+ * in the plan, made on their first INT8-split call and reused while the same wave pointer is passed
+ * (the FP32 / BF16x3 forward likewise keeps its converted wave for one of the two precisions at a
+ * time): after changing the wave in place (or passing new contents at the same address), call this
+ * function, which rebuilds the forward's digits and, once the adjoint has made its own, the
+ * adjoint's.  Optional otherwise.  Returns in *bytes (nullable) the device bytes the digits it made
+ * hold (the forward's, plus the adjoint's when it rebuilt them). */
 int frpa_rho1_prepare_wave(frpa_rho1_plan *plan, int precision, const double *wave, long long *bytes,
                            void *stream);
 

@@ -47,6 +47,8 @@ WORKLOADS = {
 DEFAULT = "combined_1024atoms_60kbatches_6Mpts_l8"
 
 PEAK_FP64 = 78.6      # TFLOP/s, dense FP64 MFMA / vector
+PEAK_FP32 = 157.3     # TFLOP/s, dense FP32 MFMA
+PEAK_BF16 = 2500.0    # TFLOP/s, dense BF16 MFMA
 PEAK_INT8 = 5000.0    # TOP/s, dense INT8 MFMA
 PEAK_HBM = 8000.0     # GB/s, HBM3E
 
@@ -1151,6 +1153,19 @@ def bench_config3(steps, warmup, no_cpu, threads):
     for v in res["precisions"].values():
         if "ms" in v:
             v["speedup_vs_fp64"] = fp64_ms / v["ms"]
+    # matrix-core rate of each precision's algorithmic GEMM work against that precision's dense peak
+    # (FP64: the full product as the reference; the others multiply the upper tile pairs of the
+    # symmetric part: INT8 split S(S+1)/2 digit products, BF16x3 three products, FP32 one)
+    nc, npt = _nc_np(prob)
+    tri = float(np.sum(nc * (nc + 1) / 2 * npt))
+    mfma = {"fp64": (float(np.sum(2 * nc * nc * npt)), PEAK_FP64, "TFLOP/s"),
+            "int8_split_s5": (15 * 2 * tri, PEAK_INT8, "TOP/s"), "int8_split_s4": (10 * 2 * tri, PEAK_INT8, "TOP/s"),
+            "fp32_mfma": (2 * tri, PEAK_FP32, "TFLOP/s"), "bf16x3": (3 * 2 * tri, PEAK_BF16, "TFLOP/s")}
+    for name, v in res["precisions"].items():
+        if "ms" in v and name in mfma:
+            ops, peak, unit = mfma[name]
+            ach = ops / (v["ms"] / 1e3) / 1e12
+            v["mfma_rate"] = {"achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak, "ops": ops}
     # GGA gradient: FP64 and the INT8 split (every tile pair of P1) on a random gradient wave
     gw = torch.empty(3 * wl, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0, generator=g)
     gb = torch.zeros(3 * prob.n_max_batch_size * prob.n_batches, dtype=torch.float64, device="cuda")

@@ -545,7 +545,8 @@ def load_traffic(workload):
 def compare_precisions(D, rho_fp64, steps):
     """rho1 of every precision on the same inputs: ms per call and the error against the FP64 GPU path,
     max over points of |d rho1(p)| / max|rho1| and the median over points with |rho1(p)| > 1e-3 max of
-    the pointwise |d rho1(p)| / |rho1(p)|."""
+    the pointwise |d rho1(p)| / |rho1(p)|.  fp64_symmetric is FP64 on the INT8 split's algorithm (the
+    upper tile pairs of the symmetric part of P1, the same launch)."""
     import torch
 
     from fast_rpa_eval_amd import api
@@ -556,8 +557,8 @@ def compare_precisions(D, rho_fp64, steps):
     big = ref.abs() > 1e-3 * ref.abs().max()
     out = torch.empty_like(rho_fp64)
     res = {}
-    for name, prec in (("int8_split_s5", api.OZAKI), ("int8_split_s4", api.OZAKI4), ("fp32_mfma", api.FP32),
-                       ("bf16x3", api.BF16X3)):
+    for name, prec in (("fp64_symmetric", api.FP64_SYM), ("int8_split_s5", api.OZAKI), ("int8_split_s4", api.OZAKI4),
+                       ("fp32_mfma", api.FP32), ("bf16x3", api.BF16X3)):
         try:
             D["plan"].forward(D["wave"], D["P1"], precision=prec, out=out)
         except FrpaError as e:
@@ -926,6 +927,18 @@ def main():
 
     if a.compare_precisions and "fp64" in outs and "rho1" in names:
         line["rho1_precision_comparison"] = compare_precisions(D, outs["fp64"]["rho"], a.steps)
+        sym = line["rho1_precision_comparison"].get("fp64_symmetric", {})
+        if "speedup_vs_fp64_same_far" in line and "ms" in sym and "adjoint" in res["fp64"]["per"]:
+            # FP64 on the fast path's algorithms: rho1 on the symmetric half (the same launch as the INT8
+            # split), the FP64 adjoint (upper triangle, as the INT8 one), the FP64 Horner far field
+            f, d = res["fast"]["per"], res["fp64"]["per"]
+            t_f = sum(f.values())
+            t_d = sym["ms"] + d["adjoint"] + f["far"]
+            line["speedup_vs_fp64_same_algorithm"] = {
+                "value": t_d / t_f, "fast_ms": t_f, "fp64_ms": t_d,
+                "note": "FP64 symmetric-half rho1 (rho1_precision_comparison.fp64_symmetric) + FP64 adjoint + FP64 "
+                        "Horner far field against the fast path, per-kernel times: the precision change alone"}
+            log(f"speedup vs FP64 on the same algorithms: {t_d:.1f} / {t_f:.1f} ms = {t_d / t_f:.2f}")
 
     # roofline of every kernel of each path: algorithmic work / measured time against the peak of
     # whichever resource binds the kernel at its algorithmic work; the fast path's dominant kernel is
@@ -1127,17 +1140,17 @@ def bench_config3(steps, warmup, no_cpu, threads):
         return e0.elapsed_time(e1) / steps
 
     tuned = {}
-    for prec in (api.FP64, api.OZAKI, api.FP32, api.BF16X3):
+    for prec in (api.FP64, api.FP64_SYM, api.OZAKI, api.FP32, api.BF16X3):
         tuned[prec] = plan.autotune(wave, P1, precision=prec, reps=2).split(";")[0]
     ref = plan.forward(wave, P1, precision=api.FP64)[valid].clone()
     cond = plan.forward(wave.abs(), P1.abs(), precision=api.FP64)[valid].clone()
     big = ref.abs() > 1e-3 * ref.abs().max()
     res = {"workload": f"{C3_WORKLOAD}: {prob.n_batches} batches, {nb} basis functions, n_compute 80-800, "
                        f"{pts} points", "points": pts,
-           "launch": {"fp64": tuned[api.FP64], "int8_split_s5": tuned[api.OZAKI], "fp32_mfma": tuned[api.FP32],
-                      "bf16x3": tuned[api.BF16X3]}, "precisions": {}}
-    for name, prec in (("fp64", api.FP64), ("int8_split_s5", api.OZAKI), ("int8_split_s4", api.OZAKI4),
-                       ("fp32_mfma", api.FP32), ("bf16x3", api.BF16X3)):
+           "launch": {"fp64": tuned[api.FP64], "fp64_symmetric": tuned[api.FP64_SYM], "int8_split_s5": tuned[api.OZAKI],
+                      "fp32_mfma": tuned[api.FP32], "bf16x3": tuned[api.BF16X3]}, "precisions": {}}
+    for name, prec in (("fp64", api.FP64), ("fp64_symmetric", api.FP64_SYM), ("int8_split_s5", api.OZAKI),
+                       ("int8_split_s4", api.OZAKI4), ("fp32_mfma", api.FP32), ("bf16x3", api.BF16X3)):
         try:
             ms = timed(lambda: plan.forward(wave, P1, precision=prec, out=out))
         except FrpaError as e:
@@ -1155,10 +1168,12 @@ def bench_config3(steps, warmup, no_cpu, threads):
             v["speedup_vs_fp64"] = fp64_ms / v["ms"]
     # matrix-core rate of each precision's algorithmic GEMM work against that precision's dense peak
     # (FP64: the full product as the reference; the others multiply the upper tile pairs of the
-    # symmetric part: INT8 split S(S+1)/2 digit products, BF16x3 three products, FP32 one)
+    # symmetric part: FP64 symmetric one product, INT8 split S(S+1)/2 digit products, BF16x3 three
+    # products, FP32 one)
     nc, npt = _nc_np(prob)
     tri = float(np.sum(nc * (nc + 1) / 2 * npt))
     mfma = {"fp64": (float(np.sum(2 * nc * nc * npt)), PEAK_FP64, "TFLOP/s"),
+            "fp64_symmetric": (2 * tri, PEAK_FP64, "TFLOP/s"),
             "int8_split_s5": (15 * 2 * tri, PEAK_INT8, "TOP/s"), "int8_split_s4": (10 * 2 * tri, PEAK_INT8, "TOP/s"),
             "fp32_mfma": (2 * tri, PEAK_FP32, "TFLOP/s"), "bf16x3": (3 * 2 * tri, PEAK_BF16, "TFLOP/s")}
     for name, v in res["precisions"].items():

@@ -16,6 +16,7 @@ FRPA_PREC_OZAKI = 1
 FRPA_PREC_OZAKI4 = 2
 FRPA_PREC_FP32 = 3
 FRPA_PREC_BF16X3 = 4
+FRPA_PREC_FP64_SYM = 5
 FRPA_FAR_TERMS = 0
 FRPA_FAR_HORNER = 1
 

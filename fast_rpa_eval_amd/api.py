@@ -25,6 +25,7 @@ from ._lib import FrpaError, check, lib  # noqa: F401  (FrpaError re-exported)
 
 FP64, OZAKI, OZAKI4 = _lib.FRPA_PREC_FP64, _lib.FRPA_PREC_OZAKI, _lib.FRPA_PREC_OZAKI4
 FP32, BF16X3 = _lib.FRPA_PREC_FP32, _lib.FRPA_PREC_BF16X3
+FP64_SYM = _lib.FRPA_PREC_FP64_SYM
 FAR_TERMS, FAR_HORNER = _lib.FRPA_FAR_TERMS, _lib.FRPA_FAR_HORNER
 
 

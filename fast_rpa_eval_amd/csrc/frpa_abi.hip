@@ -311,6 +311,7 @@ Rho1Plan *make_rho1_plan(int n_batches, int n_basis, int n_full_points, int n_ma
   P->cfg[FRPA_PREC_OZAKI4] = LaunchCfg{3, 1, 1};
   P->cfg[FRPA_PREC_FP32] = LaunchCfg{3, 1, 1};
   P->cfg[FRPA_PREC_BF16X3] = LaunchCfg{3, 1, 1};
+  P->cfg[FRPA_PREC_FP64_SYM] = LaunchCfg{3, 1, 1};
   return P.release();
 }
 
@@ -488,7 +489,7 @@ int frpa_rho1_forward(frpa_rho1_plan *plan, int atom_tile_size, int n_tile, int 
       rho1_forward_fp64(P, c, s);
     } else if (precision == FRPA_PREC_OZAKI || precision == FRPA_PREC_OZAKI4) {
       rho1_forward_ozaki(P, c, precision == FRPA_PREC_OZAKI ? 5 : 4, s);
-    } else if (precision == FRPA_PREC_FP32 || precision == FRPA_PREC_BF16X3) {
+    } else if (precision == FRPA_PREC_FP32 || precision == FRPA_PREC_BF16X3 || precision == FRPA_PREC_FP64_SYM) {
       rho1_forward_lowp(P, c, precision, s);
     } else {
       FRPA_REQUIRE(false, FRPA_ERR_INVALID, "unknown precision %d", precision);
@@ -508,7 +509,7 @@ int frpa_rho1_adjoint(frpa_rho1_plan *plan, int atom_tile_size, int precision, c
 int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave) {
   return guarded([&] {
     FRPA_REQUIRE(plan, FRPA_ERR_INVALID, "null plan");
-    FRPA_REQUIRE(precision >= FRPA_PREC_FP64 && precision <= FRPA_PREC_BF16X3, FRPA_ERR_INVALID,
+    FRPA_REQUIRE(precision >= FRPA_PREC_FP64 && precision <= FRPA_PREC_FP64_SYM, FRPA_ERR_INVALID,
                  "unknown precision %d", precision);
     FRPA_REQUIRE(order == 0 || order == 1 || ((order == 2 || order == 3) && precision != FRPA_PREC_FP64),
                  FRPA_ERR_INVALID, "order must be 0 or 1 (or 2, 3 for the INT8 split, FP32 and BF16x3)");
@@ -521,7 +522,9 @@ int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int per
                      (points_per_wave == 4 || points_per_wave == 2 || points_per_wave == 1)) ||
                     (precision == FRPA_PREC_OZAKI4 && (points_per_wave == 4 || points_per_wave == 5 ||
                                                        points_per_wave == 2 || points_per_wave == 1)) ||
-                    ((precision == FRPA_PREC_FP32 || precision == FRPA_PREC_BF16X3) && points_per_wave == 1);
+                    ((precision == FRPA_PREC_FP32 || precision == FRPA_PREC_BF16X3 ||
+                      precision == FRPA_PREC_FP64_SYM) &&
+                     points_per_wave == 1);
     FRPA_REQUIRE(ok, FRPA_ERR_INVALID, "%d point groups per wave not available for precision %d", points_per_wave,
                  precision);
     reinterpret_cast<Rho1Plan *>(plan)->cfg[precision] = LaunchCfg{order, persistent, points_per_wave};
@@ -534,7 +537,7 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
   return guarded([&] {
     FRPA_REQUIRE(plan && density_matrix && first_order_rho && atom_tile_size >= 1 && reps >= 1, FRPA_ERR_INVALID,
                  "bad autotune arguments");
-    FRPA_REQUIRE(precision >= FRPA_PREC_FP64 && precision <= FRPA_PREC_BF16X3, FRPA_ERR_INVALID,
+    FRPA_REQUIRE(precision >= FRPA_PREC_FP64 && precision <= FRPA_PREC_FP64_SYM, FRPA_ERR_INVALID,
                  "unknown precision %d", precision);
     Rho1Plan &P = *reinterpret_cast<Rho1Plan *>(plan);
     if (P.n_valid == 0)
@@ -548,7 +551,7 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
     else if (precision == FRPA_PREC_OZAKI)
       npgs = {0, 2, 1};
     else
-      npgs = {1};  // FP32 / BF16x3: one block shape
+      npgs = {1};  // FP32 / BF16x3 / FP64 symmetric half: one block shape
     std::vector<LaunchCfg> cands;
     // INT8, FP32, BF16x3: order 2, point passes as items; order 3, row-tile ranges of point passes as items
     const int n_orders = precision == FRPA_PREC_FP64 ? 2 : 4;

@@ -98,7 +98,7 @@ struct Rho1Plan {
   DevBuf<int> d_b2_rows;            // 1 + n_basis: [0] = number of rows R, then their basis indices
   DevBuf<double> d_b2_tab;          // (b2_cap, n_basis): column c holds the R rows at c
   int b2_cap = 0;
-  LaunchCfg cfg[5];                 // per precision (FP64, INT8 split S = 5, S = 4, FP32, BF16x3)
+  LaunchCfg cfg[6];                 // per precision (FP64, INT8 split S = 5, S = 4, FP32, BF16x3, FP64 symmetric)
   int adj_spin_limit = 0;           // polls per slot wait of the INT8 adjoint (0 = its default)
   ~Rho1Plan();
 };

@@ -55,6 +55,10 @@ typedef struct frpa_rho1_plan frpa_rho1_plan;
                            /* K step added in FP64 (~1e-7)                                       */
 #define FRPA_PREC_BF16X3 4 /* rho1 forward only, for comparison: bf16 hi/lo split, 3 products,   */
                            /* FP32 sums per K step added in FP64 (~5e-6)                         */
+#define FRPA_PREC_FP64_SYM 5 /* rho1 forward only, no gradient: FP64 MFMA on the symmetric half */
+                             /* of P1 (tile pairs J >= I, the INT8 split's algorithm and launch), */
+                             /* wave 16-byte aligned; FRPA_PREC_FP64 multiplies the full P1 as    */
+                             /* the reference does                                                */
 
 /*
  * Batch metadata (host arrays, copied into the plan):
@@ -137,10 +141,11 @@ int frpa_rho1_autotune(frpa_rho1_plan *plan, int atom_tile_size, int precision, 
  * four-wave workgroups over 64-point passes (two per CU, where the 8-wave workgroups over 128-point
  * passes fill a CU's LDS alone), 1 = those workgroups with the phi digits staged by direct
  * global -> LDS loads and the P1 digit gathers issued one K step ahead through a buffer resource
- * (n_basis^2 * 8 < 2^32; larger bases use 2); FP32 and BF16x3 have that one block shape (1, direct
- * global -> LDS staging, 64-point passes; n_basis^2 * 4 < 2^32).  Results do not depend on it.  A
- * new plan starts with FP64 (1, 1, 13) and the INT8 split, FP32 and BF16x3 (3, 1, 1), the
- * autotuner's choices on a 60k-batch workload.  No counterpart in the reference
+ * (n_basis^2 * 8 < 2^32; larger bases use 2); FP32, BF16x3 and the FP64 symmetric half have that one
+ * block shape (1, direct global -> LDS staging, 64-point passes; n_basis^2 * 4 < 2^32, FP64
+ * n_basis^2 * 8 < 2^32).  Results do not depend on it.  A new plan starts with FP64 (1, 1, 13) and
+ * the INT8 split, FP32, BF16x3 and FP64 symmetric half (3, 1, 1), the autotuner's choices on a
+ * 60k-batch workload.  No counterpart in the reference
  * (compile-time tiling). */
 int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int persistent, int points_per_wave);
 

@@ -7,7 +7,8 @@
   INT8 adjoint merges neighbouring batches into groups).  The wave is placed 2^31 + 4096 doubles into
   a device buffer, so every batch offset needs 64 bits.  FP64 forward and adjoint within 1e-12 of
   max|.|, the INT8 split (5 digits) within 1e-10, the INT8 adjoint with merged groups (threshold 0.8)
-  and with every batch alone.
+  and with every batch alone; the FP32-MFMA / BF16x3 forwards within 1e-6 / 5e-5, the FP64
+  symmetric-half forward within 1e-12.
 * 512 atoms (7 168 basis functions), n_compute ~ U{80..800}: every precision of the forward against
   the oracle, with the maximum, median and condition-normalised errors printed (the last divides
   |d rho1(p)| by sum_ij |phi_i(p)| |P1_ij| |phi_j(p)|, the oracle run on |phi| and |P1|).
@@ -83,7 +84,8 @@ def shape1024(cuda, oracle):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("precision,tol", [(api.FP64, 1e-12), (api.OZAKI, 1e-10)])
+@pytest.mark.parametrize("precision,tol", [(api.FP64, 1e-12), (api.OZAKI, 1e-10), (api.FP32, 1e-6),
+                                           (api.BF16X3, 5e-5), (api.FP64_SYM, 1e-12)])
 def test_rho1_1024atom_shapes_64bit_offsets(shape1024, precision, tol):
     D = shape1024
     got = D["plan"].forward(D["wave"], D["P1"], precision=precision).cpu().numpy()

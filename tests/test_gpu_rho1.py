@@ -213,12 +213,13 @@ def test_rho1_launch_configs_bit_identical(cuda, oracle, monkeypatch, precision,
     assert torch.equal(plan.forward(wave, P1, T=T, precision=precision), ref)
 
 
-@pytest.mark.parametrize("precision,tol", [(api.FP32, 1e-6), (api.BF16X3, 5e-5)])
+@pytest.mark.parametrize("precision,tol", [(api.FP32, 1e-6), (api.BF16X3, 5e-5), (api.FP64_SYM, 1e-12)])
 def test_rho1_reduced_precision_comparison_paths(cuda, oracle, precision, tol):
     """FP32-MFMA and BF16x3 rho1 (the comparison paths): their max|d rho1| / max|rho1| against the FP64
     oracle is printed and bounded by what their operand precision allows (FP32: 24-bit operands,
     BF16x3: ~16-bit hi + lo operands, both summed in FP32 per 64-row K step and in FP64 across
-    steps); it stays far above the 1e-9 the INT8 split meets."""
+    steps); it stays far above the 1e-9 the INT8 split meets.  The FP64 symmetric-half path on the
+    same launch meets the FP64 bar, 1e-12."""
     prob = synth.make_rho1_problem(24, 14.0, 90, (1, 230), (1, 180), config_k=3, zero_nc=(3,))
     P1 = synth.make_p1(prob.atom_xyz, prob.basis, 3003)
     bias, _ = _bias(prob, 1, 4)
@@ -230,10 +231,11 @@ def test_rho1_reduced_precision_comparison_paths(cuda, oracle, precision, tol):
     assert np.all(np.isfinite(got)) and rel <= tol
 
 
-@pytest.mark.parametrize("precision,tol", [(api.FP32, 1e-6), (api.BF16X3, 5e-5)])
+@pytest.mark.parametrize("precision,tol", [(api.FP32, 1e-6), (api.BF16X3, 5e-5), (api.FP64_SYM, 1e-12)])
 def test_rho1_reduced_precision_launch_configs(cuda, oracle, monkeypatch, precision, tol):
-    """FP32 / BF16x3 rho1 with a nonsymmetric P1, T = 3 with a short tile, bias, five row tiles and
-    up to four 64-point passes, a batch with n_compute 0 and one with no points: against the oracle
+    """FP32 / BF16x3 / FP64 symmetric-half rho1 with a nonsymmetric P1, T = 3 with a short tile, bias,
+    five row tiles and up to four 64-point passes, a batch with n_compute 0 and one with no points
+    (FP64: n_compute not a multiple of 32, NaN padding rows of the wave): against the oracle
     at the precision's bound, and every launch order (whole batches, point passes as items, row-tile
     ranges as items -- one row tile per item here --, one workgroup per item or a persistent grid)
     gives the same bits; the autotuner keeps one of them."""
@@ -264,7 +266,7 @@ def test_rho1_reduced_precision_launch_configs(cuda, oracle, monkeypatch, precis
     assert desc.count("order=") == 8
     with pytest.raises(RuntimeError, match="point groups"):
         plan.set_launch(precision, 0, 0, 4)
-    with pytest.raises(RuntimeError, match="rho1 only"):
+    with pytest.raises(RuntimeError, match="rho1 only"):  # the gradient needs the full product
         plan.forward(wave, P1, T=T, precision=precision, gradient_wave=api.to_device(synth.random_like_wave(prob, 3, 3)),
                      grad_bias=torch.zeros(3 * prob.n_max_batch_size * prob.n_batches * T, dtype=torch.float64,
                                            device="cuda"))

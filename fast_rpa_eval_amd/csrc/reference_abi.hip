@@ -85,13 +85,16 @@ void run_or_die(const char *where, F &&f) {
 }
 
 // FRPA_RHO1_PRECISION: ozaki (INT8 split with 5 digits, ~1e-12 of max|rho1| and of max|grad rho1|;
-// the default), ozaki4 (4 digits, ~1e-9), fp64 (FP64 MFMA).  FRPA_RHO1_GRADIENT=0: first_order_
-// gradient_rho is not computed (left zero; LDA callers), which lets the INT8 split multiply only the
-// symmetric half of P1.
-int rho1_precision_from_env() {
+// the default), ozaki4 (4 digits, ~1e-9), fp64 (FP64 MFMA, the full product as the reference),
+// fp64sym (FP64 MFMA on the symmetric half of P1 when the gradient is off, else the full product).
+// FRPA_RHO1_GRADIENT=0: first_order_gradient_rho is not computed (left zero; LDA callers), which lets
+// the INT8 split and fp64sym multiply only the symmetric half of P1.
+int rho1_precision_from_env(bool want_grad) {
   const char *p = getenv("FRPA_RHO1_PRECISION");
   if (p && strcmp(p, "fp64") == 0)
     return FRPA_PREC_FP64;
+  if (p && strcmp(p, "fp64sym") == 0)
+    return want_grad ? FRPA_PREC_FP64 : FRPA_PREC_FP64_SYM;
   if (p && strcmp(p, "ozaki4") == 0)
     return FRPA_PREC_OZAKI4;
   return FRPA_PREC_OZAKI;
@@ -167,7 +170,7 @@ Rho1Plan *plan_from_reference_arrays(int nb, int n_basis, int n_full, int nmb, i
 void rho1_run(Rho1Plan &plan, int T, int n_tile, const double *wave, const double *gwave, const double *P1,
               const double *bias, const double *gbias, double *rho, double *grad, hipStream_t s) {
   const bool want_grad = rho1_gradient_from_env();
-  const int prec = rho1_precision_from_env();
+  const int prec = rho1_precision_from_env(want_grad);
   const int rc = frpa_rho1_forward(reinterpret_cast<frpa_rho1_plan *>(&plan), T, n_tile, prec, wave, P1, bias, rho,
                                    want_grad ? gwave : nullptr, want_grad ? gbias : nullptr,
                                    want_grad ? grad : nullptr, s);

@@ -111,7 +111,7 @@ def test_reference_entry_points_against_oracle(cuda, oracle, monkeypatch):
     # precision (INT8 split, 5 digits, GGA gradient on every tile pair) within 1e-10, FP64 within 1e-12
     ref, gref = oracle.rho1(prob, keep["P1"], T=T, bias=keep["bias"], gwave=keep["gwave"], gbias=keep["gbias"],
                             j_atom_begin=4, j_atom_end=4)
-    for prec, tol in ((None, 1e-10), ("fp64", 1e-12)):
+    for prec, tol in ((None, 1e-10), ("fp64", 1e-12), ("fp64sym", 1e-12)):  # fp64sym: full product (gradient on)
         if prec:
             monkeypatch.setenv("FRPA_RHO1_PRECISION", prec)
         keep["rho"][:] = -7.0
@@ -131,6 +131,13 @@ def test_reference_entry_points_against_oracle(cuda, oracle, monkeypatch):
     rel2 = np.abs(keep["rho"] - ref2).max() / np.abs(ref2).max()
     print(f"[reference rho1 entry, INT8 split, no gradient] rel {rel2:.3e}", flush=True)
     assert rel2 <= 1e-10 and np.all(keep["grad"] == 0.0)
+    # ... and FP64 on the symmetric half of P1
+    monkeypatch.setenv("FRPA_RHO1_PRECISION", "fp64sym")
+    keep["rho"][:] = -7.0
+    rho1_call(3, 4)
+    rel3 = np.abs(keep["rho"] - ref2).max() / np.abs(ref2).max()
+    print(f"[reference rho1 entry, FP64 symmetric half, no gradient] rel {rel3:.3e}", flush=True)
+    assert rel3 <= 1e-12 and np.all(keep["grad"] == 0.0)
 
     # whole potential (near-field splines + far-field multipoles): accumulates into delta_v_hartree,
     # both perturbations, two species with their own log grids

@@ -64,7 +64,8 @@ struct AdjOzState {
   DevBuf<int> d_gsib;            // relabelled basis index of every group row, ascending per group
   DevBuf<int> d_gmem;            // n_groups + 1: group g's members are [d_gmem[g], d_gmem[g + 1])
   DevBuf<GMember> d_members;
-  DevBuf<int> d_rowmap;
+  DevBuf<int> d_mdi;             // per member: its BatchDesc index (into Rho1Plan::d_desc)
+  DevBuf<int> d_rowmap;          // per member, per group row: its row in the caller's list, or -1
   DevBuf<long long> d_off;       // per group: byte offset of its digit planes
   DevBuf<signed char> d_fdig;    // phi digits (per wave)
   DevBuf<double> d_finv;         // 1 / sf per group row (compact, at ib_off)
@@ -84,6 +85,48 @@ constexpr int PMAX = 256;  // points held in LDS per pass (larger batches take s
 constexpr int GMAX = 16;   // batches per group
 
 __device__ __forceinline__ int ceil64(int x) { return (x + 63) & ~63; }
+
+// Row maps of the groups (once per wave): for member m of group g, rowmap[rowmap_off(m) + r] = the
+// caller's local row of group row r in m's block, or -1 where m lacks that basis function.  One
+// workgroup per group: the map is cleared, then each member row finds its group row by binary search
+// of its label in the group's ascending labels (every member label is in the union).
+__global__ __launch_bounds__(256) void h1_rowmap_kernel(const BatchDesc *__restrict__ gdesc,
+                                                        const int *__restrict__ gmem,
+                                                        const GMember *__restrict__ members,
+                                                        const int *__restrict__ mdi,
+                                                        const BatchDesc *__restrict__ descs,
+                                                        const int *__restrict__ sib_all,
+                                                        const int *__restrict__ srow_all,
+                                                        const int *__restrict__ gsib, int *__restrict__ rowmap) {
+  const BatchDesc gd = gdesc[blockIdx.x];
+  const int *U = gsib + gd.ib_off;
+  const int nU = gd.nc;
+  const int m0 = gmem[blockIdx.x], m1 = gmem[blockIdx.x + 1];
+  for (int m = m0; m < m1; m++) {
+    int *rm = rowmap + members[m].rowmap_off;
+    for (int r = threadIdx.x; r < nU; r += blockDim.x)
+      rm[r] = -1;
+  }
+  __syncthreads();  // cleared before any member row is placed
+  for (int m = m0; m < m1; m++) {
+    const BatchDesc d = descs[mdi[m]];
+    const int *lab = sib_all + d.ib_off, *srow = srow_all + d.ib_off;
+    int *rm = rowmap + members[m].rowmap_off;
+    for (int q = threadIdx.x; q < d.nc; q += blockDim.x) {
+      const int x = lab[q];
+      int lo = 0, hi = nU;  // first r with U[r] >= x
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (U[mid] < x)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo < nU)
+        rm[lo] = srow[q];
+    }
+  }
+}
 
 // Phi digit planes of one group (once per wave): rows in the group's order (ascending relabelled
 // index), one scale per row over all the group's points; a member's points where the row is not in
@@ -601,8 +644,9 @@ double merge_threshold() {
   return e && e[0] ? std::atof(e) : 0.8;
 }
 
-// Builds the groups into st; returns their descriptors (launch order).
-std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge) {
+// Builds the groups into st (their row maps on the device, stream s); returns their descriptors
+// (launch order).
+std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, hipStream_t s) {
   const Relabel &R = get_relabel(P);
   std::vector<int> order(P.n_valid);
   for (int i = 0; i < P.n_valid; i++)
@@ -653,8 +697,13 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge) {
   }
   std::stable_sort(gorder.begin(), gorder.end(), [&](int x, int y) { return wk[x] > wk[y]; });
   std::vector<BatchDesc> gdesc(ng);
-  std::vector<int> gsib, gmem(ng + 1, 0), rowmap;
+  size_t n_gsib = 0;
+  for (int g = 0; g < ng; g++)
+    n_gsib += rows[g].size();
+  std::vector<int> gsib, gmem(ng + 1, 0), mdi;
+  gsib.reserve(n_gsib);
   std::vector<GMember> members;
+  long long rm_off = 0;  // row-map entries so far (one per member and group row)
   for (int k = 0; k < ng; k++) {
     const int g = gorder[k];
     const std::vector<int> &u = rows[g];
@@ -668,21 +717,14 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge) {
       const BatchDesc &d = P.h_desc[i];
       GMember M{};
       M.wave_off = d.wave_off;
-      M.rowmap_off = (long long)rowmap.size();
+      M.rowmap_off = rm_off;
+      rm_off += gd.nc;
       M.ld = (d.nc + 3) & ~3;
       M.np = d.np;
       M.p0 = p0;
       p0 += d.np;
       members.push_back(M);
-      // group row r -> the caller's row of the same basis function in this member, or -1
-      const int *lab = R.h_sib.data() + d.ib_off;
-      const int *srow = R.h_srow.data() + d.ib_off;
-      int q = 0;
-      for (int r = 0; r < gd.nc; r++) {
-        while (q < d.nc && lab[q] < u[r])
-          q++;
-        rowmap.push_back(q < d.nc && lab[q] == u[r] ? srow[q] : -1);
-      }
+      mdi.push_back(i);
     }
     gd.np = p0;
     gdesc[k] = gd;
@@ -694,14 +736,21 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge) {
   st.d_gsib.upload(gsib.empty() ? std::vector<int>{0} : gsib);
   st.d_gmem.upload(gmem);
   st.d_members.upload(members.empty() ? std::vector<GMember>(1) : members);
-  st.d_rowmap.upload(rowmap.empty() ? std::vector<int>{0} : rowmap);
+  st.d_mdi.upload(mdi.empty() ? std::vector<int>{0} : mdi);
+  st.d_rowmap.alloc((size_t)std::max<long long>(rm_off, 1));
+  if (ng) {
+    hipLaunchKernelGGL(h1_rowmap_kernel, dim3(ng), dim3(256), 0, s, st.d_gdesc.ptr, st.d_gmem.ptr,
+                       st.d_members.ptr, st.d_mdi.ptr, P.d_desc.ptr, R.d_sib.ptr, R.d_srow.ptr, st.d_gsib.ptr,
+                       st.d_rowmap.ptr);
+    check_launch("h1_rowmap_kernel");
+  }
   st.merge = merge;
   return gdesc;
 }
 
 template <int S>
 void prepare_f(Rho1Plan &P, AdjOzState &st, const double *wave, double merge, hipStream_t s) {
-  const std::vector<BatchDesc> gdesc = build_groups(P, st, merge);
+  const std::vector<BatchDesc> gdesc = build_groups(P, st, merge, s);
   std::vector<long long> off(st.n_groups + 1, 0);
   size_t rows = 0;
   for (int i = 0; i < st.n_groups; i++) {

@@ -270,3 +270,37 @@ def test_rho1_reduced_precision_launch_configs(cuda, oracle, monkeypatch, precis
         plan.forward(wave, P1, T=T, precision=precision, gradient_wave=api.to_device(synth.random_like_wave(prob, 3, 3)),
                      grad_bias=torch.zeros(3 * prob.n_max_batch_size * prob.n_batches * T, dtype=torch.float64,
                                            device="cuda"))
+
+
+def test_rho1_reduced_precision_wave_changed_in_place(cuda, oracle):
+    """The FP32 / BF16x3 forward keeps the converted wave of one precision, by address: after an
+    in-place change frpa_rho1_prepare_wave rebuilds it (C-ABI contract) and the host mirror does so by
+    itself (tensor version); switching between the two precisions converts again each time."""
+    prob = synth.make_rho1_problem(20, 13.0, 60, (20, 150), (30, 220), config_k=15)
+    P1h = synth.make_p1(prob.atom_xyz, prob.basis, 3015)
+    prob2 = synth.make_rho1_problem(20, 13.0, 60, (20, 150), (30, 220), config_k=15)
+    prob2.wave = synth.random_like_wave(prob2, 1, 91)
+    ref1, _ = oracle.rho1(prob, P1h, T=1)
+    ref2, _ = oracle.rho1(prob2, P1h, T=1)
+    plan = api.Rho1Plan.from_problem(prob)
+    wave, P1 = api.to_device(prob.wave), api.to_device(np.asfortranarray(P1h))
+
+    def rel(got, ref):
+        return np.abs(got.cpu().numpy() - ref[:, 0]).max() / np.abs(ref).max()
+    for prec, tol in ((api.FP32, 1e-6), (api.BF16X3, 5e-5), (api.FP32, 1e-6)):
+        assert rel(plan.forward(wave, P1, precision=prec), ref1) <= tol
+    # C-ABI: change in place, then frpa_rho1_prepare_wave (bypassing the mirror's tracking)
+    wave.copy_(api.to_device(prob2.wave))
+    plan.prepare_wave(wave, api.FP32)
+    out = torch.empty(prob.n_full_points, dtype=torch.float64, device="cuda")
+    api._lib.check(api.lib().frpa_rho1_forward(plan.handle, 1, 1, api.FP32, api._dp(wave), api._dp(P1), None,
+                                               api._dp(out), None, None, None, api._stream()), "frpa_rho1_forward")
+    e2 = rel(out, ref2)
+    print(f"[rho1 FP32, wave changed + prepare_wave] rel {e2:.3e}", flush=True)
+    assert e2 <= 1e-6
+    # host mirror: back to the first wave in place, no explicit call, then the other precision
+    wave.copy_(api.to_device(prob.wave))
+    e1 = rel(plan.forward(wave, P1, precision=api.FP32), ref1)
+    e3 = rel(plan.forward(wave, P1, precision=api.BF16X3), ref1)
+    print(f"[rho1 FP32 / BF16x3, wave changed, mirror refresh] rel {e1:.3e} / {e3:.3e}", flush=True)
+    assert e1 <= 1e-6 and e3 <= 5e-5

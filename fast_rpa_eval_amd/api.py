@@ -70,8 +70,8 @@ class Rho1Plan:
         k = self._keep
         self.n_batches = len(k[0])
         self.n_basis, self.n_full_points, self.n_max_batch_size = n_basis, n_full_points, n_max_batch_size
-        # per converted-wave family (INT8 digits; FP32 / BF16x3 planes): (weakref, version, address) of
-        # the wave they were made from
+        # per converted wave (the INT8 digits; the FP32 planes; the BF16x3 planes): (weakref, version,
+        # address) of the wave it was made from
         self._wave_seen = {}
         self.handle = ctypes.c_void_p()
         check(lib().frpa_rho1_plan_create(
@@ -98,9 +98,9 @@ class Rho1Plan:
         different tensor at the same address (a freed one's memory reused) or an in-place change (a new
         version) needs them rebuilt, which frpa_rho1_prepare_wave does for the forward and the adjoint."""
         if precision in (OZAKI, OZAKI4):
-            fam = "int8"
+            fam = "int8"  # one digit set at a time; the library converts again when S changes
         elif precision in (FP32, BF16X3):
-            fam = "lowp"
+            fam = precision  # a converted wave per precision
         else:
             return
         seen = self._wave_seen.get(fam)

@@ -44,13 +44,16 @@ typedef unsigned v2u __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
 
-struct LowpState {
-  int prec = -1;
+// The converted wave of one precision (FP32 or BF16x3), each kept while the same wave pointer is passed.
+struct LowpConv {
   const double *wave = nullptr;  // the wave the phi planes were made from
   long long bytes = 0;
   DevBuf<long long> d_boff;      // per BatchDesc: byte offset of its phi planes
   DevBuf<unsigned char> d_bdat;  // phi planes (4 bytes per element)
-  DevBuf<unsigned> d_a;          // per call: P' words in the new labels, (n, n, T)
+};
+struct LowpState {
+  LowpConv conv[2];              // [0] FP32, [1] BF16x3
+  DevBuf<unsigned> d_a;          // per call: P' words in the new labels (FP64: caller's numbering), (n, n, T)
 };
 
 void lowp_release(LowpState *st) { delete st; }
@@ -498,8 +501,10 @@ __global__ __launch_bounds__(256) void rho1_lowp_finish_kernel(const PassRec *__
   }
 }
 
+constexpr int conv_index(int prec) { return prec == FRPA_PREC_FP32 ? 0 : 1; }
+
 template <int PREC>
-void prepare(Rho1Plan &P, LowpState &st, const double *wave, hipStream_t s) {
+void prepare(Rho1Plan &P, LowpConv &st, const double *wave, hipStream_t s) {
   std::vector<long long> off(P.n_valid + 1, 0);
   for (int i = 0; i < P.n_valid; i++) {
     const BatchDesc &d = P.h_desc[i];
@@ -511,7 +516,6 @@ void prepare(Rho1Plan &P, LowpState &st, const double *wave, hipStream_t s) {
   hipLaunchKernelGGL(phi_lowp_kernel<PREC>, dim3(P.n_valid), dim3(256), 0, s, P.d_desc.ptr, st.d_boff.ptr,
                      R.d_srow.ptr, wave, st.d_bdat.ptr);
   check_launch("phi_lowp_kernel");
-  st.prec = PREC;
   st.wave = wave;
   st.bytes = off[P.n_valid];
 }
@@ -544,9 +548,10 @@ void forward(Rho1Plan &P, const Rho1Call &c, hipStream_t s) {
   if (ts)
     lc.order = 0;  // the item list is already in launch-position order
   const LaunchArgs la = launch_args(P, lc, c.T, per_cu, s, n_pos);
+  const LowpConv &cv = st.conv[conv_index(PREC)];
   hipLaunchKernelGGL(rho1_lowp_kernel<PREC>, la.grid, dim3(256), 0, s, P.d_desc.ptr, la.order, la.counters, n_pos,
                      c.T, F64 ? P.d_ibasis0.ptr : R.d_sib.ptr, P.d_map.ptr, st.d_a.ptr,
-                     F64 ? nullptr : st.d_bdat.ptr, st.d_boff.ptr, c.wave, n, c.bias, c.rho, P.n_full, P.nmb,
+                     F64 ? nullptr : cv.d_bdat.ptr, cv.d_boff.ptr, c.wave, n, c.bias, c.rho, P.n_full, P.nmb,
                      P.n_batches, c.n_tile, (int)items, ts ? ts->d_items.ptr : nullptr,
                      ts ? P.d_tile_part.ptr : nullptr, ts ? ts->part_doubles : 0LL);
   check_launch("rho1_lowp_kernel");
@@ -567,11 +572,12 @@ long long lowp_prepare_wave(Rho1Plan &P, const double *wave, int precision, hipS
     P.lowp = new LowpState();
   if (P.n_valid == 0)
     return 0;
+  LowpConv &cv = P.lowp->conv[conv_index(precision)];
   if (precision == FRPA_PREC_FP32)
-    prepare<FRPA_PREC_FP32>(P, *P.lowp, wave, s);
+    prepare<FRPA_PREC_FP32>(P, cv, wave, s);
   else
-    prepare<FRPA_PREC_BF16X3>(P, *P.lowp, wave, s);
-  return P.lowp->bytes;
+    prepare<FRPA_PREC_BF16X3>(P, cv, wave, s);
+  return cv.bytes;
 }
 
 void rho1_forward_lowp(Rho1Plan &P, const Rho1Call &c, int precision, hipStream_t s) {
@@ -594,7 +600,7 @@ void rho1_forward_lowp(Rho1Plan &P, const Rho1Call &c, int precision, hipStream_
     forward<FRPA_PREC_FP64_SYM>(P, c, s);
     return;
   }
-  if (!P.lowp || P.lowp->wave != c.wave || P.lowp->prec != precision)
+  if (!P.lowp || P.lowp->conv[conv_index(precision)].wave != c.wave)
     lowp_prepare_wave(P, c.wave, precision, s);
   if (precision == FRPA_PREC_FP32)
     forward<FRPA_PREC_FP32>(P, c, s);

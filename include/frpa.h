@@ -153,8 +153,9 @@ int frpa_rho1_set_launch(frpa_rho1_plan *plan, int precision, int order, int per
  * precisions (FRPA_PREC_FP32 / FRPA_PREC_BF16X3: into the FP32 values / bf16 hi and lo planes of
  * those forwards, 4 bytes per element).  The forward and the adjoint each keep digits of their own
  * in the plan, made on their first INT8-split call and reused while the same wave pointer is passed
- * (the FP32 / BF16x3 forward likewise keeps its converted wave for one of the two precisions at a
- * time): after changing the wave in place (or passing new contents at the same address), call this
+ * (the FP32 and BF16x3 forwards likewise keep a converted wave each, so alternating them converts
+ * nothing and can be captured in a graph): after changing the wave in place (or passing new
+ * contents at the same address), call this
  * function, which rebuilds the forward's digits and, once the adjoint has made its own, the
  * adjoint's.  Optional otherwise.  Returns in *bytes (nullable) the device bytes the digits it made
  * hold (the forward's, plus the adjoint's when it rebuilt them). */

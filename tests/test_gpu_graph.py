@@ -63,3 +63,41 @@ def test_step_graph_replays_match_eager(cuda):
         step()  # eager work between replays
         torch.cuda.synchronize()
         assert torch.equal(rho, ref[0])
+
+
+def test_rho1_precisions_graph_replays_match_eager(cuda):
+    """Every rho1 forward of the precision study (FP64 full product and symmetric half, INT8 split,
+    FP32-MFMA, BF16x3) captured in one HIP graph at its default launch (persistent, work-claiming
+    counters cleared by a fill kernel, per-call P' conversion) and replayed four times with eager
+    work in between: each replay gives the eager bits."""
+    prob = synth.make_rho1_problem(16, 12.0, 150, (30, 200), (20, 230), config_k=16)
+    plan = api.Rho1Plan.from_problem(prob)
+    wave = api.to_device(prob.wave)
+    P1 = api.to_device(synth.make_p1(prob.atom_xyz, prob.basis, 3016))
+    precs = (api.FP64, api.FP64_SYM, api.OZAKI, api.FP32, api.BF16X3)
+    outs = [torch.empty(prob.n_full_points, dtype=torch.float64, device="cuda") for _ in precs]
+
+    def run():
+        for p, o in zip(precs, outs):
+            plan.forward(wave, P1, precision=p, out=o)
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        run()  # conversions and work lists made outside the capture
+        run()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    ref = [o.clone() for o in outs]
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        run()
+    for rep in range(4):
+        for o in outs:
+            o.fill_(np.nan)
+        plan.forward(wave, P1, precision=api.FP64, out=outs[0])  # eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for p, o, r in zip(precs, outs, ref):
+            assert torch.equal(o, r), f"replay {rep}: precision {p} differs"

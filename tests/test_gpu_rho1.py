@@ -273,9 +273,9 @@ def test_rho1_reduced_precision_launch_configs(cuda, oracle, monkeypatch, precis
 
 
 def test_rho1_reduced_precision_wave_changed_in_place(cuda, oracle):
-    """The FP32 / BF16x3 forward keeps the converted wave of one precision, by address: after an
-    in-place change frpa_rho1_prepare_wave rebuilds it (C-ABI contract) and the host mirror does so by
-    itself (tensor version); switching between the two precisions converts again each time."""
+    """The FP32 and BF16x3 forwards keep a converted wave each, by address: after an in-place change
+    frpa_rho1_prepare_wave rebuilds the one asked for (C-ABI contract) and the host mirror rebuilds each
+    on its next use (tensor version, tracked per precision)."""
     prob = synth.make_rho1_problem(20, 13.0, 60, (20, 150), (30, 220), config_k=15)
     P1h = synth.make_p1(prob.atom_xyz, prob.basis, 3015)
     prob2 = synth.make_rho1_problem(20, 13.0, 60, (20, 150), (30, 220), config_k=15)

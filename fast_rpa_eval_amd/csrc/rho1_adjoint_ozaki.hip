@@ -4,14 +4,14 @@
 // Split (radix-254 signed digits, round to nearest, as in the forward) with one scale per ROW, the
 // point index being the contraction index:
 //   Y(i, p) = g_p phi_i(p):  Y sy_i = sum_s y_s 254^-s,   sy_i = 127 / max_p |Y(i, p)|   (per call
-//                            and pass of at most 256 points, made while staging tile I)
+//                            and pass of at most 320 points, made while staging tile I)
 //   phi(k, p):               phi sf_k = sum_u f_u 254^-u,  sf_k = 127 / max_p |phi(k, p)| (per wave)
 //   H(i, k) = sum_l 254^-l C_l(i, k) / (sy_i sf_k),  C_l = sum_{s+u=l} sum_p y_s(i, p) f_u(k, p)
 // with the levels s + u <= S - 1 (exact int32: |C_l| <= 5 np 127^2).
 // Reference: evaluate_first_order_H.cpp:144-215 (contract = 0.5 w v1 phi, dsyr2k, scatter-add).
 //
 // Work items are groups of consecutive batches (the caller's order, contiguous points, at most one
-// 256-point pass, union U of the compute sets with |U|^2 <= 0.8 sum nc^2; build_groups): over U the
+// 320-point pass, union U of the compute sets with |U|^2 <= 0.8 sum nc^2; build_groups): over U the
 // group is one batch whose rows are zero at the points of members lacking them, so its product adds
 // the sum of the members' products with one atomic per H element instead of one per member -- the
 // atomics, which bound this kernel, drop ~1.2x on a spatially ordered batch list for ~4% more
@@ -81,7 +81,8 @@ namespace {
 
 constexpr double RADIX = 254.0;
 constexpr int AZ_THREADS = 512;
-constexpr int PMAX = 256;  // points held in LDS per pass (larger batches take several passes)
+constexpr int PMAX = 320;  // points held in LDS per pass (larger batches take several passes)
+constexpr int NH = (PMAX / 16 + 7) / 8;  // 16-point groups per wave when a tile's Y digits are staged
 constexpr int GMAX = 16;   // batches per group
 
 __device__ __forceinline__ int ceil64(int x) { return (x + 63) & ~63; }
@@ -303,9 +304,14 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
   constexpr int YROW = PMAX + 16;  // LDS bytes per (plane, row) of the staged Y tile
   __shared__ __attribute__((aligned(16))) signed char Ys[S * 64 * YROW];
   __shared__ double gs[PMAX];      // g(map(p), t) of the pass's points
-  __shared__ double red[8][64];
   __shared__ double ysc_s[64];     // 1 / sy of the tile's rows
-  __shared__ long long slot[PC ? PC_CW : 1][PC ? 16 * 64 : 1];  // q(k row kk, i row ii) at kk * 64 + ii
+  // the slots (q(k row kk, i row ii) at kk * 64 + ii) and the row maxima of the Y staging share LDS:
+  // the staging ends with a barrier before the first block is multiplied, and the next tile's staging
+  // starts after the barrier that follows the last block drained
+  constexpr int SLOT_WORDS = PC ? PC_CW * 16 * 64 : 1;
+  __shared__ __attribute__((aligned(16))) long long slot_mem[SLOT_WORDS > 8 * 64 ? SLOT_WORDS : 8 * 64];
+  long long(*slot)[16 * 64] = reinterpret_cast<long long(*)[16 * 64]>(slot_mem);
+  double(*red)[64] = reinterpret_cast<double(*)[64]>(slot_mem);
   __shared__ int slot_full[PC_CW], slot_cols[PC_CW][16];      // cols: H column labels of the block's 16 rows
   const int di = blockIdx.x, t = blockIdx.y;
   const BatchDesc d = descs[di];
@@ -332,37 +338,35 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
     for (int I = 0; I < nT; I++) {
       // Y = g phi for the tile's 64 sorted rows and the pass's points, as S digit planes in LDS with
       // one scale per row.  phi comes from its own digits (x = phi sf, 16 points of one row per
-      // 16-byte load per plane): z = g x, held in registers for the row maximum m; the Y digits are
-      // those of 127 z / m and 1 / sy = m / (127 sf).  Row a = tid % 64; the 8 waves take the
-      // 16-point groups w and w + 8 of the pass.
+      // 16-byte load per plane, read once for the row maximum and once more for the digits): z = g x
+      // gives the row maximum m; the Y digits are those of 127 z / m and 1 / sy = m / (127 sf).  Row a =
+      // tid % 64; the 8 waves take the 16-point groups w, w + 8, w + 16 of the pass.
       const int a = tid & 63;
       const int ia = 64 * I + a;
       block_sync();  // gs written; the previous tile's Y, scales and slots are no longer read
       if (PC && tid < PC_CW)
         slot_full[tid] = 0;  // read again only after the barrier that ends the staging
-      double z[2][16];
-      double mx = 0.0;
+      v4i fr[S];
+      auto load_fr = [&](int c16) {
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
+        for (int sl = 0; sl < S; sl++)
+          fr[sl] = *reinterpret_cast<const v4i *>(F + sl * plane + (size_t)npk * ia + pp + 16 * c16);
+      };
+      auto zval = [&](int c16, int j) {
+        double x = (double)(signed char)((unsigned)fr[S - 1][j >> 2] >> (8 * (j & 3)));
+#pragma unroll
+        for (int sl = S - 2; sl >= 0; sl--)
+          x = x * (1.0 / RADIX) + (double)(signed char)((unsigned)fr[sl][j >> 2] >> (8 * (j & 3)));
+        return x * gs[16 * c16 + j];
+      };
+      double mx = 0.0;
+      for (int h = 0; h < NH; h++) {
         const int c16 = w + 8 * h;
         if (c16 < npp / 16) {
-          v4i fr[S];
-#pragma unroll
-          for (int sl = 0; sl < S; sl++)
-            fr[sl] = *reinterpret_cast<const v4i *>(F + sl * plane + (size_t)npk * ia + pp + 16 * c16);
-#pragma unroll
-          for (int j = 0; j < 16; j++) {
-            double x = (double)(signed char)((unsigned)fr[S - 1][j >> 2] >> (8 * (j & 3)));
-#pragma unroll
-            for (int sl = S - 2; sl >= 0; sl--)
-              x = x * (1.0 / RADIX) + (double)(signed char)((unsigned)fr[sl][j >> 2] >> (8 * (j & 3)));
-            z[h][j] = x * gs[16 * c16 + j];
-            mx = fmax(mx, fabs(z[h][j]));
-          }
-        } else {
+          load_fr(c16);
 #pragma unroll
           for (int j = 0; j < 16; j++)
-            z[h][j] = 0.0;
+            mx = fmax(mx, fabs(zval(c16, j)));
         }
       }
       red[w][a] = mx;
@@ -377,10 +381,10 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
       }
       block_sync();
       const double sc = red[0][a];
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
+      for (int h = 0; h < NH; h++) {
         const int c16 = w + 8 * h;
         if (c16 < npp / 16) {
+          load_fr(c16);  // again (L2): holding them across the barrier would spill
           unsigned int w4[S][4];
 #pragma unroll
           for (int sl = 0; sl < S; sl++)
@@ -389,7 +393,7 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
               w4[sl][u] = 0u;
 #pragma unroll
           for (int j = 0; j < 16; j++) {
-            double x = z[h][j] * sc;
+            double x = zval(c16, j) * sc;
 #pragma unroll
             for (int sl = 0; sl < S; sl++) {
               const double dg = rint(x);

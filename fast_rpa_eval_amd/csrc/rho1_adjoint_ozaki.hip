@@ -51,6 +51,13 @@ struct GMember {
 };
 static_assert(sizeof(GMember) == 32, "GMember layout");
 
+struct AzItem {
+  int g;       // group (launch-order index), or -1: nothing to do
+  int i0, i1;  // its row tiles I in [i0, i1)
+  int pad;
+};
+static_assert(sizeof(AzItem) == 16, "AzItem layout");
+
 struct AdjOzState {
   int S = 0;
   const double *wave = nullptr;  // the wave the phi digits were made from
@@ -70,6 +77,10 @@ struct AdjOzState {
   DevBuf<signed char> d_fdig;    // phi digits (per wave)
   DevBuf<double> d_finv;         // 1 / sf per group row (compact, at ib_off)
   DevBuf<double> d_pmax;         // max_i |phi_i(p)| per valid point (pstart + p)
+  // Work items of the kernel: (group, row tiles [i0, i1)), in grid order; a group's items are
+  // consecutive in one XCD's share of the grid (see xcd_items), g < 0 pads the grid.
+  int n_items = 0;
+  DevBuf<AzItem> d_items;
   DevBuf<double> d_bound;        // per call: B per perturbation
   DevBuf<double> d_bpart;        // per call: B's partial sums, one per workgroup of h1_bound_kernel
   DevBuf<int> d_spin_fail;       // waits of the producer / consumer mode that gave up (0 when all is well)
@@ -291,7 +302,8 @@ constexpr int PC_PER = (PC_CW + PC_NA - 1) / PC_NA;
 constexpr int PC_SPIN = 1 << 16; // default bound on the polls of one wait (each ~64 clocks: ~2 ms)
 
 template <int S, bool PC>
-__global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *__restrict__ descs,
+__global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const AzItem *__restrict__ items,
+                                                              const BatchDesc *__restrict__ descs,
                                                               const long long *__restrict__ off,
                                                               const int *__restrict__ sib_all,
                                                               const int *__restrict__ map,
@@ -313,7 +325,10 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
   long long(*slot)[16 * 64] = reinterpret_cast<long long(*)[16 * 64]>(slot_mem);
   double(*red)[64] = reinterpret_cast<double(*)[64]>(slot_mem);
   __shared__ int slot_full[PC_CW], slot_cols[PC_CW][16];      // cols: H column labels of the block's 16 rows
-  const int di = blockIdx.x, t = blockIdx.y;
+  const AzItem item = items[blockIdx.x];
+  if (item.g < 0)
+    return;  // grid padding: the whole block leaves before any barrier
+  const int di = item.g, t = blockIdx.y;
   const BatchDesc d = descs[di];
   const int nc = d.nc, np = d.np;
   const int nc64 = ceil64(nc), npk = ceil64(np);
@@ -335,7 +350,7 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const BatchDesc *_
     block_sync();
     for (int j = tid; j < npp; j += AZ_THREADS)
       gs[j] = pp + j < np ? gt[map[d.pstart + pp + j]] : 0.0;
-    for (int I = 0; I < nT; I++) {
+    for (int I = item.i0; I < item.i1; I++) {
       // Y = g phi for the tile's 64 sorted rows and the pass's points, as S digit planes in LDS with
       // one scale per row.  phi comes from its own digits (x = phi sf, 16 points of one row per
       // 16-byte load per plane, read once for the row maximum and once more for the digits): z = g x
@@ -752,9 +767,39 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, h
   return gdesc;
 }
 
+// Work items of the kernel: one per row tile I of a group (its 4 (nT - I) units of tiles J >= I), all
+// items of a group in one XCD's share of the grid and consecutive there, so that they run side by
+// side and read the group's phi digits from that XCD's L2 (a group alone per block streamed them
+// from memory once per row tile).  Groups are dealt in launch order (descending work) to the XCD with
+// the least work so far; workgroup w of the launch runs on XCD w % N_XCD, so item k of XCD x's list
+// is grid position N_XCD k + x, and shorter lists are padded with empty items.
+std::vector<AzItem> xcd_items(const std::vector<BatchDesc> &gdesc) {
+  std::vector<std::vector<AzItem>> lists(N_XCD);
+  std::vector<double> load(N_XCD, 0.0);
+  for (int g = 0; g < (int)gdesc.size(); g++) {
+    const int nT = (gdesc[g].nc + 63) / 64;
+    const double npk = (double)((gdesc[g].np + 63) / 64);
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    for (int I = 0; I < nT; I++)
+      lists[x].push_back(AzItem{g, I, I + 1, 0});
+    load[x] += npk * nT * (nT + 1) / 2.0;
+  }
+  size_t len = 0;
+  for (const auto &l : lists)
+    len = std::max(len, l.size());
+  std::vector<AzItem> out(N_XCD * len, AzItem{-1, 0, 0, 0});
+  for (int x = 0; x < N_XCD; x++)
+    for (size_t k = 0; k < lists[x].size(); k++)
+      out[N_XCD * k + x] = lists[x][k];
+  return out;
+}
+
 template <int S>
 void prepare_f(Rho1Plan &P, AdjOzState &st, const double *wave, double merge, hipStream_t s) {
   const std::vector<BatchDesc> gdesc = build_groups(P, st, merge, s);
+  const std::vector<AzItem> items = xcd_items(gdesc);
+  st.n_items = (int)items.size();
+  st.d_items.upload(items.empty() ? std::vector<AzItem>(1, AzItem{-1, 0, 0, 0}) : items);
   std::vector<long long> off(st.n_groups + 1, 0);
   size_t rows = 0;
   for (int i = 0; i < st.n_groups; i++) {
@@ -800,8 +845,8 @@ void run(Rho1Plan &P, AdjOzState &st, int T, const double *wave, const double *g
   const char *pc = std::getenv("FRPA_ADJOINT_PC");
   auto kern = (pc && pc[0] == '0') ? h1_ozaki_kernel<S, false> : h1_ozaki_kernel<S, true>;
   const int spin_limit = P.adj_spin_limit > 0 ? P.adj_spin_limit : PC_SPIN;
-  if (st.n_groups) {
-    hipLaunchKernelGGL(kern, dim3(st.n_groups, T), dim3(AZ_THREADS), 0, s, st.d_gdesc.ptr, st.d_off.ptr,
+  if (st.n_items) {
+    hipLaunchKernelGGL(kern, dim3(st.n_items, T), dim3(AZ_THREADS), 0, s, st.d_items.ptr, st.d_gdesc.ptr, st.d_off.ptr,
                        st.d_gsib.ptr, P.d_map.ptr, g, P.n_full, st.d_fdig.ptr, st.d_finv.ptr, P.n_basis,
                        st.d_bound.ptr, Hs, st.d_spin_fail.ptr, spin_limit);
     check_launch("h1_ozaki_kernel");

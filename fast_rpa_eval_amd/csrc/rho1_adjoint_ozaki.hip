@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <iterator>
+#include <numeric>
 #include <vector>
 
 #include "rho1_common.hpp"
@@ -664,7 +665,7 @@ double merge_threshold() {
 }
 
 // Builds the groups into st (their row maps on the device, stream s); returns their descriptors
-// (launch order).
+// (group order).
 std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, hipStream_t s) {
   const Relabel &R = get_relabel(P);
   std::vector<int> order(P.n_valid);
@@ -700,7 +701,8 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, h
     np_sum = d.np;
   }
   const int ng = (int)mem.size();
-  // launch order: descending work |U|^2 points, ties by the first member's position
+  // group order (their index and the layout of their digits): descending work |U|^2 points, ties by
+  // the first member's position; the launch order is that of the work items (xcd_items)
   auto work = [&](int g) {
     long long np = 0;
     for (int i : mem[g])
@@ -767,22 +769,40 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, h
   return gdesc;
 }
 
-// Work items of the kernel: one per row tile I of a group (its 4 (nT - I) units of tiles J >= I), all
-// items of a group in one XCD's share of the grid and consecutive there, so that they run side by
-// side and read the group's phi digits from that XCD's L2 (a group alone per block streamed them
-// from memory once per row tile).  Groups are dealt in launch order (descending work) to the XCD with
-// the least work so far; workgroup w of the launch runs on XCD w % N_XCD, so item k of XCD x's list
-// is grid position N_XCD k + x, and shorter lists are padded with empty items.
+// Work items of the kernel: ranges of row tiles I of a group (tile I has the 4 (nT - I) units of tiles
+// J >= I; a range closes once it holds 8 or more column tiles), all items of a group consecutive in
+// one XCD's share of the grid, so that they run side by side and read the group's phi digits from
+// that XCD's L2 (a group alone per block streamed them from memory once per row tile).  The groups
+// in the caller's (spatial) order are cut into N_XCD contiguous ranges of equal work, one per XCD,
+// so that the H elements one XCD updates at a time are few and neighbouring.  Workgroup w of the
+// launch runs on XCD w % N_XCD: item k of XCD x's list is grid position N_XCD k + x, and shorter
+// lists are padded with empty items.
 std::vector<AzItem> xcd_items(const std::vector<BatchDesc> &gdesc) {
+  const int ng = (int)gdesc.size();
+  std::vector<int> spatial(ng);
+  std::iota(spatial.begin(), spatial.end(), 0);
+  std::stable_sort(spatial.begin(), spatial.end(), [&](int a, int b) { return gdesc[a].b < gdesc[b].b; });
+  auto work = [&](int g) {
+    const double nT = (double)((gdesc[g].nc + 63) / 64);
+    return (double)((gdesc[g].np + 63) / 64) * nT * (nT + 1.0) / 2.0;
+  };
+  double total = 0.0;
+  for (int g = 0; g < ng; g++)
+    total += work(g);
   std::vector<std::vector<AzItem>> lists(N_XCD);
-  std::vector<double> load(N_XCD, 0.0);
-  for (int g = 0; g < (int)gdesc.size(); g++) {
+  double before = 0.0;  // work of the groups ahead of g in the spatial order
+  for (int g : spatial) {
+    const int x = total > 0.0 ? std::min(N_XCD - 1, (int)(before / total * N_XCD)) : 0;
     const int nT = (gdesc[g].nc + 63) / 64;
-    const double npk = (double)((gdesc[g].np + 63) / 64);
-    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    for (int I = 0; I < nT; I++)
-      lists[x].push_back(AzItem{g, I, I + 1, 0});
-    load[x] += npk * nT * (nT + 1) / 2.0;
+    for (int I = 0, I0 = 0, w = 0; I < nT; I++) {
+      w += nT - I;
+      if (w >= 8 || I == nT - 1) {
+        lists[x].push_back(AzItem{g, I0, I + 1, 0});
+        I0 = I + 1;
+        w = 0;
+      }
+    }
+    before += work(g);
   }
   size_t len = 0;
   for (const auto &l : lists)

@@ -16,9 +16,11 @@
 // the sum of the members' products with one atomic per H element instead of one per member -- the
 // atomics, which bound this kernel, drop ~1.2x on a spatially ordered batch list for ~4% more
 // multiplications.
-// Layout: per group, S digit planes of phi with (|U| rounded to 64) rows in relabelled order
-// (get_relabel) times (points rounded to 64), point index contiguous, so that a lane's MFMA
-// fragment (16 consecutive points of one row) is one 16-byte load.  The kernel computes the Y digits
+// Layout: per group, S digit planes of phi, rows in relabelled order (get_relabel), |U| rounded to 64
+// rows (nc64) and the points rounded to 64; a plane is a sequence of 64-point chunks of nc64 rows x
+// 64 bytes (row r, point p at (p / 64) nc64 64 + 64 r + p % 64), so that a lane's MFMA fragment (16
+// consecutive points of one row) is one 16-byte load and a wave's fragments (16 rows x 64 points)
+// one contiguous kilobyte.  The kernel computes the Y digits
 // of one 64-row tile I for all points of the pass from the phi digits and g into LDS, and
 // streams the phi digits of the tiles J >= I from global memory (two steps of register prefetch, no
 // barrier inside the J loop); the upper triangle of each tile pair is added into the relabelled
@@ -75,6 +77,7 @@ struct AdjOzState {
   DevBuf<int> d_mdi;             // per member: its BatchDesc index (into Rho1Plan::d_desc)
   DevBuf<int> d_rowmap;          // per member, per group row: its row in the caller's list, or -1
   DevBuf<long long> d_off;       // per group: byte offset of its digit planes
+  DevBuf<int2> d_dtiles;         // (group, 64-row tile) work items of h1_digits_kernel
   DevBuf<signed char> d_fdig;    // phi digits (per wave)
   DevBuf<double> d_finv;         // 1 / sf per group row (compact, at ib_off)
   DevBuf<double> d_pmax;         // max_i |phi_i(p)| per valid point (pstart + p)
@@ -141,85 +144,102 @@ __global__ __launch_bounds__(256) void h1_rowmap_kernel(const BatchDesc *__restr
   }
 }
 
-// Phi digit planes of one group (once per wave): rows in the group's order (ascending relabelled
+// Phi digit planes of the groups (once per wave): rows in the group's order (ascending relabelled
 // index), one scale per row over all the group's points; a member's points where the row is not in
-// its compute set are zero.  Threads run over the group's rows (each member's reads go down a point's
-// column, twice: row maximum, then digits); rows nc .. nc64-1 are zero.
+// its compute set are zero.  One workgroup per (group, 64-row tile): lane = row of the tile, so every
+// wave load reads consecutive rows of one point (a member's rows of the tile are mostly consecutive
+// there); the 4 waves take every fourth point for the row maxima, then every fourth 16-point piece
+// for the digits (the tile's wave values again, now from L2), each lane writing 16 points of its row
+// per plane: the 4 waves fill the tile's 4 KB of a 64-point chunk together.  Rows nc .. nc64-1 and
+// points np .. npk-1 are zero.
 template <int S>
-__global__ __launch_bounds__(256) void h1_digits_kernel(const BatchDesc *__restrict__ gdesc,
+__global__ __launch_bounds__(256) void h1_digits_kernel(const int2 *__restrict__ tiles,
+                                                        const BatchDesc *__restrict__ gdesc,
                                                         const long long *__restrict__ off,
                                                         const int *__restrict__ gmem,
                                                         const GMember *__restrict__ members,
                                                         const int *__restrict__ rowmap,
                                                         const double *__restrict__ wave,
                                                         signed char *__restrict__ dig, double *__restrict__ inv) {
-  const BatchDesc d = gdesc[blockIdx.x];
+  const int2 tile = tiles[blockIdx.x];
+  const BatchDesc d = gdesc[tile.x];
   const int nc = d.nc, np = d.np;
   const int nc64 = ceil64(nc), npk = ceil64(np);
   const size_t plane = (size_t)nc64 * npk;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r = 64 * tile.y + lane;
   __shared__ GMember sm[GMAX];
-  const int m0 = gmem[blockIdx.x], nm = gmem[blockIdx.x + 1] - m0;
-  if ((int)threadIdx.x < nm)
-    sm[threadIdx.x] = members[m0 + threadIdx.x];
+  __shared__ int lrs[GMAX][64];  // member m's row of tile row lane, or -1
+  __shared__ double red[4][64];
+  const int m0 = gmem[tile.x], nm = gmem[tile.x + 1] - m0;
+  if (tid < nm)
+    sm[tid] = members[m0 + tid];
   __syncthreads();
-  signed char *out = dig + off[blockIdx.x];
-  double *iv = inv + d.ib_off;
-  for (int r0 = 0; r0 < nc64; r0 += 256) {
-    const int r = r0 + threadIdx.x;
-    if (r >= nc64)
+  for (int e = tid; e < 64 * nm; e += 256) {
+    const int m = e >> 6, rr = 64 * tile.y + (e & 63);
+    lrs[m][e & 63] = rr < nc ? rowmap[sm[m].rowmap_off + rr] : -1;
+  }
+  __syncthreads();
+  double mx = 0.0;
+  for (int m = 0; m < nm; m++) {
+    const int lr = lrs[m][lane];
+    if (lr < 0)
       continue;
-    const bool real = r < nc;
-    double mx = 0.0;
-    if (real)
-      for (int m = 0; m < nm; m++) {
-        const int lr = rowmap[sm[m].rowmap_off + r];
-        if (lr >= 0)
-          for (int p = 0; p < sm[m].np; p++)
-            mx = fmax(mx, fabs(wave[sm[m].wave_off + lr + (size_t)sm[m].ld * p]));
-      }
-    const double sc = mx > 0.0 ? 127.0 / mx : 1.0;
-    if (real)
-      iv[r] = 1.0 / sc;
-    // member m holds points [pend - its np, pend); Wr = its column of this row (nullptr: row absent)
-    int m = 0, pend = nm ? sm[0].np : 0;
-    const double *Wr = nullptr;
-    if (real && nm) {
-      const int lr = rowmap[sm[0].rowmap_off + r];
-      Wr = lr >= 0 ? wave + sm[0].wave_off + lr : nullptr;
-    }
-    for (int p0 = 0; p0 < npk; p0 += 16) {
-      unsigned int w4[S][4];
+    const double *W = wave + sm[m].wave_off + lr;
+    const size_t ld = (size_t)sm[m].ld;
+    const int mp = sm[m].np;
+    int p = w;
+    for (; p + 12 < mp; p += 16)
+      mx = fmax(fmax(fmax(mx, fabs(W[ld * p])), fabs(W[ld * (p + 4)])), fmax(fabs(W[ld * (p + 8)]), fabs(W[ld * (p + 12)])));
+    for (; p < mp; p += 4)
+      mx = fmax(mx, fabs(W[ld * p]));
+  }
+  red[w][lane] = mx;
+  __syncthreads();
+  const double rmax = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+  const double sc = rmax > 0.0 ? 127.0 / rmax : 1.0;
+  if (w == 0 && r < nc)
+    inv[d.ib_off + r] = 1.0 / sc;
+  signed char *out = dig + off[tile.x] + (size_t)64 * r;  // + (p0 / 64) nc64 64 + p0 % 64
+  // member m holds group points [pend - its np, pend); Wr = its column of this row (nullptr: absent)
+  int m = 0, pend = nm ? sm[0].np : 0;
+  const double *Wr = nullptr;
+  if (r < nc && nm) {
+    const int lr = lrs[0][lane];
+    Wr = lr >= 0 ? wave + sm[0].wave_off + lr : nullptr;
+  }
+  for (int p0 = 16 * w; p0 < npk; p0 += 64) {
+    unsigned int w4[S][4];
 #pragma unroll
-      for (int sl = 0; sl < S; sl++)
+    for (int sl = 0; sl < S; sl++)
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-          w4[sl][u] = 0u;
+      for (int u = 0; u < 4; u++)
+        w4[sl][u] = 0u;
 #pragma unroll
-      for (int j = 0; j < 16; j++) {
-        const int p = p0 + j;
-        double x = 0.0;
-        if (real && p < np) {
-          while (p >= pend) {  // p < np = the members' points, so m stays < nm
-            m++;
-            pend += sm[m].np;
-            const int lr = rowmap[sm[m].rowmap_off + r];
-            Wr = lr >= 0 ? wave + sm[m].wave_off + lr : nullptr;
-          }
-          if (Wr)
-            x = Wr[(size_t)sm[m].ld * (p - (pend - sm[m].np))] * sc;
+    for (int j = 0; j < 16; j++) {
+      const int p = p0 + j;
+      double x = 0.0;
+      if (r < nc && p < np) {
+        while (p >= pend) {  // p < np = the members' points, so m stays < nm
+          m++;
+          pend += sm[m].np;
+          const int lr = lrs[m][lane];
+          Wr = lr >= 0 ? wave + sm[m].wave_off + lr : nullptr;
         }
-#pragma unroll
-        for (int sl = 0; sl < S; sl++) {
-          const double dg = rint(x);
-          w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
-          x = (x - dg) * RADIX;
-        }
+        if (Wr)
+          x = Wr[(size_t)sm[m].ld * (p - (pend - sm[m].np))] * sc;
       }
 #pragma unroll
-      for (int sl = 0; sl < S; sl++)
-        *reinterpret_cast<v4i *>(out + sl * plane + (size_t)npk * r + p0) =
-            v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
+      for (int sl = 0; sl < S; sl++) {
+        const double dg = rint(x);
+        w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
+        x = (x - dg) * RADIX;
+      }
     }
+#pragma unroll
+    for (int sl = 0; sl < S; sl++)
+      *reinterpret_cast<v4i *>(out + sl * plane + (size_t)(p0 >> 6) * nc64 * 64 + (p0 & 63)) =
+          v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
   }
 }
 
@@ -366,7 +386,8 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const AzItem *__re
       auto load_fr = [&](int c16) {
 #pragma unroll
         for (int sl = 0; sl < S; sl++)
-          fr[sl] = *reinterpret_cast<const v4i *>(F + sl * plane + (size_t)npk * ia + pp + 16 * c16);
+          fr[sl] = *reinterpret_cast<const v4i *>(F + sl * plane + (size_t)((pp >> 6) + (c16 >> 2)) * nc64 * 64 +
+                                                  (size_t)64 * ia + 16 * (c16 & 3));
       };
       auto zval = [&](int c16, int j) {
         double x = (double)(signed char)((unsigned)fr[S - 1][j >> 2] >> (8 * (j & 3)));
@@ -442,7 +463,8 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const AzItem *__re
             int J, kwu;
             unit_jk(st, J, kwu);
             const int c = st % nck;
-            const signed char *src = F + (size_t)npk * (64 * J + 16 * kwu + lr) + pp + 64 * c + 16 * lk;
+            const signed char *src =
+                F + (size_t)((pp >> 6) + c) * nc64 * 64 + (size_t)64 * (64 * J + 16 * kwu + lr) + 16 * lk;
 #pragma unroll
             for (int sl = 0; sl < S; sl++)
               a[sl] = *reinterpret_cast<const v4i *>(src + sl * plane);
@@ -584,7 +606,8 @@ __global__ __launch_bounds__(AZ_THREADS) void h1_ozaki_kernel(const AzItem *__re
         // 64 J + 16 kw + lr, points pp + 64 c + 16 lk .. + 15; step s + 2 is in flight while s runs.
         auto load_a = [&](int st, v4i (&a)[S]) {
           const int J = I + iw + 2 * (st / nck), c = st % nck;
-          const signed char *src = F + (size_t)npk * (64 * J + 16 * kw + lr) + pp + 64 * c + 16 * lk;
+          const signed char *src =
+              F + (size_t)((pp >> 6) + c) * nc64 * 64 + (size_t)64 * (64 * J + 16 * kw + lr) + 16 * lk;
   #pragma unroll
           for (int sl = 0; sl < S; sl++)
             a[sl] = *reinterpret_cast<const v4i *>(src + sl * plane);
@@ -832,9 +855,15 @@ void prepare_f(Rho1Plan &P, AdjOzState &st, const double *wave, double merge, hi
   st.bytes = std::max<long long>(off[st.n_groups], 16);
   st.d_fdig.alloc(st.bytes);
   st.d_finv.alloc(std::max<size_t>(rows, 1));
-  if (st.n_groups) {
-    hipLaunchKernelGGL((h1_digits_kernel<S>), dim3(st.n_groups), dim3(256), 0, s, st.d_gdesc.ptr, st.d_off.ptr,
-                       st.d_gmem.ptr, st.d_members.ptr, st.d_rowmap.ptr, wave, st.d_fdig.ptr, st.d_finv.ptr);
+  std::vector<int2> tiles;  // (group, 64-row tile) of the digits kernel
+  for (int i = 0; i < st.n_groups; i++)
+    for (int I = 0; I < (gdesc[i].nc + 63) / 64; I++)
+      tiles.push_back(make_int2(i, I));
+  if (!tiles.empty()) {
+    st.d_dtiles.upload(tiles);
+    hipLaunchKernelGGL((h1_digits_kernel<S>), dim3((unsigned)tiles.size()), dim3(256), 0, s, st.d_dtiles.ptr,
+                       st.d_gdesc.ptr, st.d_off.ptr, st.d_gmem.ptr, st.d_members.ptr, st.d_rowmap.ptr, wave,
+                       st.d_fdig.ptr, st.d_finv.ptr);
     check_launch("h1_digits_kernel");
   }
   int nvp = 0;

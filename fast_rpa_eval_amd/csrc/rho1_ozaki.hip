@@ -26,8 +26,9 @@
 // is a few long runs of consecutive labels and the per-tile gathers touch few cache lines.  Per call
 // the P' digits are written as one int64 per element (its S digits) in the new labels; the phi
 // digits are prepared once per wave (frpa_rho1_prepare_wave) as S int8 planes per batch, rows in the
-// batch's sorted order, (ncK, np) column-major with ncK = nc rounded up to 64 (zero rows), so a
-// lane's B fragment (16 consecutive k of one point) is one 16-byte load.
+// batch's sorted order, ncK = nc rounded up to 64 (zero rows): a plane is a sequence of 64-row chunks,
+// each np points x 64 bytes (row k, point p at (k / 64) np 64 + 64 p + k % 64), so a lane's B fragment
+// (16 consecutive k of one point) is one 16-byte load and a chunk's staged points one contiguous run.
 //
 // Launch shapes: 8-wave blocks over 128-point passes or 4-wave blocks over 64-point passes with the
 // phi chunks staged through registers, or (ASYNC) 4-wave blocks whose phi chunks go straight from
@@ -74,7 +75,7 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // ---- phi digits (one-off) -------------------------------------------------------------------
 // One wave per point: lanes run over the batch's rows in sorted order (row k is the caller's local
-// row srow[k]); the digit planes are written 64 consecutive bytes at a time.
+// row srow[k]); a wave store writes the point's 64-byte run of one chunk of a plane.
 template <int S>
 __global__ __launch_bounds__(256) void phi_digits_kernel(const BatchDesc *__restrict__ descs,
                                                          const long long *__restrict__ boff,
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(256) void phi_digits_kernel(const BatchDesc *__rest
   const double *W = wave + d.wave_off;
   const int *srow = srow_all + d.ib_off;
   signed char *out = bdig + boff[blockIdx.x];
-  const size_t plane = (size_t)ncK * d.np;
+  const size_t plane = (size_t)ncK * d.np, chunk = (size_t)64 * d.np;
   const int lane = threadIdx.x & 63;
   for (int p = threadIdx.x >> 6; p < d.np; p += 4) {
     const double *Wp = W + (size_t)ld * p;
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(256) void phi_digits_kernel(const BatchDesc *__rest
 #pragma unroll
       for (int sl = 0; sl < S; sl++) {
         const double dg = rint(r);
-        out[sl * plane + (size_t)ncK * p + k] = (signed char)dg;
+        out[sl * plane + chunk * (k >> 6) + (size_t)64 * p + (k & 63)] = (signed char)dg;
         r = (r - dg) * RADIX;
       }
     }
@@ -369,7 +370,8 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
       const int e = tid + THREADS * u;
       const int q = e & 3, p = (e >> 2) % PCH, sl = (e >> 2) / PCH;
       bq[u] = (e < NQ && p < npc)
-                  ? *reinterpret_cast<const v4i *>(Bg + sl * plane + (size_t)ncK * (pc + p) + j0 + 16 * q)
+                  ? *reinterpret_cast<const v4i *>(Bg + sl * plane + (size_t)64 * d.np * (j0 >> 6) +
+                                                   (size_t)64 * (pc + p) + 16 * q)
                   : v4i{0, 0, 0, 0};
     }
   };
@@ -399,7 +401,7 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
         const int e = 64 * blk + lane;
         const int q = e & 3, p = min((e >> 2) % PCH, npc - 1), sl = (e >> 2) / PCH;
         __builtin_amdgcn_global_load_lds(
-            static_cast<const void *>(Bg + sl * plane + (size_t)ncK * (pc + p) + 64 * c + 16 * q),
+            static_cast<const void *>(Bg + sl * plane + (size_t)64 * d.np * c + (size_t)64 * (pc + p) + 16 * q),
             (lds_void *)(B + 1024 * blk), 16, 0, 0);
       }
     }

@@ -148,10 +148,10 @@ __global__ __launch_bounds__(256) void h1_rowmap_kernel(const BatchDesc *__restr
 // index), one scale per row over all the group's points; a member's points where the row is not in
 // its compute set are zero.  One workgroup per (group, 64-row tile): lane = row of the tile, so every
 // wave load reads consecutive rows of one point (a member's rows of the tile are mostly consecutive
-// there); the 4 waves take every fourth point for the row maxima, then every fourth 16-point piece
-// for the digits (the tile's wave values again, now from L2), each lane writing 16 points of its row
-// per plane: the 4 waves fill the tile's 4 KB of a 64-point chunk together.  Rows nc .. nc64-1 and
-// points np .. npk-1 are zero.
+// there); the 4 waves take every fourth point for the row maxima, then a 16-point piece each of
+// every 64-point chunk for the digits (the tile's wave values again), staged in LDS so that each
+// plane's 4 KB of the chunk is written contiguously.  Rows nc .. nc64-1 and points np .. npk-1 are
+// zero.
 template <int S>
 __global__ __launch_bounds__(256) void h1_digits_kernel(const int2 *__restrict__ tiles,
                                                         const BatchDesc *__restrict__ gdesc,
@@ -189,8 +189,15 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const int2 *__restrict__
     const size_t ld = (size_t)sm[m].ld;
     const int mp = sm[m].np;
     int p = w;
-    for (; p + 12 < mp; p += 16)
-      mx = fmax(fmax(fmax(mx, fabs(W[ld * p])), fabs(W[ld * (p + 4)])), fmax(fabs(W[ld * (p + 8)]), fabs(W[ld * (p + 12)])));
+    for (; p + 60 < mp; p += 64) {  // 16 loads in flight per lane
+      double x[16];
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+        x[j] = W[ld * (p + 4 * j)];
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+        mx = fmax(mx, fabs(x[j]));
+    }
     for (; p < mp; p += 4)
       mx = fmax(mx, fabs(W[ld * p]));
   }
@@ -200,7 +207,11 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const int2 *__restrict__
   const double sc = rmax > 0.0 ? 127.0 / rmax : 1.0;
   if (w == 0 && r < nc)
     inv[d.ib_off + r] = 1.0 / sc;
-  signed char *out = dig + off[tile.x] + (size_t)64 * r;  // + (p0 / 64) nc64 64 + p0 % 64
+  // digits of one 64-point chunk at a time: wave w takes the chunk's points 16 w .. + 15 of every row
+  // (its 16 loads in flight together), leaves them in LDS as the chunk's layout, and the block writes
+  // each plane's 4 KB of the tile contiguously
+  __shared__ __attribute__((aligned(16))) signed char stage[S][64 * 64];
+  signed char *obase = dig + off[tile.x] + (size_t)64 * 64 * tile.y;
   // member m holds group points [pend - its np, pend); Wr = its column of this row (nullptr: absent)
   int m = 0, pend = nm ? sm[0].np : 0;
   const double *Wr = nullptr;
@@ -208,17 +219,13 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const int2 *__restrict__
     const int lr = lrs[0][lane];
     Wr = lr >= 0 ? wave + sm[0].wave_off + lr : nullptr;
   }
-  for (int p0 = 16 * w; p0 < npk; p0 += 64) {
-    unsigned int w4[S][4];
-#pragma unroll
-    for (int sl = 0; sl < S; sl++)
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-        w4[sl][u] = 0u;
+  for (int cq = 0; cq < npk / 64; cq++) {
+    const int p0 = 64 * cq + 16 * w;
+    double x[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
       const int p = p0 + j;
-      double x = 0.0;
+      x[j] = 0.0;
       if (r < nc && p < np) {
         while (p >= pend) {  // p < np = the members' points, so m stays < nm
           m++;
@@ -227,19 +234,36 @@ __global__ __launch_bounds__(256) void h1_digits_kernel(const int2 *__restrict__
           Wr = lr >= 0 ? wave + sm[m].wave_off + lr : nullptr;
         }
         if (Wr)
-          x = Wr[(size_t)sm[m].ld * (p - (pend - sm[m].np))] * sc;
+          x[j] = Wr[(size_t)sm[m].ld * (p - (pend - sm[m].np))];
       }
+    }
+    unsigned int w4[S][4];
+#pragma unroll
+    for (int sl = 0; sl < S; sl++)
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        w4[sl][u] = 0u;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      double v = x[j] * sc;
 #pragma unroll
       for (int sl = 0; sl < S; sl++) {
-        const double dg = rint(x);
+        const double dg = rint(v);
         w4[sl][j >> 2] |= (unsigned int)(unsigned char)(signed char)dg << (8 * (j & 3));
-        x = (x - dg) * RADIX;
+        v = (v - dg) * RADIX;
       }
     }
 #pragma unroll
     for (int sl = 0; sl < S; sl++)
-      *reinterpret_cast<v4i *>(out + sl * plane + (size_t)(p0 >> 6) * nc64 * 64 + (p0 & 63)) =
+      *reinterpret_cast<v4i *>(&stage[sl][64 * lane + 16 * w]) =
           v4i{(int)w4[sl][0], (int)w4[sl][1], (int)w4[sl][2], (int)w4[sl][3]};
+    __syncthreads();
+    for (int e = tid; e < S * 256; e += 256) {
+      const int sl = e >> 8, q = e & 255;
+      *reinterpret_cast<v4i *>(obase + sl * plane + (size_t)cq * nc64 * 64 + 16 * q) =
+          *reinterpret_cast<const v4i *>(&stage[sl][16 * q]);
+    }
+    __syncthreads();
   }
 }
 

@@ -75,7 +75,10 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // ---- phi digits (one-off) -------------------------------------------------------------------
 // One wave per point: lanes run over the batch's rows in sorted order (row k is the caller's local
-// row srow[k]); a wave store writes the point's 64-byte run of one chunk of a plane.
+// row srow[k]); a wave store writes the point's 64-byte run of one chunk of a plane.  Batches of up
+// to PHI_ROWS rows (the usual case) stage srow in LDS and hold the point's column in registers, so
+// all its loads are in flight at once and the column is read once; larger ones loop over it twice.
+constexpr int PHI_ROWS = 1280;
 template <int S>
 __global__ __launch_bounds__(256) void phi_digits_kernel(const BatchDesc *__restrict__ descs,
                                                          const long long *__restrict__ boff,
@@ -89,23 +92,53 @@ __global__ __launch_bounds__(256) void phi_digits_kernel(const BatchDesc *__rest
   signed char *out = bdig + boff[blockIdx.x];
   const size_t plane = (size_t)ncK * d.np, chunk = (size_t)64 * d.np;
   const int lane = threadIdx.x & 63;
+  __shared__ int srow_s[PHI_ROWS];
+  const bool fits = d.nc <= PHI_ROWS;
+  if (fits)
+    for (int k = threadIdx.x; k < d.nc; k += 256)
+      srow_s[k] = srow[k];
+  __syncthreads();
+  auto put = [&](int k, int p, double r) {  // the S digits of r at row k, point p
+#pragma unroll
+    for (int sl = 0; sl < S; sl++) {
+      const double dg = rint(r);
+      out[sl * plane + chunk * (k >> 6) + (size_t)64 * p + (k & 63)] = (signed char)dg;
+      r = (r - dg) * RADIX;
+    }
+  };
   for (int p = threadIdx.x >> 6; p < d.np; p += 4) {
     const double *Wp = W + (size_t)ld * p;
-    double mx = 0.0;
-    for (int k = lane; k < d.nc; k += 64)
-      mx = fmax(mx, fabs(Wp[srow[k]]));
-    mx = wave_max(mx);
-    const double sb = mx > 0.0 ? 127.0 / mx : 1.0;
-    if (lane == 0)
-      binv[d.pstart + p] = 1.0 / sb;
-    for (int k = lane; k < ncK; k += 64) {
-      double r = k < d.nc ? Wp[srow[k]] * sb : 0.0;
+    if (fits) {
+      double v[PHI_ROWS / 64];
+      double mx = 0.0;
 #pragma unroll
-      for (int sl = 0; sl < S; sl++) {
-        const double dg = rint(r);
-        out[sl * plane + chunk * (k >> 6) + (size_t)64 * p + (k & 63)] = (signed char)dg;
-        r = (r - dg) * RADIX;
+      for (int i = 0; i < PHI_ROWS / 64; i++) {
+        const int k = lane + 64 * i;
+        v[i] = k < d.nc ? Wp[srow_s[k]] : 0.0;
       }
+#pragma unroll
+      for (int i = 0; i < PHI_ROWS / 64; i++)
+        mx = fmax(mx, fabs(v[i]));
+      mx = wave_max(mx);
+      const double sb = mx > 0.0 ? 127.0 / mx : 1.0;
+      if (lane == 0)
+        binv[d.pstart + p] = 1.0 / sb;
+#pragma unroll
+      for (int i = 0; i < PHI_ROWS / 64; i++) {
+        const int k = lane + 64 * i;
+        if (k < ncK)
+          put(k, p, v[i] * sb);
+      }
+    } else {
+      double mx = 0.0;
+      for (int k = lane; k < d.nc; k += 64)
+        mx = fmax(mx, fabs(Wp[srow[k]]));
+      mx = wave_max(mx);
+      const double sb = mx > 0.0 ? 127.0 / mx : 1.0;
+      if (lane == 0)
+        binv[d.pstart + p] = 1.0 / sb;
+      for (int k = lane; k < ncK; k += 64)
+        put(k, p, k < d.nc ? Wp[srow[k]] * sb : 0.0);
     }
   }
 }

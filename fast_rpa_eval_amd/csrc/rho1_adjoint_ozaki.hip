@@ -35,6 +35,7 @@
 #include <cstdlib>
 #include <iterator>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "rho1_common.hpp"
@@ -276,9 +277,20 @@ __global__ __launch_bounds__(256) void h1_pointmax_kernel(const BatchDesc *__res
   const double *W = wave + d.wave_off;
   const int lane = threadIdx.x & 63;
   for (int p = threadIdx.x >> 6; p < d.np; p += 4) {
+    const double *Wp = W + (size_t)ld * p;
     double mx = 0.0;
-    for (int r = lane; r < d.nc; r += 64)
-      mx = fmax(mx, fabs(W[r + (size_t)ld * p]));
+    int r = lane;
+    for (; r + 64 * 7 < d.nc; r += 64 * 8) {  // 8 loads in flight per lane
+      double x[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        x[j] = Wp[r + 64 * j];
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        mx = fmax(mx, fabs(x[j]));
+    }
+    for (; r < d.nc; r += 64)
+      mx = fmax(mx, fabs(Wp[r]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
       mx = fmax(mx, __shfl_xor(mx, o));
@@ -711,6 +723,49 @@ double merge_threshold() {
   return e && e[0] ? std::atof(e) : 0.8;
 }
 
+// The groups of a contiguous piece of the caller's batch order (BatchDesc indices order[0 .. n)):
+// consecutive batches whose points are contiguous join a group while its points fit one pass (PMAX),
+// it has fewer than GMAX members and the union U of the compute sets keeps |U|^2 <= merge * sum nc^2.
+struct GroupList {
+  std::vector<std::vector<int>> mem;   // members (BatchDesc indices) per group
+  std::vector<std::vector<int>> rows;  // union of the members' relabelled indices per group, ascending
+};
+
+void group_range(const Rho1Plan &P, const Relabel &R, const int *order, int n, double merge, GroupList &out) {
+  std::vector<int> U;
+  long long sq = 0;
+  int np_end = 0, np_sum = 0;
+  for (int k = 0; k < n; k++) {
+    const int idx = order[k];
+    const BatchDesc &d = P.h_desc[idx];
+    const int *lab = R.h_sib.data() + d.ib_off;
+    if (!out.mem.empty() && merge > 0.0 && (int)out.mem.back().size() < GMAX && d.pstart == np_end &&
+        np_sum + d.np <= PMAX) {
+      U.clear();
+      std::set_union(out.rows.back().begin(), out.rows.back().end(), lab, lab + d.nc, std::back_inserter(U));
+      const long long sq2 = sq + (long long)d.nc * d.nc;
+      if ((double)U.size() * (double)U.size() <= merge * (double)sq2) {
+        out.mem.back().push_back(idx);
+        out.rows.back().swap(U);
+        sq = sq2;
+        np_end = d.pstart + d.np;
+        np_sum += d.np;
+        continue;
+      }
+    }
+    out.mem.push_back({idx});
+    out.rows.emplace_back(lab, lab + d.nc);
+    sq = (long long)d.nc * d.nc;
+    np_end = d.pstart + d.np;
+    np_sum = d.np;
+  }
+}
+
+// The caller's order is cut into GROUP_PIECES contiguous pieces of equal batch count, grouped on up
+// to GROUP_PIECES host threads (a group never spans two pieces; the cut does not depend on the
+// thread count, so the groups, and H, are the same on every machine).
+constexpr int GROUP_PIECES = 16;
+
 // Builds the groups into st (their row maps on the device, stream s); returns their descriptors
 // (group order).
 std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, hipStream_t s) {
@@ -719,33 +774,29 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, h
   for (int i = 0; i < P.n_valid; i++)
     order[i] = i;
   std::sort(order.begin(), order.end(), [&](int x, int y) { return P.h_desc[x].b < P.h_desc[y].b; });
-  std::vector<std::vector<int>> mem;   // members (BatchDesc indices) per group
-  std::vector<std::vector<int>> rows;  // union of the members' relabelled indices per group
-  std::vector<int> U;
-  long long sq = 0;
-  int np_end = 0, np_sum = 0;
-  for (int idx : order) {
-    const BatchDesc &d = P.h_desc[idx];
-    const int *lab = R.h_sib.data() + d.ib_off;
-    if (!mem.empty() && merge > 0.0 && (int)mem.back().size() < GMAX && d.pstart == np_end &&
-        np_sum + d.np <= PMAX) {
-      U.clear();
-      std::set_union(rows.back().begin(), rows.back().end(), lab, lab + d.nc, std::back_inserter(U));
-      const long long sq2 = sq + (long long)d.nc * d.nc;
-      if ((double)U.size() * (double)U.size() <= merge * (double)sq2) {
-        mem.back().push_back(idx);
-        rows.back().swap(U);
-        sq = sq2;
-        np_end = d.pstart + d.np;
-        np_sum += d.np;
-        continue;
-      }
-    }
-    mem.push_back({idx});
-    rows.emplace_back(lab, lab + d.nc);
-    sq = (long long)d.nc * d.nc;
-    np_end = d.pstart + d.np;
-    np_sum = d.np;
+  const int npieces = std::max(1, std::min(GROUP_PIECES, P.n_valid));
+  std::vector<GroupList> pieces(npieces);
+  auto piece = [&](int c) {
+    const int a = (int)((long long)P.n_valid * c / npieces), b = (int)((long long)P.n_valid * (c + 1) / npieces);
+    group_range(P, R, order.data() + a, b - a, merge, pieces[c]);
+  };
+  const int nthreads = (int)std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), npieces));
+  {
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nthreads; i++)
+      pool.emplace_back([&, i] {
+        for (int c = i; c < npieces; c += nthreads)
+          piece(c);
+      });
+    for (int c = 0; c < npieces; c += nthreads)
+      piece(c);
+    for (std::thread &th : pool)
+      th.join();
+  }
+  std::vector<std::vector<int>> mem, rows;
+  for (GroupList &g : pieces) {
+    std::move(g.mem.begin(), g.mem.end(), std::back_inserter(mem));
+    std::move(g.rows.begin(), g.rows.end(), std::back_inserter(rows));
   }
   const int ng = (int)mem.size();
   // group order (their index and the layout of their digits): descending work |U|^2 points, ties by

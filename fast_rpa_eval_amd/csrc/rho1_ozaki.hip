@@ -193,12 +193,17 @@ __global__ void p1_rowscale_kernel(const unsigned long long *__restrict__ rowbit
 // Digits of P'(pinv[x], pinv[y]) / ainv[x] (FULL: of P1(pinv[x], pinv[y]) / ainv[x]) at x + n y,
 // digit s in byte s of one 8-byte word (one gather load per element); one 64 x 64 tile of the new
 // labels per block; the transposed tile P1(pinv[y], pinv[x]) goes through LDS so both reads run down
-// columns.  |P1(r, c)| and |P'(r, c)| are both at most bound_r, the row scale.
+// columns.  |P1(r, c)| and |P'(r, c)| are both at most bound_r, the row scale.  The symmetric
+// forward reads P' only at x <= y (a batch's sorted rows ascend in the new labels, and it multiplies
+// the column tiles J >= I), so only the tiles on and above the diagonal are made; a padded column
+// may read a stale word below it, which meets zero phi digits (exact integer products).
 template <int S, bool FULL>
 __global__ __launch_bounds__(256) void p1_digits_kernel(const double *__restrict__ P1, const int *__restrict__ pinv,
                                                         const double *__restrict__ ainv, int n_basis,
                                                         unsigned long long *__restrict__ adig) {
   __shared__ double tr[64][65];
+  if (!FULL && blockIdx.x > blockIdx.y)
+    return;  // below the diagonal: never read by the symmetric forward
   const int x0 = 64 * blockIdx.x, y0 = 64 * blockIdx.y, t = blockIdx.z;
   const size_t nn = (size_t)n_basis * n_basis;
   const double *Pt = P1 + nn * t;

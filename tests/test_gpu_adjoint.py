@@ -62,10 +62,12 @@ def test_h1_adjoint_identity_with_gpu_forward(cuda):
 
 
 @pytest.mark.parametrize("precision,tol", [(api.OZAKI, 1e-10), (api.OZAKI4, 1e-8)])
-@pytest.mark.parametrize("k,args", [(1, (8, 10.0, 64, 100, None)), (3, (40, 16.0, 120, (1, 300), (1, 260)))])
+@pytest.mark.parametrize("k,args", [(1, (8, 10.0, 64, 100, None)), (3, (40, 16.0, 120, (1, 300), (1, 260))),
+                                    (4, (16, 12.0, 24, (330, 700), (50, 200)))])
 def test_h1_int8_split_against_oracle(cuda, oracle, k, args, precision, tol):
     """INT8-split adjoint (per-row digit scales, point index contracted on INT8 MFMA): max|dH| / max|H|
-    within the stated bound of the FP64 oracle; T = 2, batches over one 256-point pass, nc = 0."""
+    within the stated bound of the FP64 oracle; T = 2, batches of one point pass (at most 320 points)
+    and, in the third case, of two and three passes; nc = 0."""
     prob = synth.make_rho1_problem(*args, config_k=k, zero_nc=(2,) if k == 3 else ())
     T = 2
     rng = np.random.default_rng(10 + k)

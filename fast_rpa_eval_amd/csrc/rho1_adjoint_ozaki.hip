@@ -33,6 +33,7 @@
 // phi leave H undefined.
 #include <algorithm>
 #include <cstdlib>
+#include <exception>
 #include <iterator>
 #include <numeric>
 #include <thread>
@@ -732,6 +733,40 @@ double merge_threshold() {
   return e && e[0] ? std::atof(e) : 0.8;
 }
 
+// Runs job(i) for i in [0, n) on up to `threads` host threads, the calling one included (job i on
+// thread i mod the thread count; a thread that cannot be started has its jobs run here); an exception
+// from a job is rethrown here once every thread has joined.
+template <class Job>
+void host_parallel(int n, int threads, const Job &job) {
+  const int nt = std::max(1, std::min(n, threads));
+  std::vector<std::exception_ptr> err(nt);
+  auto body = [&](int t) {
+    try {
+      for (int i = t; i < n; i += nt)
+        job(i);
+    } catch (...) {
+      err[t] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> pool;
+  int started = 1;
+  try {
+    for (; started < nt; started++)
+      pool.emplace_back(body, started);
+  } catch (...) {
+  }
+  for (int t = started; t < nt; t++)
+    body(t);
+  body(0);
+  for (std::thread &th : pool)
+    th.join();
+  for (const std::exception_ptr &e : err)
+    if (e)
+      std::rethrow_exception(e);
+}
+
+int host_threads() { return (int)std::max(1u, std::thread::hardware_concurrency()); }
+
 // The groups of a contiguous piece of the caller's batch order (BatchDesc indices order[0 .. n)):
 // consecutive batches whose points are contiguous join a group while its points fit one pass (PMAX),
 // it has fewer than GMAX members and the union U of the compute sets keeps |U|^2 <= merge * sum nc^2.
@@ -789,19 +824,7 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, h
     const int a = (int)((long long)P.n_valid * c / npieces), b = (int)((long long)P.n_valid * (c + 1) / npieces);
     group_range(P, R, order.data() + a, b - a, merge, pieces[c]);
   };
-  const int nthreads = (int)std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), npieces));
-  {
-    std::vector<std::thread> pool;
-    for (int i = 1; i < nthreads; i++)
-      pool.emplace_back([&, i] {
-        for (int c = i; c < npieces; c += nthreads)
-          piece(c);
-      });
-    for (int c = 0; c < npieces; c += nthreads)
-      piece(c);
-    for (std::thread &th : pool)
-      th.join();
-  }
+  host_parallel(npieces, host_threads(), piece);
   std::vector<std::vector<int>> mem, rows;
   for (GroupList &g : pieces) {
     std::move(g.mem.begin(), g.mem.end(), std::back_inserter(mem));
@@ -874,13 +897,9 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, h
       for (int k = k0; k < k1; k++)
         std::copy(rows[gorder[k]].begin(), rows[gorder[k]].end(), dst + gdesc[k].ib_off);
     };
-    const int nth = (int)std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), GROUP_PIECES));
-    std::vector<std::thread> pool;
-    for (int i = 1; i < nth; i++)
-      pool.emplace_back([&, i] { fill((int)((long long)ng * i / nth), (int)((long long)ng * (i + 1) / nth)); });
-    fill(0, (int)((long long)ng / nth));
-    for (std::thread &th : pool)
-      th.join();
+    host_parallel(GROUP_PIECES, host_threads(), [&](int c) {
+      fill((int)((long long)ng * c / GROUP_PIECES), (int)((long long)ng * (c + 1) / GROUP_PIECES));
+    });
   }
   st.d_gsib.alloc((size_t)std::max<long long>(n_gsib, 1));
   if (n_gsib)

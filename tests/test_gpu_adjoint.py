@@ -91,6 +91,22 @@ def test_h1_int8_split_against_oracle(cuda, oracle, k, args, precision, tol):
     assert np.abs(H2 - ref2).max() <= tol * np.abs(ref2).max()
 
 
+def test_h1_int8_split_more_than_1280_rows(cuda, oracle):
+    """Batches of 1316 to 1444 compute rows (21 to 23 row tiles, 88 to 92 producer units in the
+    first row tile) against the oracle."""
+    prob = synth.make_rho1_problem(110, 22.0, 6, (30, 90), (1290, 1450), config_k=8)
+    assert prob.n_compute_c_batches.min() > 1280
+    rng = np.random.default_rng(29)
+    g = np.asfortranarray(rng.standard_normal((prob.n_full_points, 1)))
+    ref = _oracle_h1(oracle, prob, g, 1)
+    plan = api.Rho1Plan.from_problem(prob)
+    H = plan.adjoint(api.to_device(prob.wave), api.to_device(g), T=1, precision=api.OZAKI).cpu().numpy().reshape(
+        (prob.n_basis, prob.n_basis, 1), order="F")
+    err = np.abs(H - ref).max() / np.abs(ref).max()
+    print(f"[h1 int8 split nc > 1280] rel {err:.3e}", flush=True)
+    assert err <= 1e-10
+
+
 @pytest.mark.parametrize("precision", [api.OZAKI, api.OZAKI4])
 def test_h1_int8_split_is_bit_reproducible(cuda, precision):
     """The INT8-split adjoint accumulates H in 64-bit fixed point with integer atomics, so repeated

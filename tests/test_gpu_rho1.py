@@ -167,6 +167,19 @@ def test_rho1_int8_split_nonsymmetric_multi_tile(cuda, oracle):
     _check(got, ref, rel=1e-10, what="rho1 int8 split, nonsymmetric P1, T = 3")
 
 
+def test_rho1_int8_split_more_than_1280_rows(cuda, oracle):
+    """Batches of 1316 to 1444 compute rows (21 to 23 row tiles): beyond the 1280 rows whose point
+    columns the digit conversion holds in registers, so its looping form makes the digits."""
+    prob = synth.make_rho1_problem(110, 22.0, 6, (30, 90), (1290, 1450), config_k=8)
+    assert prob.n_compute_c_batches.min() > 1280
+    rng = np.random.default_rng(23)
+    A = rng.uniform(-1, 1, size=(prob.n_basis, prob.n_basis))
+    P1 = np.asfortranarray((A + A.T)[:, :, None])
+    ref, _ = oracle.rho1(prob, P1, T=1)
+    got, _ = _run_gpu(prob, P1, 1, 1, precision=api.OZAKI)
+    _check(got, ref, rel=1e-10, what="rho1 int8 split, nc > 1280")
+
+
 @pytest.mark.parametrize("precision,npgs", [(api.FP64, (4, 8, 13)), (api.OZAKI, (4, 2, 1)),
                                             (api.OZAKI4, (4, 5, 2, 1))])
 def test_rho1_launch_configs_bit_identical(cuda, oracle, monkeypatch, precision, npgs):

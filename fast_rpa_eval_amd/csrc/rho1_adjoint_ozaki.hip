@@ -57,7 +57,7 @@ struct GMember {
 static_assert(sizeof(GMember) == 32, "GMember layout");
 
 struct AzItem {
-  int g;       // group (launch-order index), or -1: nothing to do
+  int g;       // group (its index in group order), or -1: nothing to do
   int i0, i1;  // its row tiles I in [i0, i1)
   int pad;
 };
@@ -881,7 +881,7 @@ std::vector<BatchDesc> build_groups(Rho1Plan &P, AdjOzState &st, double merge, h
   st.n_groups = ng;
   st.d_gdesc.upload(gdesc);
   // the groups' rows, in group order, through a pinned host buffer kept by the plan (filled on the
-  // host threads; one DMA copy): 4 bytes per group row, ~0.1 GB on the 60k workload
+  // host threads; one DMA copy): 4 bytes per group row, ~0.1 GB for 60 000 batches of 100-1200 rows
   const size_t gbytes = (size_t)std::max<long long>(n_gsib, 1) * sizeof(int);
   if (st.h_gsib_bytes < gbytes) {
     if (st.h_gsib)

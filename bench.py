@@ -939,7 +939,7 @@ def main():
                 "note": "FP64 symmetric-half rho1 (rho1_precision_comparison.fp64_symmetric) + FP64 adjoint + FP64 "
                         "Horner far field against the fast path, per-kernel times: the precision change alone for "
                         "rho1 and the far field; the FP64 adjoint scatters every batch on its own, the INT8 one "
-                        "merges batch groups (28% fewer H updates on this workload)"}
+                        "merges batch groups (fewer H updates)"}
             log(f"speedup vs FP64 on the same algorithms: {t_d:.1f} / {t_f:.1f} ms = {t_d / t_f:.2f}")
 
     # roofline of every kernel of each path: algorithmic work / measured time against the peak of

@@ -429,9 +429,8 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
   };
   // ASYNC: chunk c's digits straight into buffer buf, 1 KB block k of 64 pieces by wave k mod (4 H);
   // points past npc read the last point's digits (their sums are never used)
-  auto stage_b = [&](int c, int buf) {
+  auto stage_b = [&](int c, signed char *B) {
     constexpr int NBLK = S * PCH / 16, NW = 4 * H;
-    signed char *B = Bs + buf * BUF;
 #pragma unroll
     for (int u = 0; u < (NBLK + NW - 1) / NW; u++) {
       const int blk = NW * u + w;
@@ -497,57 +496,73 @@ __device__ __forceinline__ void ozaki_chunk(signed char *Bs, int *jbs, double *r
       auto col_off = [&](int j) { return j < nc ? nb8 * (unsigned)sib[j] : 0u; };
       const int nsteps = GRAD ? nT : nT - I;
       auto tile_of = [&](int st) { return GRAD ? (I + st) % nT : I + st; };
-      stage_b(I, 2);
-      if (nsteps > 1)
-        stage_b(tile_of(1), 0);
-      if (tid < 64) {
-        jco[128 + tid] = col_off(64 * I + tid);
+      // The staging buffers reach the K steps as distinct restrict pointers, each step's buffers
+      // fixed at compile time (step 0 reads the diagonal chunk from B2, then B0 and B1 alternate,
+      // two steps per iteration), so the compiler can tell that a step's B reads do not touch the
+      // buffer whose direct loads are still in flight and leaves the P' gathers in flight across
+      // the MFMAs (with one dynamically indexed buffer it waits for every load before the first
+      // B read).
+      auto ksteps = [&](signed char *__restrict__ B0, signed char *__restrict__ B1, signed char *__restrict__ B2) {
+        stage_b(I, B2);
         if (nsteps > 1)
-          jco[tid] = col_off(64 * tile_of(1) + tid);
-      }
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      gather(2);
-      if (!GRAD) {
-#pragma unroll
-        for (int q = 0; q < 16; q++)  // P'(i, k) = 0 below the diagonal (k < i)
-          if (64 * I + 16 * lk + q < irow)
-            el[q] = u2{0u, 0u};
-      }
-      repack_a_perm<S>(el, a);
-      for (int st = 0; st < nsteps; st++) {
-        const bool hn = st + 1 < nsteps, hnn = st + 2 < nsteps;
-        const int cur = st == 0 ? 2 : (st - 1) & 1, bn = st & 1, bnn = (st + 1) & 1;
-        // labels of chunk st + 2 (index clamped into the batch; used only if hnn), issued before
-        // the gathers and consumed after the MFMAs: waiting for them leaves the gathers in flight
-        const int jn = 64 * tile_of(st + 2) + lane;
-        const int sj = sib[min(jn, nc - 1)];
-        __builtin_amdgcn_sched_barrier(0);
-        if (hn)
-          gather(bn);  // in flight during this step's MFMAs
-        __builtin_amdgcn_sched_barrier(0);
-        const signed char *B = Bs + cur * BUF;
-#pragma unroll
-        for (int g = 0; g < NG; g++) {
-          v4i b[S];
-#pragma unroll
-          for (int sl = 0; sl < S; sl++)
-            b[sl] = *reinterpret_cast<const v4i *>(&B[(sl * PCH + 16 * (gp0 + g) + lr) * RB + 16 * lk]);
-#pragma unroll
-          for (int sa = 0; sa < S; sa++)
-#pragma unroll
-            for (int sb = 0; sb < S - sa; sb++)
-              acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
+          stage_b(tile_of(1), B0);
+        if (tid < 64) {
+          jco[128 + tid] = col_off(64 * I + tid);
+          if (nsteps > 1)
+            jco[tid] = col_off(64 * tile_of(1) + tid);
         }
-        if (hnn && tid < 64)
-          jco[64 * bnn + tid] = jn < nc ? nb8 * (unsigned)sj : 0u;
-        __builtin_amdgcn_s_waitcnt(0);  // chunk c + 1 staged, its P' gathers arrived
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (hnn)
-          stage_b(tile_of(st + 2), bnn);  // into the buffer this step's chunk was read from
-        if (hn)
-          repack_a_perm<S>(el, a);
-      }
+        gather(2);
+        if (!GRAD) {
+#pragma unroll
+          for (int q = 0; q < 16; q++)  // P'(i, k) = 0 below the diagonal (k < i)
+            if (64 * I + 16 * lk + q < irow)
+              el[q] = u2{0u, 0u};
+        }
+        repack_a_perm<S>(el, a);
+        // step st: phi chunk in B, P' column offsets of step st + 1 in jco slot st & 1; chunk st + 2
+        // is staged into Bnext once every wave has read B
+        auto step = [&](int st, const signed char *B, signed char *Bnext) {
+          const bool hn = st + 1 < nsteps, hnn = st + 2 < nsteps;
+          const int bn = st & 1, bnn = (st + 1) & 1;
+          // labels of chunk st + 2 (index clamped into the batch; used only if hnn), issued before
+          // the gathers and consumed after the MFMAs: waiting for them leaves the gathers in flight
+          const int jn = 64 * tile_of(st + 2) + lane;
+          const int sj = sib[min(jn, nc - 1)];
+          __builtin_amdgcn_sched_barrier(0);
+          if (hn)
+            gather(bn);  // in flight during this step's MFMAs
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int g = 0; g < NG; g++) {
+            v4i b[S];
+#pragma unroll
+            for (int sl = 0; sl < S; sl++)
+              b[sl] = *reinterpret_cast<const v4i *>(&B[(sl * PCH + 16 * (gp0 + g) + lr) * RB + 16 * lk]);
+#pragma unroll
+            for (int sa = 0; sa < S; sa++)
+#pragma unroll
+              for (int sb = 0; sb < S - sa; sb++)
+                acc[sa + sb][g] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb][g], 0, 0, 0);
+          }
+          if (hnn && tid < 64)
+            jco[64 * bnn + tid] = jn < nc ? nb8 * (unsigned)sj : 0u;
+          __builtin_amdgcn_s_waitcnt(0);  // chunk st + 1 staged, its P' gathers arrived
+          __syncthreads();
+          if (hnn)
+            stage_b(tile_of(st + 2), Bnext);
+          if (hn)
+            repack_a_perm<S>(el, a);
+        };
+        step(0, B2, B1);
+        for (int st = 1; st < nsteps; st += 2) {
+          step(st, B0, B0);
+          if (st + 1 < nsteps)
+            step(st + 1, B1, B1);
+        }
+      };
+      ksteps(Bs, Bs + BUF, Bs + 2 * BUF);
     } else {
       load_b(I);
       store_b(I, 2);

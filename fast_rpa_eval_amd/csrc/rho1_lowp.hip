@@ -199,8 +199,7 @@ __device__ __forceinline__ void lowp_chunk(unsigned char *Bs, unsigned *jco, dou
   // 16-byte part q = e & 3 of sub-row sl = e / (4 PCH) of point p; points past npc read the last
   // point's values (their sums are never used); FP64: sub-row sl = rows KC c + 8 sl .. + 7 of the
   // wave block, a piece past nc_pad reads row 0 of the point
-  auto stage_b = [&](int c, int buf) {
-    unsigned char *B = Bs + buf * LP_BUF;
+  auto stage_b = [&](int c, unsigned char *B) {
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int blk = 4 * u + rg;
@@ -297,9 +296,14 @@ __device__ __forceinline__ void lowp_chunk(unsigned char *Bs, unsigned *jco, dou
     // the row tile's K steps: chunks c0 .. nC - 1, c0 the one holding row 64 I
     const int c0 = 64 * I / KC;
     const int nsteps = nC - c0;
-    stage_b(c0, 2);
+    // The staging buffers reach the K steps as distinct restrict pointers, each step's buffers
+    // fixed at compile time (step 0 reads the diagonal chunk from B2, then B0 and B1 alternate, two
+    // steps per iteration), so the compiler can tell that a step's B reads do not touch the buffer
+    // whose direct loads are still in flight and leaves the P' gathers in flight across the MFMAs.
+    auto ksteps = [&](unsigned char *__restrict__ B0, unsigned char *__restrict__ B1, unsigned char *__restrict__ B2) {
+    stage_b(c0, B2);
     if (nsteps > 1)
-      stage_b(c0 + 1, 0);
+      stage_b(c0 + 1, B0);
     if (tid < 64) {
       jco[128 + tid] = col_off(KC * c0 + tid);
       if (nsteps > 1)
@@ -309,10 +313,12 @@ __device__ __forceinline__ void lowp_chunk(unsigned char *Bs, unsigned *jco, dou
     __syncthreads();
     gather(2);
     repack(c0);
-    for (int st = 0; st < nsteps; st++) {
+    // step st: chunk c0 + st in B, P' column offsets of step st + 1 in jco slot st & 1; chunk
+    // c0 + st + 2 is staged into Bnext once every wave has read B
+    auto step = [&](int st, const unsigned char *B, unsigned char *Bnext) {
       const int c = c0 + st;
       const bool hn = st + 1 < nsteps, hnn = st + 2 < nsteps;
-      const int cur = st == 0 ? 2 : (st - 1) & 1, bn = st & 1, bnn = (st + 1) & 1;
+      const int bn = st & 1, bnn = (st + 1) & 1;
       // labels of chunk c + 2 (index clamped into the batch; used only if hnn), loaded before the
       // gathers and consumed after the MFMAs
       const int jn = KC * (c + 2) + lane;
@@ -321,7 +327,6 @@ __device__ __forceinline__ void lowp_chunk(unsigned char *Bs, unsigned *jco, dou
       if (hn)
         gather(bn);  // in flight during this step's MFMAs
       __builtin_amdgcn_sched_barrier(0);
-      const unsigned char *B = Bs + cur * LP_BUF;
 #pragma unroll
       for (int g = 0; g < 4; g++) {
         const int p = 16 * g + lr;
@@ -373,10 +378,18 @@ __device__ __forceinline__ void lowp_chunk(unsigned char *Bs, unsigned *jco, dou
       __builtin_amdgcn_s_waitcnt(0);  // chunk c + 1 staged, its P' gathers arrived
       __syncthreads();
       if (hnn)
-        stage_b(c + 2, bnn);  // into the buffer this step's chunk was read from
+        stage_b(c + 2, Bnext);
       if (hn)
         repack(c + 1);
+    };
+    step(0, B2, B1);
+    for (int st = 1; st < nsteps; st += 2) {
+      step(st, B0, B0);
+      if (st + 1 < nsteps)
+        step(st + 1, B1, B1);
     }
+    };
+    ksteps(Bs, Bs + LP_BUF, Bs + 2 * LP_BUF);
 
     // epilogue: the FP32 / BF16 MFMAs leave D(4 (lane / 16) + r, lane % 16) in acc[r], the FP64 one
     // D(4 r + lane / 16, lane % 16), so accd[g][r] = W(i, p = pc + 16 g + lr) with i = 64 I + 16 rg +
@@ -439,7 +452,7 @@ __device__ __forceinline__ void lowp_chunk(unsigned char *Bs, unsigned *jco, dou
 // sib_all: per batch the P' index of each compute row -- FP32 / BF16x3: its new label (sorted rows),
 // FP64: its 0-based basis index (the caller's rows)
 template <int PREC>
-__global__ __launch_bounds__(256, 2) void rho1_lowp_kernel(
+__global__ __launch_bounds__(256, 3) void rho1_lowp_kernel(
     const BatchDesc *__restrict__ descs, const int *__restrict__ order, int *__restrict__ counters, int n_pos, int T,
     const int *__restrict__ sib_all, const int *__restrict__ map, const unsigned *__restrict__ adat,
     const unsigned char *__restrict__ bdat, const long long *__restrict__ boff, const double *__restrict__ wave,
